@@ -1,0 +1,246 @@
+/*
+ * lantern_gpu.h - C ABI of the MI355X (gfx950) rendering path for Lantern.
+ *
+ * The library replaces Lantern's hot path: Embree's rtcIntersect1 (through
+ * Scene::Intersect, source/core/scene/scene.cpp:116-121), the TBB tile loop of
+ * Integrator::RenderFrame (source/core/integrator/integrator.cpp:28-43) and
+ * the per-pixel estimator RenderPixel (integrator.cpp:98-250), and the
+ * FrameBuffer accumulation (source/core/camera/frame_buffer.h:17-34).
+ *
+ * Conventions
+ *  - Every function returns an int status: 0 on success, a negative LT_E_*
+ *    code on failure. lt_last_error() returns a thread-local message for the
+ *    last failure on the calling thread. No C++ exception crosses the ABI.
+ *  - Handles are opaque. Device buffers are owned by the handles; host
+ *    pointers passed in are borrowed for the duration of the call.
+ *  - Functions whose name ends in _device take device pointers and a HIP
+ *    stream (hipStream_t passed as void*, NULL = the device handle's stream);
+ *    they return once the work is enqueued.
+ *  - One host thread drives one device handle at a time.
+ */
+#ifndef LANTERN_GPU_H
+#define LANTERN_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LT_OK 0
+#define LT_E_INVALID (-1)   /* bad argument or handle */
+#define LT_E_HIP (-2)       /* HIP runtime error */
+#define LT_E_STATE (-3)     /* call not valid in the object's state (e.g. scene not committed) */
+#define LT_E_NOMEM (-4)     /* allocation failed */
+#define LT_E_IO (-5)        /* file could not be read / parsed */
+
+/* RTC_INVALID_GEOMETRY_ID (embree3/rtcore_common.h:66): the miss marker. */
+#define LT_INVALID_ID 0xFFFFFFFFu
+
+typedef struct lt_device lt_device;
+typedef struct lt_scene lt_scene;
+typedef struct lt_film lt_film;
+
+/* Thread-local message of the last failure (replaces rtcGetDeviceError,
+ * embree3/rtcore_device.h:85). Never NULL. */
+const char *lt_last_error(void);
+
+/* ---- device --------------------------------------------------------------
+ * rtcNewDevice / rtcReleaseDevice (scene.cpp:39, 46). */
+int lt_device_create(int ordinal, lt_device **out);
+int lt_device_destroy(lt_device *dev);
+/* hipStream_t the device handle launches on. */
+int lt_device_stream(lt_device *dev, void **out_stream);
+int lt_device_synchronize(lt_device *dev);
+
+/* ---- scene ---------------------------------------------------------------- */
+enum { LT_BSDF_LAMBERT = 0, LT_BSDF_MIRROR = 1, LT_BSDF_DIELECTRIC = 2, LT_BSDF_GGX = 3 };
+enum { LT_MEDIUM_NON_SCATTERING = 0, LT_MEDIUM_ISOTROPIC = 1, LT_MEDIUM_GRID = 2 };
+enum { LT_TEX_CONSTANT = -1, LT_TEX_UV = -2 };
+
+/* Material = BSDF + optional medium (materials/material.h:14-26, the bsdf
+ * entries of scene.cpp:261-316). albedo is used when texture == LT_TEX_CONSTANT
+ * (ConstantTexture); LT_TEX_UV is UVTexture; a texture id is an FP32 RGB image
+ * sampled like ImageTexture (image_texture.h:32-39, image_cache.cpp:42-56).
+ * LT_BSDF_GGX (GGX conductor, roughness = alpha) has no reference counterpart. */
+typedef struct lt_material {
+    int32_t bsdf;
+    float albedo[3];
+    int32_t texture;
+    float ior;
+    float roughness;
+    int32_t medium;      /* -1: none */
+} lt_material;
+
+/* Media (medium.h:17-36, non_scattering_medium.h, isotropic_scattering_medium.h;
+ * scene.cpp:318-339). LT_MEDIUM_GRID (heterogeneous FP32 density grid, delta
+ * tracking, Henyey-Greenstein phase) has no reference counterpart. */
+typedef struct lt_medium {
+    int32_t type;
+    float absorption_color[3];
+    float absorption_at_distance;
+    float scattering_distance;
+    int32_t grid;
+    float sigma_t_scale;
+    float albedo;
+    float hg_g;
+} lt_medium;
+
+typedef struct lt_build_params {
+    uint32_t max_leaf_size;   /* 1..8, default 4 (Embree sahBlockSize/minLeaf 4) */
+    uint32_t reserved[7];
+} lt_build_params;
+
+typedef struct lt_scene_info {
+    uint32_t num_geometries;
+    uint32_t num_triangles;
+    uint32_t num_nodes;
+    uint32_t num_leaves;
+    uint32_t max_depth;
+    uint32_t num_lights;
+    float sah_cost;
+    float build_ms;
+    uint64_t device_bytes;
+} lt_scene_info;
+
+/* rtcNewScene + rtcSetSceneFlags/BuildQuality (scene.cpp:196-198). */
+int lt_scene_create(lt_device *dev, lt_scene **out);
+int lt_scene_destroy(lt_scene *scene);
+/* rtcNewGeometry(TRIANGLE|QUAD) + rtcSetNewGeometryBuffer(VERTEX, INDEX,
+ * VERTEX_ATTRIBUTE slot 0 normals float3, slot 1 uvs float2) + rtcAttachGeometry
+ * (scene.cpp:451-582). Positions are already transformed (AddMesh applies the
+ * JSON transform on the host, scene.cpp:456-461). verts_per_prim is 3 or 4;
+ * quads are split into (0,1,2) and (0,2,3) triangles sharing one primID
+ * numbering of 2*q and 2*q+1. geomIDs are sequential from 0, as rtcAttachGeometry
+ * assigns them. normals/uvs may be NULL (hasNormals/hasTexCoords false). */
+int lt_scene_add_mesh(lt_scene *scene, const float *positions, size_t position_stride_bytes,
+                      uint32_t num_vertices, const uint32_t *indices, uint32_t num_prims,
+                      uint32_t verts_per_prim, const float *normals, const float *uvs,
+                      uint32_t *out_geom_id);
+int lt_scene_add_material(lt_scene *scene, const lt_material *material, uint32_t *out_id);
+int lt_scene_add_medium(lt_scene *scene, const lt_medium *medium, uint32_t *out_id);
+/* FP32 RGB texture, row-major, w*h*3 floats. */
+int lt_scene_add_texture(lt_scene *scene, uint32_t w, uint32_t h, const float *rgb, uint32_t *out_id);
+/* FP32 density grid, x fastest, over the world box [bmin, bmax]. */
+int lt_scene_add_grid(lt_scene *scene, uint32_t nx, uint32_t ny, uint32_t nz, const float *density,
+                      const float bmin[3], const float bmax[3], uint32_t *out_id);
+int lt_scene_set_material(lt_scene *scene, uint32_t geom_id, uint32_t material_id);
+/* AreaLight(color, radiantPower, area, geomId, boundingSphere) (area_light.h:16-21,
+ * created for "emission" primitives at scene.cpp:437-444). Area and bounding
+ * sphere are computed from the geometry as AddMesh does (scene.cpp:464-473). */
+int lt_scene_set_emission(lt_scene *scene, uint32_t geom_id, const float color[3], float radiant_power);
+/* Scene::BackgroundColor (scene.h:40; scene.cpp:215-219). */
+int lt_scene_set_background(lt_scene *scene, const float rgb[3]);
+/* rtcCommitScene (scene.cpp:86): uploads the scene and builds the BVH on the GPU. */
+int lt_scene_commit(lt_scene *scene, const lt_build_params *params);
+int lt_scene_get_info(lt_scene *scene, lt_scene_info *out);
+/* Light table as rendered: per light Le[3], area, sphere[4], geomID (9 floats). */
+int lt_scene_get_lights(lt_scene *scene, float *out, uint32_t capacity, uint32_t *out_count);
+/* Maps global primitive indices (as returned in lt_hit.prim) to (geomID, primID). */
+int lt_scene_split_prims(lt_scene *scene, const uint32_t *global_prims, uint32_t n, uint32_t *geom_ids,
+                         uint32_t *prim_ids);
+
+/* ---- ray queries -----------------------------------------------------------
+ * RTCRay/RTCHit reduced to what the hot path reads (embree3/rtcore_ray.h:26-64):
+ * 32 B ray, 16 B hit. */
+typedef struct lt_ray {
+    float org[3];
+    float tnear;
+    float dir[3];
+    float tfar;
+} lt_ray;
+typedef struct lt_hit {
+    float t, u, v;
+    uint32_t prim;   /* global primitive index; LT_INVALID_ID on a miss */
+} lt_hit;
+
+/* rtcIntersect1 semantics (rtcore.cpp:300-317; Moeller-Trumbore as in
+ * triangle_intersector_moeller.h:75-113): a hit needs tnear < t <= tfar
+ * (strict/inclusive), u,v >= 0, u+v <= 1; on a miss prim = LT_INVALID_ID and
+ * t = tfar. Device pointers, n rays, enqueued on stream. */
+int lt_intersect_closest_device(lt_scene *scene, const lt_ray *rays, lt_hit *hits, uint32_t n, void *stream);
+/* rtcOccluded1 analogue: occluded[i] = 1 if any hit in (tnear, tfar]. */
+int lt_intersect_any_device(lt_scene *scene, const lt_ray *rays, uint32_t *occluded, uint32_t n, void *stream);
+/* Host-pointer conveniences: copy in, trace, copy out, synchronize. */
+int lt_intersect_closest(lt_scene *scene, const lt_ray *rays, lt_hit *hits, uint32_t n);
+int lt_intersect_any(lt_scene *scene, const lt_ray *rays, uint32_t *occluded, uint32_t n);
+
+/* ---- film ------------------------------------------------------------------
+ * FrameBuffer(width, height) (frame_buffer.h:17-34): per pixel an RGB sum
+ * (ColorData), a bounce sum (Bounces) and a sample count (ColorSampleCount). */
+int lt_film_create(lt_device *dev, uint32_t width, uint32_t height, lt_film **out);
+int lt_film_destroy(lt_film *film);
+/* FrameBuffer::Reset (frame_buffer.cpp:28-34). */
+int lt_film_reset(lt_film *film);
+/* rgb_sum: w*h*3 floats; bounces, counts: w*h u32. Any pointer may be NULL. */
+int lt_film_download(lt_film *film, float *rgb_sum, uint32_t *bounces, uint32_t *counts);
+/* Restores an accumulation (resume from a saved (sum, bounces, count)). */
+int lt_film_upload(lt_film *film, const float *rgb_sum, const uint32_t *bounces, const uint32_t *counts);
+/* Resolved image sum/count (visualizer.cpp:355-370), w*h*3 floats. */
+int lt_film_resolve(lt_film *film, float *rgb);
+
+/* ---- rendering ------------------------------------------------------------ */
+/* PinholeCamera(phi, theta, radius, clientWidth, clientHeight, target, fov,
+ * filterType) (pinhole_camera.h:25; filter 0 tent, 1 dirac, 2 box). */
+typedef struct lt_camera {
+    float phi, theta, radius;
+    uint32_t width, height;
+    float target[3];
+    float fov;
+    int32_t filter;
+} lt_camera;
+
+typedef struct lt_render_params {
+    uint32_t spp;          /* frames: one sample per pixel each (RenderFrame calls) */
+    uint32_t max_depth;    /* bounce limit; the reference's is 1500 (integrator.cpp:114) */
+    uint32_t rr_start;     /* RR when bounces >= rr_start; the reference's is 4 (`> 3`, integrator.cpp:235) */
+    uint32_t seed;         /* counter-based RNG key */
+    uint32_t frame0;       /* frame number of the first sample (Integrator::m_frameNumber) */
+    uint32_t stream;       /* RNG stream id (scene index in a batch) */
+    uint32_t wave_paths;   /* paths in flight per wave; 0 = automatic */
+    uint32_t block_size;   /* traversal block size (64..1024); 0 = automatic */
+    int32_t use_graph;     /* 1: capture the bounce loop in a hipGraph and replay it */
+    uint32_t reserved[7];
+} lt_render_params;
+
+typedef struct lt_render_stats {
+    uint64_t samples;        /* camera paths traced */
+    uint64_t closest_rays;   /* extension rays traced (closest hit) */
+    uint64_t shadow_rays;    /* light-sample and BSDF-sample rays of next-event estimation */
+    uint64_t path_segments;  /* shade-kernel invocations */
+    double total_ms;         /* wall time of lt_render on the device stream */
+    double extend_ms;        /* kernel time (HIP events), summed */
+    double shade_ms;
+    double shadow_ms;
+    double resolve_ms;
+    double generate_ms;
+    double accumulate_ms;
+    uint32_t extend_launches;
+    uint32_t shadow_launches;
+    uint32_t waves;
+    uint32_t reserved;
+} lt_render_stats;
+
+/* spp x Integrator::RenderFrame (integrator.cpp:28-43) into film; the film
+ * must be width x height of the camera. stats may be NULL. Synchronous. */
+int lt_render(lt_scene *scene, const lt_camera *camera, const lt_render_params *params, lt_film *film,
+              lt_render_stats *stats);
+/* Many scenes with their own cameras and films, one launch sequence. */
+int lt_render_batch(lt_scene *const *scenes, const lt_camera *cameras, lt_film *const *films, uint32_t n,
+                    const lt_render_params *params, lt_render_stats *stats);
+
+/* Host-side generators used by the scene loader (geometry_generator.cpp).
+ * Pass NULL buffers to query sizes: returns the vertex count, *num_tris gets
+ * the triangle count. Positions/normals nv*3 floats, uvs nv*2. */
+uint32_t lt_geometry_geosphere(float radius, uint32_t subdivisions, float *pos, float *nrm, uint32_t *idx,
+                               uint32_t *num_tris);
+uint32_t lt_geometry_grid(float width, float depth, uint32_t m, uint32_t n, float *pos, float *nrm, float *uv,
+                          uint32_t *idx, uint32_t *num_tris);
+uint32_t lt_geometry_box(float width, float height, float depth, float *pos, float *nrm, float *uv,
+                         uint32_t *idx, uint32_t *num_tris);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,209 @@
+"""lantern_amd - MI355X (gfx950) rendering path for Lantern.
+
+Python mirror of the host surface (Scene / PinholeCamera / Integrator /
+FrameBuffer) over the C ABI in include/lantern_gpu.h. Every call goes through
+liblantern_gpu.so; there is no CPU fallback.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+from ._capi import (LanternError, LT_BSDF_DIELECTRIC, LT_BSDF_GGX, LT_BSDF_LAMBERT, LT_BSDF_MIRROR,
+                    LT_INVALID_ID, LT_MEDIUM_GRID, LT_MEDIUM_ISOTROPIC, LT_MEDIUM_NON_SCATTERING, LT_TEX_CONSTANT,
+                    LT_TEX_UV, check, lib)
+from .scenes import SceneDesc
+
+__all__ = ["Device", "Scene", "Film", "render", "SceneDesc", "LanternError", "camera_struct", "params_struct",
+           "LT_INVALID_ID"]
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+
+
+def _up(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32)) if a is not None else None
+
+
+class Device:
+    """lt_device: one GPU and its stream (rtcNewDevice, scene.cpp:39)."""
+
+    def __init__(self, ordinal=0):
+        self.h = C.c_void_p()
+        check(lib().lt_device_create(ordinal, C.byref(self.h)), "lt_device_create")
+
+    def close(self):
+        if self.h:
+            lib().lt_device_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def synchronize(self):
+        check(lib().lt_device_synchronize(self.h), "lt_device_synchronize")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Scene:
+    """lt_scene built from a SceneDesc (Scene::LoadSceneFromJSON + rtcCommitScene)."""
+
+    def __init__(self, device, desc=None, max_leaf_size=4):
+        self.dev = device
+        self.h = C.c_void_p()
+        check(lib().lt_scene_create(device.h, C.byref(self.h)), "lt_scene_create")
+        if desc is not None:
+            self.load(desc)
+            self.commit(max_leaf_size)
+
+    def load(self, d):
+        L = lib()
+        for i, m in enumerate(d.materials):
+            mat = _capi.lt_material(m["bsdf"], (C.c_float * 3)(*m.get("albedo", (0.0, 0.0, 0.0))),
+                                    m.get("texture", LT_TEX_CONSTANT), m.get("ior", 1.0), m.get("roughness", 0.0),
+                                    m.get("medium", -1))
+            out = C.c_uint32()
+            check(L.lt_scene_add_material(self.h, C.byref(mat), C.byref(out)), "lt_scene_add_material")
+        for t in d.textures:
+            t = np.ascontiguousarray(t, dtype=np.float32)
+            out = C.c_uint32()
+            check(L.lt_scene_add_texture(self.h, t.shape[1], t.shape[0], _fp(t), C.byref(out)), "lt_scene_add_texture")
+        for g in d.grids:
+            dens = np.ascontiguousarray(g["density"], dtype=np.float32)
+            nz, ny, nx = dens.shape
+            bmin = np.asarray(g["bmin"], np.float32)
+            bmax = np.asarray(g["bmax"], np.float32)
+            out = C.c_uint32()
+            check(L.lt_scene_add_grid(self.h, nx, ny, nz, _fp(dens), _fp(bmin), _fp(bmax), C.byref(out)),
+                  "lt_scene_add_grid")
+        for m in d.media:
+            md = _capi.lt_medium(m["type"], (C.c_float * 3)(*m.get("absorption_color", (1.0, 1.0, 1.0))),
+                                 m.get("absorption_at_distance", 1.0), m.get("scattering_distance", 1.0),
+                                 m.get("grid", -1), m.get("sigma_t_scale", 0.0), m.get("albedo", 1.0),
+                                 m.get("hg_g", 0.0))
+            out = C.c_uint32()
+            check(L.lt_scene_add_medium(self.h, C.byref(md), C.byref(out)), "lt_scene_add_medium")
+        for mesh in d.meshes:
+            pos = np.ascontiguousarray(mesh["pos"], dtype=np.float32)
+            idx = np.ascontiguousarray(mesh["idx"], dtype=np.uint32)
+            nrm = mesh.get("nrm")
+            uv = mesh.get("uv")
+            nrm = np.ascontiguousarray(nrm, dtype=np.float32) if nrm is not None else None
+            uv = np.ascontiguousarray(uv, dtype=np.float32) if uv is not None else None
+            gid = C.c_uint32()
+            check(L.lt_scene_add_mesh(self.h, _fp(pos), 12, pos.shape[0], _up(idx), idx.shape[0], idx.shape[1],
+                                      _fp(nrm), _fp(uv), C.byref(gid)), "lt_scene_add_mesh")
+            check(L.lt_scene_set_material(self.h, gid.value, mesh["material"]), "lt_scene_set_material")
+            if mesh.get("emission") is not None:
+                col, power = mesh["emission"]
+                check(L.lt_scene_set_emission(self.h, gid.value, (C.c_float * 3)(*col), power),
+                      "lt_scene_set_emission")
+        check(L.lt_scene_set_background(self.h, (C.c_float * 3)(*d.background)), "lt_scene_set_background")
+
+    def commit(self, max_leaf_size=4):
+        bp = _capi.lt_build_params()
+        bp.max_leaf_size = max_leaf_size
+        check(lib().lt_scene_commit(self.h, C.byref(bp)), "lt_scene_commit")
+
+    def info(self):
+        i = _capi.lt_scene_info()
+        check(lib().lt_scene_get_info(self.h, C.byref(i)), "lt_scene_get_info")
+        return {k: getattr(i, k) for k, _ in i._fields_}
+
+    def lights(self):
+        n = C.c_uint32()
+        check(lib().lt_scene_get_lights(self.h, None, 0, C.byref(n)), "lt_scene_get_lights")
+        out = np.zeros((max(n.value, 1), 9), np.float32)
+        check(lib().lt_scene_get_lights(self.h, _fp(out), n.value, C.byref(n)), "lt_scene_get_lights")
+        return out[: n.value]
+
+    def intersect(self, rays, any_hit=False):
+        """rays: (n, 8) float32 [org xyz, tnear, dir xyz, tfar]. Returns
+        (tuv (n,3) float32, prim (n,) uint32) for closest hit, or occluded (n,) uint32."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        n = rays.shape[0]
+        if any_hit:
+            out = np.zeros(n, np.uint32)
+            check(lib().lt_intersect_any(self.h, rays.ctypes.data, out.ctypes.data, n), "lt_intersect_any")
+            return out
+        hits = np.zeros((n, 4), np.float32)
+        check(lib().lt_intersect_closest(self.h, rays.ctypes.data, hits.ctypes.data, n), "lt_intersect_closest")
+        return hits[:, :3].copy(), hits[:, 3].view(np.uint32).copy()
+
+    def split_prims(self, prims):
+        prims = np.ascontiguousarray(prims, dtype=np.uint32)
+        g = np.zeros_like(prims)
+        p = np.zeros_like(prims)
+        check(lib().lt_scene_split_prims(self.h, _up(prims), prims.size, _up(g), _up(p)), "lt_scene_split_prims")
+        return g, p
+
+    def close(self):
+        if self.h:
+            lib().lt_scene_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+class Film:
+    """lt_film: the FrameBuffer (frame_buffer.h:17-34) in device memory."""
+
+    def __init__(self, device, width, height):
+        self.w, self.h_ = width, height
+        self.h = C.c_void_p()
+        check(lib().lt_film_create(device.h, width, height, C.byref(self.h)), "lt_film_create")
+
+    def reset(self):
+        check(lib().lt_film_reset(self.h), "lt_film_reset")
+
+    def download(self):
+        s = np.zeros((self.h_, self.w, 3), np.float32)
+        b = np.zeros((self.h_, self.w), np.uint32)
+        c = np.zeros((self.h_, self.w), np.uint32)
+        check(lib().lt_film_download(self.h, _fp(s), _up(b), _up(c)), "lt_film_download")
+        return s, b, c
+
+    def upload(self, s, b, c):
+        s = np.ascontiguousarray(s, np.float32)
+        b = np.ascontiguousarray(b, np.uint32)
+        c = np.ascontiguousarray(c, np.uint32)
+        check(lib().lt_film_upload(self.h, _fp(s), _up(b), _up(c)), "lt_film_upload")
+
+    def resolve(self):
+        out = np.zeros((self.h_, self.w, 3), np.float32)
+        check(lib().lt_film_resolve(self.h, _fp(out)), "lt_film_resolve")
+        return out
+
+    def close(self):
+        if self.h:
+            lib().lt_film_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+def camera_struct(cam):
+    return _capi.lt_camera(cam["phi"], cam["theta"], cam["radius"], cam["width"], cam["height"],
+                           (C.c_float * 3)(*cam.get("target", (0.0, 0.0, 0.0))), cam["fov"], cam.get("filter", 0))
+
+
+def params_struct(p):
+    r = _capi.lt_render_params()
+    r.spp = p["spp"]
+    r.max_depth = p.get("max_depth", 1500)
+    r.rr_start = p.get("rr_start", 4)
+    r.seed = p.get("seed", 0x4C414E54)
+    r.frame0 = p.get("frame0", 0)
+    r.stream = p.get("stream", 0)
+    r.wave_paths = p.get("wave_paths", 0)
+    r.block_size = p.get("block_size", 0)
+    r.use_graph = p.get("use_graph", 0)
+    return r
+
+
+def render(scene, camera, params, film):
+    """spp x Integrator::RenderFrame into film. Returns the lt_render_stats as a dict."""
+    st = _capi.lt_render_stats()
+    cam = camera_struct(camera)
+    prm = params_struct(params)
+    check(lib().lt_render(scene.h, C.byref(cam), C.byref(prm), film.h, C.byref(st)), "lt_render")
+    return {k: getattr(st, k) for k, _ in st._fields_}

@@ -1,0 +1,143 @@
+"""ctypes binding of liblantern_gpu.so (declarations in include/lantern_gpu.h).
+
+The library is loaded from this package directory only. There is no fallback:
+if the shared object is missing the import raises, so nothing silently runs a
+different implementation.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblantern_gpu.so")
+
+LT_OK = 0
+LT_INVALID_ID = 0xFFFFFFFF
+
+LT_BSDF_LAMBERT, LT_BSDF_MIRROR, LT_BSDF_DIELECTRIC, LT_BSDF_GGX = 0, 1, 2, 3
+LT_MEDIUM_NON_SCATTERING, LT_MEDIUM_ISOTROPIC, LT_MEDIUM_GRID = 0, 1, 2
+LT_TEX_CONSTANT, LT_TEX_UV = -1, -2
+
+
+class lt_material(C.Structure):
+    _fields_ = [("bsdf", C.c_int32), ("albedo", C.c_float * 3), ("texture", C.c_int32), ("ior", C.c_float),
+                ("roughness", C.c_float), ("medium", C.c_int32)]
+
+
+class lt_medium(C.Structure):
+    _fields_ = [("type", C.c_int32), ("absorption_color", C.c_float * 3), ("absorption_at_distance", C.c_float),
+                ("scattering_distance", C.c_float), ("grid", C.c_int32), ("sigma_t_scale", C.c_float),
+                ("albedo", C.c_float), ("hg_g", C.c_float)]
+
+
+class lt_build_params(C.Structure):
+    _fields_ = [("max_leaf_size", C.c_uint32), ("reserved", C.c_uint32 * 7)]
+
+
+class lt_scene_info(C.Structure):
+    _fields_ = [("num_geometries", C.c_uint32), ("num_triangles", C.c_uint32), ("num_nodes", C.c_uint32),
+                ("num_leaves", C.c_uint32), ("max_depth", C.c_uint32), ("num_lights", C.c_uint32),
+                ("sah_cost", C.c_float), ("build_ms", C.c_float), ("device_bytes", C.c_uint64)]
+
+
+class lt_ray(C.Structure):
+    _fields_ = [("org", C.c_float * 3), ("tnear", C.c_float), ("dir", C.c_float * 3), ("tfar", C.c_float)]
+
+
+class lt_hit(C.Structure):
+    _fields_ = [("t", C.c_float), ("u", C.c_float), ("v", C.c_float), ("prim", C.c_uint32)]
+
+
+class lt_camera(C.Structure):
+    _fields_ = [("phi", C.c_float), ("theta", C.c_float), ("radius", C.c_float), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("target", C.c_float * 3), ("fov", C.c_float), ("filter", C.c_int32)]
+
+
+class lt_render_params(C.Structure):
+    _fields_ = [("spp", C.c_uint32), ("max_depth", C.c_uint32), ("rr_start", C.c_uint32), ("seed", C.c_uint32),
+                ("frame0", C.c_uint32), ("stream", C.c_uint32), ("wave_paths", C.c_uint32),
+                ("block_size", C.c_uint32), ("use_graph", C.c_int32), ("reserved", C.c_uint32 * 7)]
+
+
+class lt_render_stats(C.Structure):
+    _fields_ = [("samples", C.c_uint64), ("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("path_segments", C.c_uint64), ("total_ms", C.c_double), ("extend_ms", C.c_double),
+                ("shade_ms", C.c_double), ("shadow_ms", C.c_double), ("resolve_ms", C.c_double),
+                ("generate_ms", C.c_double), ("accumulate_ms", C.c_double), ("extend_launches", C.c_uint32),
+                ("shadow_launches", C.c_uint32), ("waves", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+_vp = C.c_void_p
+_u32 = C.c_uint32
+_i32 = C.c_int32
+_f = C.c_float
+_pf = C.POINTER(C.c_float)
+_pu32 = C.POINTER(C.c_uint32)
+
+# name -> (restype, argtypes); every symbol declared in include/lantern_gpu.h
+SIGNATURES = {
+    "lt_last_error": (C.c_char_p, []),
+    "lt_device_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "lt_device_destroy": (C.c_int, [_vp]),
+    "lt_device_stream": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "lt_device_synchronize": (C.c_int, [_vp]),
+    "lt_scene_create": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "lt_scene_destroy": (C.c_int, [_vp]),
+    "lt_scene_add_mesh": (C.c_int, [_vp, _pf, C.c_size_t, _u32, _pu32, _u32, _u32, _pf, _pf, _pu32]),
+    "lt_scene_add_material": (C.c_int, [_vp, C.POINTER(lt_material), _pu32]),
+    "lt_scene_add_medium": (C.c_int, [_vp, C.POINTER(lt_medium), _pu32]),
+    "lt_scene_add_texture": (C.c_int, [_vp, _u32, _u32, _pf, _pu32]),
+    "lt_scene_add_grid": (C.c_int, [_vp, _u32, _u32, _u32, _pf, _pf, _pf, _pu32]),
+    "lt_scene_set_material": (C.c_int, [_vp, _u32, _u32]),
+    "lt_scene_set_emission": (C.c_int, [_vp, _u32, _pf, _f]),
+    "lt_scene_set_background": (C.c_int, [_vp, _pf]),
+    "lt_scene_commit": (C.c_int, [_vp, C.POINTER(lt_build_params)]),
+    "lt_scene_get_info": (C.c_int, [_vp, C.POINTER(lt_scene_info)]),
+    "lt_scene_get_lights": (C.c_int, [_vp, _pf, _u32, _pu32]),
+    "lt_scene_split_prims": (C.c_int, [_vp, _pu32, _u32, _pu32, _pu32]),
+    "lt_intersect_closest_device": (C.c_int, [_vp, _vp, _vp, _u32, _vp]),
+    "lt_intersect_any_device": (C.c_int, [_vp, _vp, _vp, _u32, _vp]),
+    "lt_intersect_closest": (C.c_int, [_vp, _vp, _vp, _u32]),
+    "lt_intersect_any": (C.c_int, [_vp, _vp, _vp, _u32]),
+    "lt_film_create": (C.c_int, [_vp, _u32, _u32, C.POINTER(_vp)]),
+    "lt_film_destroy": (C.c_int, [_vp]),
+    "lt_film_reset": (C.c_int, [_vp]),
+    "lt_film_download": (C.c_int, [_vp, _pf, _pu32, _pu32]),
+    "lt_film_upload": (C.c_int, [_vp, _pf, _pu32, _pu32]),
+    "lt_film_resolve": (C.c_int, [_vp, _pf]),
+    "lt_render": (C.c_int, [_vp, C.POINTER(lt_camera), C.POINTER(lt_render_params), _vp,
+                            C.POINTER(lt_render_stats)]),
+    "lt_render_batch": (C.c_int, [C.POINTER(_vp), C.POINTER(lt_camera), C.POINTER(_vp), _u32,
+                                  C.POINTER(lt_render_params), C.POINTER(lt_render_stats)]),
+    "lt_geometry_geosphere": (_u32, [_f, _u32, _pf, _pf, _pu32, _pu32]),
+    "lt_geometry_grid": (_u32, [_f, _f, _u32, _u32, _pf, _pf, _pf, _pu32, _pu32]),
+    "lt_geometry_box": (_u32, [_f, _f, _f, _pf, _pf, _pf, _pu32, _pu32]),
+}
+
+
+class LanternError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Loads liblantern_gpu.so once. Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LanternError(f"{LIB_PATH} is missing: run __graft_entry__.build() (make -C lantern_amd)")
+        handle = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != LT_OK:
+        msg = lib().lt_last_error().decode(errors="replace")
+        raise LanternError(f"{what} failed ({rc}): {msg}")
+    return rc

@@ -1,0 +1,727 @@
+/*
+ * lt_api.hip - the C ABI of liblantern_gpu.so (declared in include/lantern_gpu.h).
+ */
+#include "lt_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+namespace lt {
+
+int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &prm, lt_film *film,
+                 lt_render_stats *stats);
+
+namespace {
+thread_local std::string g_error = "no error";
+}
+
+int set_error(int code, const std::string &msg) {
+    g_error = msg;
+    return code;
+}
+int hip_error(hipError_t e, const char *what, const char *file, int line) {
+    char buf[512];
+    std::snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+    g_error = buf;
+    return LT_E_HIP;
+}
+
+int DevBuf::alloc(size_t n) {
+    release();
+    if (n == 0) n = 16;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+        p = nullptr;
+        char buf[128];
+        std::snprintf(buf, sizeof buf, "hipMalloc of %zu bytes failed: %s", n, hipGetErrorString(e));
+        return set_error(LT_E_NOMEM, buf);
+    }
+    bytes = n;
+    return LT_OK;
+}
+int DevBuf::upload(const void *src, size_t n, hipStream_t st) {
+    int rc = reserve(n);
+    if (rc) return rc;
+    if (n) LT_HIP(hipMemcpyAsync(p, src, n, hipMemcpyHostToDevice, st));
+    return LT_OK;
+}
+
+/* ---- host geometry (geometry_generator.cpp, scene.cpp) ------------------ */
+namespace {
+
+struct HostMesh {
+    std::vector<V3> pos, nrm;
+    std::vector<float> uv;
+    std::vector<uint32_t> idx;
+};
+
+/* Subdivide (geometry_generator.cpp:214-282) */
+static void subdivide(HostMesh &m) {
+    HostMesh in = m;
+    m.pos.clear();
+    m.idx.clear();
+    size_t nt = in.idx.size() / 3;
+    m.pos.reserve(nt * 6);
+    m.idx.reserve(nt * 12);
+    for (size_t i = 0; i < nt; ++i) {
+        V3 a = in.pos[in.idx[3 * i]], b = in.pos[in.idx[3 * i + 1]], c = in.pos[in.idx[3 * i + 2]];
+        V3 m0 = v3(0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z));
+        V3 m1 = v3(0.5f * (b.x + c.x), 0.5f * (b.y + c.y), 0.5f * (b.z + c.z));
+        V3 m2 = v3(0.5f * (a.x + c.x), 0.5f * (a.y + c.y), 0.5f * (a.z + c.z));
+        V3 v[6] = {a, b, c, m0, m1, m2};
+        m.pos.insert(m.pos.end(), v, v + 6);
+        uint32_t o = (uint32_t)(6 * i);
+        uint32_t t[12] = {o + 0, o + 3, o + 5, o + 3, o + 4, o + 5, o + 5, o + 4, o + 2, o + 3, o + 1, o + 4};
+        m.idx.insert(m.idx.end(), t, t + 12);
+    }
+}
+
+/* CreateGeosphere (geometry_generator.cpp:306-356) */
+static HostMesh make_geosphere(float radius, uint32_t n) {
+    const float X = 0.525731f, Z = 0.850651f;
+    const V3 p[12] = {{-X, 0, -Z}, {X, 0, -Z}, {-X, 0, Z}, {X, 0, Z}, {0, Z, -X}, {0, Z, X},
+                      {0, -Z, -X}, {0, -Z, X}, {Z, X, 0}, {-Z, X, 0}, {Z, -X, 0}, {-Z, -X, 0}};
+    const uint32_t k[60] = {1, 4, 0, 4, 9, 0, 4, 5, 9, 8, 5, 4, 1, 8, 4, 1, 10, 8, 10, 3, 8, 8, 3, 5, 3, 2, 5, 3, 7, 2,
+                            3, 10, 7, 10, 6, 7, 6, 11, 7, 6, 0, 11, 6, 1, 0, 10, 1, 6, 11, 0, 9, 2, 11, 9, 5, 2, 9, 11, 2, 7};
+    HostMesh m;
+    m.pos.assign(p, p + 12);
+    m.idx.assign(k, k + 60);
+    for (uint32_t i = 0; i < n; ++i) subdivide(m);
+    m.nrm.resize(m.pos.size());
+    for (size_t i = 0; i < m.pos.size(); ++i) {
+        V3 nn = normalize(m.pos[i]);
+        m.pos[i] = radius * nn;
+        m.nrm[i] = nn;
+    }
+    return m;
+}
+
+/* CreateGrid (geometry_generator.cpp:515-566) */
+static HostMesh make_grid(float width, float depth, uint32_t mm, uint32_t nn) {
+    HostMesh m;
+    float hw = 0.5f * width, hd = 0.5f * depth;
+    float dx = width / (float)(nn - 1), dz = depth / (float)(mm - 1);
+    float du = 1.0f / (float)(nn - 1), dv = 1.0f / (float)(mm - 1);
+    m.pos.resize((size_t)mm * nn);
+    m.nrm.resize((size_t)mm * nn);
+    m.uv.resize((size_t)mm * nn * 2);
+    for (uint32_t i = 0; i < mm; ++i) {
+        float z = hd - (float)i * dz;
+        for (uint32_t j = 0; j < nn; ++j) {
+            float x = -hw + (float)j * dx;
+            m.pos[i * nn + j] = v3(x, 0.0f, -z);
+            m.nrm[i * nn + j] = v3(0.0f, 1.0f, 0.0f);
+            m.uv[(i * nn + j) * 2] = (float)j * du;
+            m.uv[(i * nn + j) * 2 + 1] = (float)i * dv;
+        }
+    }
+    for (uint32_t i = 0; i + 1 < mm; ++i)
+        for (uint32_t j = 0; j + 1 < nn; ++j) {
+            uint32_t t[6] = {i * nn + j, i * nn + j + 1, (i + 1) * nn + j, (i + 1) * nn + j, i * nn + j + 1, (i + 1) * nn + j + 1};
+            m.idx.insert(m.idx.end(), t, t + 6);
+        }
+    return m;
+}
+
+/* CreateBox (geometry_generator.cpp:22-121) */
+static HostMesh make_box(float width, float height, float depth) {
+    float w2 = 0.5f * width, h2 = 0.5f * height, d2 = 0.5f * depth;
+    /* per face: normal, then four corners (x,y,z signs) and their uvs */
+    struct Face { float n[3]; float c[4][3]; float t[4][2]; };
+    const Face f[6] = {
+        {{0, 0, 1}, {{-1, -1, 1}, {-1, 1, 1}, {1, 1, 1}, {1, -1, 1}}, {{0, 1}, {0, 0}, {1, 0}, {1, 1}}},
+        {{0, 0, -1}, {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1}}, {{1, 1}, {0, 1}, {0, 0}, {1, 0}}},
+        {{0, 1, 0}, {{-1, 1, 1}, {-1, 1, -1}, {1, 1, -1}, {1, 1, 1}}, {{0, 1}, {0, 0}, {1, 0}, {1, 1}}},
+        {{0, -1, 0}, {{-1, -1, 1}, {1, -1, 1}, {1, -1, -1}, {-1, -1, -1}}, {{1, 1}, {0, 1}, {0, 0}, {1, 0}}},
+        {{-1, 0, 0}, {{-1, -1, -1}, {-1, 1, -1}, {-1, 1, 1}, {-1, -1, 1}}, {{0, 1}, {0, 0}, {1, 0}, {1, 1}}},
+        {{1, 0, 0}, {{1, -1, 1}, {1, 1, 1}, {1, 1, -1}, {1, -1, -1}}, {{0, 1}, {0, 0}, {1, 0}, {1, 1}}},
+    };
+    HostMesh m;
+    for (int fi = 0; fi < 6; ++fi) {
+        uint32_t b = (uint32_t)m.pos.size();
+        for (int c = 0; c < 4; ++c) {
+            m.pos.push_back(v3(f[fi].c[c][0] * w2, f[fi].c[c][1] * h2, f[fi].c[c][2] * d2));
+            m.nrm.push_back(v3(f[fi].n[0], f[fi].n[1], f[fi].n[2]));
+            m.uv.push_back(f[fi].t[c][0]);
+            m.uv.push_back(f[fi].t[c][1]);
+        }
+        uint32_t t[6] = {b, b + 1, b + 2, b, b + 2, b + 3};
+        m.idx.insert(m.idx.end(), t, t + 6);
+    }
+    return m;
+}
+
+static uint32_t export_mesh(const HostMesh &m, float *pos, float *nrm, float *uv, uint32_t *idx, uint32_t *ntris) {
+    if (ntris) *ntris = (uint32_t)(m.idx.size() / 3);
+    for (size_t i = 0; i < m.pos.size(); ++i) {
+        if (pos) { pos[3 * i] = m.pos[i].x; pos[3 * i + 1] = m.pos[i].y; pos[3 * i + 2] = m.pos[i].z; }
+        if (nrm) { nrm[3 * i] = m.nrm[i].x; nrm[3 * i + 1] = m.nrm[i].y; nrm[3 * i + 2] = m.nrm[i].z; }
+    }
+    if (uv && !m.uv.empty()) std::memcpy(uv, m.uv.data(), m.uv.size() * sizeof(float));
+    if (idx) std::memcpy(idx, m.idx.data(), m.idx.size() * sizeof(uint32_t));
+    return (uint32_t)m.pos.size();
+}
+
+/* CalculateBoundingSphere (scene.cpp:49-80) */
+static void bounding_sphere(const float *p, size_t n, float out[4]) {
+    V3 c = v3(p[0], p[1], p[2]);
+    float r = 0.0001f;
+    for (int it = 0; it < 2; ++it)
+        for (size_t j = 0; j < n; ++j) {
+            V3 q = v3(p[3 * j], p[3 * j + 1], p[3 * j + 2]);
+            V3 d = q - c;
+            float len = length(d);
+            if (len > r) {
+                float a = len / r, a2 = a * a;
+                r = 0.5f * (a + 1 / a) * r;
+                c = 0.5f * ((1 + 1 / a2) * c + (1 - 1 / a2) * q);
+            }
+        }
+    for (size_t j = 0; j < n; ++j) {
+        V3 q = v3(p[3 * j], p[3 * j + 1], p[3 * j + 2]);
+        V3 d = q - c;
+        float len = length(d);
+        if (len > r) {
+            r = (r + len) / 2.0f;
+            c = c + ((len - r) / len * d);
+        }
+    }
+    out[0] = c.x; out[1] = c.y; out[2] = c.z; out[3] = r;
+}
+
+} // namespace
+} // namespace lt
+
+using namespace lt;
+
+#define LT_CHECK(cond, msg) \
+    do {                    \
+        if (!(cond)) return set_error(LT_E_INVALID, msg); \
+    } while (0)
+
+extern "C" {
+
+const char *lt_last_error(void) { return g_error.c_str(); }
+
+/* ---- device ---------------------------------------------------------------- */
+int lt_device_create(int ordinal, lt_device **out) {
+    LT_CHECK(out, "lt_device_create: out is NULL");
+    *out = nullptr;
+    int n = 0;
+    LT_HIP(hipGetDeviceCount(&n));
+    LT_CHECK(ordinal >= 0 && ordinal < n, "lt_device_create: no such device");
+    LT_HIP(hipSetDevice(ordinal));
+    hipDeviceProp_t prop;
+    LT_HIP(hipGetDeviceProperties(&prop, ordinal));
+    lt_device *d = new (std::nothrow) lt_device();
+    if (!d) return set_error(LT_E_NOMEM, "out of host memory");
+    d->ordinal = ordinal;
+    d->num_cus = prop.multiProcessorCount;
+    hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete d;
+        return hip_error(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+    }
+    int rc = d->trace_work.alloc(64);
+    if (rc) {
+        (void)hipStreamDestroy(d->stream);
+        delete d;
+        return rc;
+    }
+    *out = d;
+    return LT_OK;
+}
+
+int lt_device_destroy(lt_device *d) {
+    if (!d) return LT_OK;
+    (void)hipSetDevice(d->ordinal);
+    (void)hipStreamSynchronize(d->stream);
+    d->ws.release();
+    d->ctl.release();
+    d->trace_work.release();
+    d->spill.release();
+    (void)hipStreamDestroy(d->stream);
+    delete d;
+    return LT_OK;
+}
+
+int lt_device_stream(lt_device *d, void **out) {
+    LT_CHECK(d && out, "lt_device_stream: NULL argument");
+    *out = (void *)d->stream;
+    return LT_OK;
+}
+
+int lt_device_synchronize(lt_device *d) {
+    LT_CHECK(d, "lt_device_synchronize: NULL device");
+    LT_HIP(hipStreamSynchronize(d->stream));
+    return LT_OK;
+}
+
+/* ---- scene ----------------------------------------------------------------- */
+int lt_scene_create(lt_device *dev, lt_scene **out) {
+    LT_CHECK(dev && out, "lt_scene_create: NULL argument");
+    lt_scene *s = new (std::nothrow) lt_scene();
+    if (!s) return set_error(LT_E_NOMEM, "out of host memory");
+    s->dev = dev;
+    *out = s;
+    return LT_OK;
+}
+
+int lt_scene_destroy(lt_scene *s) {
+    if (!s) return LT_OK;
+    (void)hipStreamSynchronize(s->dev->stream);
+    delete s;
+    return LT_OK;
+}
+
+int lt_scene_add_mesh(lt_scene *s, const float *positions, size_t stride, uint32_t nv, const uint32_t *indices,
+                      uint32_t nprims, uint32_t vpp, const float *normals, const float *uvs, uint32_t *out_geom) {
+    LT_CHECK(s && positions && indices && out_geom, "lt_scene_add_mesh: NULL argument");
+    LT_CHECK(vpp == 3 || vpp == 4, "lt_scene_add_mesh: verts_per_prim must be 3 or 4");
+    LT_CHECK(nv > 0 && nprims > 0, "lt_scene_add_mesh: empty mesh");
+    if (stride == 0) stride = 3 * sizeof(float);
+    LT_CHECK(stride >= 3 * sizeof(float) && stride % sizeof(float) == 0, "lt_scene_add_mesh: bad position stride");
+    for (uint64_t i = 0; i < (uint64_t)nprims * vpp; ++i)
+        LT_CHECK(indices[i] < nv, "lt_scene_add_mesh: index out of range");
+    if (s->tri_geom.size() + (size_t)nprims * (vpp == 4 ? 2 : 1) > 0x7FFFFFFFull)
+        return set_error(LT_E_INVALID, "lt_scene_add_mesh: too many primitives");
+    uint32_t g = (uint32_t)s->geoms.size();
+    uint32_t vbase = (uint32_t)(s->pos.size() / 3);
+    DGeom dg;
+    dg.tbase = (uint32_t)s->tri_geom.size();
+    dg.material = -1;
+    dg.light = -1;
+    dg.flags = (normals ? 1u : 0u) | (uvs ? 2u : 0u);
+    const size_t sf = stride / sizeof(float);
+    for (uint32_t i = 0; i < nv; ++i) {
+        s->pos.push_back(positions[i * sf]);
+        s->pos.push_back(positions[i * sf + 1]);
+        s->pos.push_back(positions[i * sf + 2]);
+        for (int k = 0; k < 3; ++k) s->nrm.push_back(normals ? normals[3 * i + k] : 0.0f);
+        for (int k = 0; k < 2; ++k) s->uv.push_back(uvs ? uvs[2 * i + k] : 0.0f);
+    }
+    for (uint32_t p = 0; p < nprims; ++p) {
+        const uint32_t *ix = indices + (size_t)p * vpp;
+        uint32_t tri[2][3] = {{ix[0], ix[1], ix[2]}, {ix[0], vpp == 4 ? ix[2] : 0u, vpp == 4 ? ix[3] : 0u}};
+        for (uint32_t t = 0; t < (vpp == 4 ? 2u : 1u); ++t) {
+            for (int k = 0; k < 3; ++k) s->vidx.push_back(vbase + tri[t][k]);
+            s->tri_geom.push_back(g);
+        }
+    }
+    s->geoms.push_back(dg);
+    s->geom_nv.push_back(nv);
+    s->geom_vbase.push_back(vbase);
+    s->committed = false;
+    *out_geom = g;
+    return LT_OK;
+}
+
+int lt_scene_add_material(lt_scene *s, const lt_material *m, uint32_t *out_id) {
+    LT_CHECK(s && m && out_id, "lt_scene_add_material: NULL argument");
+    LT_CHECK(m->bsdf >= LT_BSDF_LAMBERT && m->bsdf <= LT_BSDF_GGX, "lt_scene_add_material: unknown bsdf");
+    DMaterial d;
+    d.bsdf = m->bsdf;
+    for (int k = 0; k < 3; ++k) d.albedo[k] = m->albedo[k];
+    d.texture = m->texture;
+    d.ior = m->ior;
+    d.roughness = m->roughness;
+    d.medium = m->medium;
+    s->mats.push_back(d);
+    s->committed = false;
+    *out_id = (uint32_t)s->mats.size() - 1;
+    return LT_OK;
+}
+
+int lt_scene_add_medium(lt_scene *s, const lt_medium *m, uint32_t *out_id) {
+    LT_CHECK(s && m && out_id, "lt_scene_add_medium: NULL argument");
+    LT_CHECK(m->type >= LT_MEDIUM_NON_SCATTERING && m->type <= LT_MEDIUM_GRID, "lt_scene_add_medium: unknown type");
+    DMedium d;
+    std::memset(&d, 0, sizeof d);
+    d.type = m->type;
+    /* Medium::Medium (medium.h:19-23): sigma_a = -log(color) / distance */
+    for (int k = 0; k < 3; ++k) d.sigma_a[k] = -logf(m->absorption_color[k]) / m->absorption_at_distance;
+    /* IsotropicScatteringMedium (isotropic_scattering_medium.h:22-25) */
+    d.sigma_s = m->type == LT_MEDIUM_ISOTROPIC ? 1.0f / m->scattering_distance : 0.0f;
+    d.grid = m->grid;
+    d.scale = m->sigma_t_scale;
+    d.albedo = m->albedo;
+    d.g = m->hg_g;
+    s->media.push_back(d);
+    s->committed = false;
+    *out_id = (uint32_t)s->media.size() - 1;
+    return LT_OK;
+}
+
+int lt_scene_add_texture(lt_scene *s, uint32_t w, uint32_t h, const float *rgb, uint32_t *out_id) {
+    LT_CHECK(s && rgb && out_id && w > 0 && h > 0, "lt_scene_add_texture: bad argument");
+    DTexture t;
+    t.w = w;
+    t.h = h;
+    t.offset = (uint32_t)s->texels.size();
+    t.pad = 0;
+    s->texels.insert(s->texels.end(), rgb, rgb + (size_t)w * h * 3);
+    s->texs.push_back(t);
+    s->committed = false;
+    *out_id = (uint32_t)s->texs.size() - 1;
+    return LT_OK;
+}
+
+int lt_scene_add_grid(lt_scene *s, uint32_t nx, uint32_t ny, uint32_t nz, const float *density, const float bmin[3],
+                      const float bmax[3], uint32_t *out_id) {
+    LT_CHECK(s && density && bmin && bmax && out_id && nx && ny && nz, "lt_scene_add_grid: bad argument");
+    DGrid g;
+    std::memset(&g, 0, sizeof g);
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    g.offset = (uint32_t)s->voxels.size();
+    size_t n = (size_t)nx * ny * nz;
+    float mx = 0.0f;
+    for (size_t i = 0; i < n; ++i) mx = std::max(mx, density[i]);
+    g.maxd = mx;
+    for (int k = 0; k < 3; ++k) { g.bmin[k] = bmin[k]; g.bmax[k] = bmax[k]; }
+    s->voxels.insert(s->voxels.end(), density, density + n);
+    s->grids.push_back(g);
+    s->committed = false;
+    *out_id = (uint32_t)s->grids.size() - 1;
+    return LT_OK;
+}
+
+int lt_scene_set_material(lt_scene *s, uint32_t geom, uint32_t material) {
+    LT_CHECK(s && geom < s->geoms.size() && material < s->mats.size(), "lt_scene_set_material: bad id");
+    s->geoms[geom].material = (int32_t)material;
+    s->committed = false;
+    return LT_OK;
+}
+
+int lt_scene_set_emission(lt_scene *s, uint32_t geom, const float color[3], float power) {
+    LT_CHECK(s && color && geom < s->geoms.size(), "lt_scene_set_emission: bad argument");
+    DGeom &g = s->geoms[geom];
+    uint32_t vb = s->geom_vbase[geom], nv = s->geom_nv[geom];
+    uint32_t t1 = geom + 1 < s->geoms.size() ? s->geoms[geom + 1].tbase : (uint32_t)s->tri_geom.size();
+    /* surface area as AddMesh sums it (scene.cpp:464-471) */
+    float area = 0.0f;
+    const float *P = s->pos.data();
+    for (uint32_t t = g.tbase; t < t1; ++t) {
+        const uint32_t *ix = &s->vidx[3 * (size_t)t];
+        V3 a = v3(P[3 * ix[0]], P[3 * ix[0] + 1], P[3 * ix[0] + 2]);
+        V3 b = v3(P[3 * ix[1]], P[3 * ix[1] + 1], P[3 * ix[1] + 2]);
+        V3 c = v3(P[3 * ix[2]], P[3 * ix[2] + 1], P[3 * ix[2] + 2]);
+        area += 0.5f * length(cross(a - b, a - c));
+    }
+    LT_CHECK(area > 0.0f, "lt_scene_set_emission: geometry has zero area");
+    DLight L;
+    std::memset(&L, 0, sizeof L);
+    bounding_sphere(P + 3 * (size_t)vb, nv, L.sphere);
+    /* AreaLight (area_light.h:16-21): Le = color * power * (1/pi) / area */
+    for (int k = 0; k < 3; ++k) L.le[k] = color[k] * power * (float)M_1_PI / area;
+    L.area = area;
+    L.geom = geom;
+    if (g.light >= 0) s->lights[g.light] = L;
+    else {
+        s->lights.push_back(L);
+        g.light = (int32_t)s->lights.size() - 1;
+    }
+    s->committed = false;
+    return LT_OK;
+}
+
+int lt_scene_set_background(lt_scene *s, const float rgb[3]) {
+    LT_CHECK(s && rgb, "lt_scene_set_background: NULL argument");
+    for (int k = 0; k < 3; ++k) s->bg[k] = rgb[k];
+    s->committed = false;
+    return LT_OK;
+}
+
+int lt_scene_commit(lt_scene *s, const lt_build_params *params) {
+    LT_CHECK(s, "lt_scene_commit: NULL scene");
+    for (size_t g = 0; g < s->geoms.size(); ++g)
+        if (s->geoms[g].material < 0) return set_error(LT_E_STATE, "lt_scene_commit: geometry without a material");
+    for (const DMaterial &m : s->mats) {
+        if (m.texture >= 0 && (size_t)m.texture >= s->texs.size())
+            return set_error(LT_E_INVALID, "lt_scene_commit: material references a missing texture");
+        if (m.medium >= 0 && (size_t)m.medium >= s->media.size())
+            return set_error(LT_E_INVALID, "lt_scene_commit: material references a missing medium");
+    }
+    for (const DMedium &m : s->media)
+        if (m.type == MEDIUM_GRID && (m.grid < 0 || (size_t)m.grid >= s->grids.size()))
+            return set_error(LT_E_INVALID, "lt_scene_commit: grid medium references a missing grid");
+    uint32_t max_leaf = params && params->max_leaf_size ? params->max_leaf_size : 4;
+    LT_CHECK(max_leaf >= 1 && max_leaf <= 8, "lt_scene_commit: max_leaf_size must be 1..8");
+    hipStream_t st = s->dev->stream;
+    LT_HIP(hipSetDevice(s->dev->ordinal));
+    int rc;
+    if ((rc = s->d_pos.upload(s->pos.data(), s->pos.size() * 4, st)) || (rc = s->d_nrm.upload(s->nrm.data(), s->nrm.size() * 4, st)) ||
+        (rc = s->d_uv.upload(s->uv.data(), s->uv.size() * 4, st)) || (rc = s->d_vidx.upload(s->vidx.data(), s->vidx.size() * 4, st)) ||
+        (rc = s->d_tri_geom.upload(s->tri_geom.data(), s->tri_geom.size() * 4, st)) ||
+        (rc = s->d_geoms.upload(s->geoms.data(), s->geoms.size() * sizeof(DGeom), st)) ||
+        (rc = s->d_mats.upload(s->mats.data(), s->mats.size() * sizeof(DMaterial), st)) ||
+        (rc = s->d_media.upload(s->media.data(), s->media.size() * sizeof(DMedium), st)) ||
+        (rc = s->d_texs.upload(s->texs.data(), s->texs.size() * sizeof(DTexture), st)) ||
+        (rc = s->d_texels.upload(s->texels.data(), s->texels.size() * 4, st)) ||
+        (rc = s->d_grids.upload(s->grids.data(), s->grids.size() * sizeof(DGrid), st)) ||
+        (rc = s->d_voxels.upload(s->voxels.data(), s->voxels.size() * 4, st)) ||
+        (rc = s->d_lights.upload(s->lights.data(), s->lights.size() * sizeof(DLight), st)))
+        return rc;
+    auto t0 = std::chrono::steady_clock::now();
+    uint32_t ntris = (uint32_t)s->tri_geom.size();
+    if ((rc = build_bvh(st, s->d_pos.as<float>(), s->d_vidx.as<uint32_t>(), ntris, max_leaf, s->d_nodes, s->d_tris, s->bvh)))
+        return rc;
+    LT_HIP(hipStreamSynchronize(st));
+    s->build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    DScene &d = s->ds;
+    std::memset(&d, 0, sizeof d);
+    d.nodes = s->d_nodes.as<float4>();
+    d.tris = s->d_tris.as<float4>();
+    d.tri_geom = s->d_tri_geom.as<uint32_t>();
+    d.tri_vidx = s->d_vidx.as<uint32_t>();
+    d.normals = s->d_nrm.as<float>();
+    d.uvs = s->d_uv.as<float>();
+    d.geoms = s->d_geoms.as<DGeom>();
+    d.mats = s->d_mats.as<DMaterial>();
+    d.lights = s->d_lights.as<DLight>();
+    d.media = s->d_media.as<DMedium>();
+    d.textures = s->d_texs.as<DTexture>();
+    d.texels = s->d_texels.as<float>();
+    d.grids = s->d_grids.as<DGrid>();
+    d.voxels = s->d_voxels.as<float>();
+    d.ntris = ntris;
+    d.nlights = (uint32_t)s->lights.size();
+    for (int k = 0; k < 3; ++k) d.bg[k] = s->bg[k];
+    s->committed = true;
+    return LT_OK;
+}
+
+int lt_scene_get_info(lt_scene *s, lt_scene_info *o) {
+    LT_CHECK(s && o, "lt_scene_get_info: NULL argument");
+    std::memset(o, 0, sizeof *o);
+    o->num_geometries = (uint32_t)s->geoms.size();
+    o->num_triangles = (uint32_t)s->tri_geom.size();
+    o->num_nodes = s->bvh.num_nodes;
+    o->num_leaves = s->bvh.num_leaves;
+    o->max_depth = s->bvh.max_depth;
+    o->num_lights = (uint32_t)s->lights.size();
+    o->sah_cost = s->bvh.sah_cost;
+    o->build_ms = s->build_ms;
+    o->device_bytes = s->d_pos.bytes + s->d_nrm.bytes + s->d_uv.bytes + s->d_vidx.bytes + s->d_tri_geom.bytes +
+                      s->d_nodes.bytes + s->d_tris.bytes + s->d_texels.bytes + s->d_voxels.bytes;
+    return LT_OK;
+}
+
+int lt_scene_get_lights(lt_scene *s, float *out, uint32_t capacity, uint32_t *out_count) {
+    LT_CHECK(s && out_count, "lt_scene_get_lights: NULL argument");
+    *out_count = (uint32_t)s->lights.size();
+    if (out)
+        for (uint32_t i = 0; i < std::min(capacity, *out_count); ++i) {
+            const DLight &L = s->lights[i];
+            float v[9] = {L.le[0], L.le[1], L.le[2], L.area, L.sphere[0], L.sphere[1], L.sphere[2], L.sphere[3], (float)L.geom};
+            std::memcpy(out + 9 * i, v, sizeof v);
+        }
+    return LT_OK;
+}
+
+int lt_scene_split_prims(lt_scene *s, const uint32_t *gp, uint32_t n, uint32_t *geom, uint32_t *prim) {
+    LT_CHECK(s && gp && geom && prim, "lt_scene_split_prims: NULL argument");
+    for (uint32_t i = 0; i < n; ++i) {
+        if (gp[i] == LT_INVALID_ID || gp[i] >= s->tri_geom.size()) {
+            geom[i] = prim[i] = LT_INVALID_ID;
+            continue;
+        }
+        uint32_t g = s->tri_geom[gp[i]];
+        geom[i] = g;
+        prim[i] = gp[i] - s->geoms[g].tbase;
+    }
+    return LT_OK;
+}
+
+/* ---- ray queries ------------------------------------------------------------ */
+static int intersect_device(lt_scene *s, const lt_ray *rays, void *out, uint32_t n, void *stream, bool any) {
+    LT_CHECK(s && rays && out, "lt_intersect: NULL argument");
+    if (!s->committed) return set_error(LT_E_STATE, "lt_intersect: scene is not committed");
+    if (n == 0) return LT_OK;
+    lt_device *d = s->dev;
+    hipStream_t st = stream ? (hipStream_t)stream : d->stream;
+    int block = trace_default_block();
+    int grid = trace_grid(d->num_cus, block);
+    int rc = d->spill.reserve(trace_spill_bytes(grid, block));
+    if (rc) return rc;
+    LT_HIP(hipMemsetAsync(d->trace_work.p, 0, sizeof(uint32_t), st));
+    if (any)
+        return launch_trace_any(st, s->ds, (const Ray *)rays, (uint32_t *)out, nullptr, n, d->trace_work.as<uint32_t>(),
+                                block, grid, (int *)d->spill.p);
+    return launch_trace_closest(st, s->ds, (const Ray *)rays, (Hit *)out, nullptr, n, d->trace_work.as<uint32_t>(),
+                                block, grid, (int *)d->spill.p);
+}
+
+int lt_intersect_closest_device(lt_scene *s, const lt_ray *rays, lt_hit *hits, uint32_t n, void *stream) {
+    return intersect_device(s, rays, hits, n, stream, false);
+}
+int lt_intersect_any_device(lt_scene *s, const lt_ray *rays, uint32_t *occluded, uint32_t n, void *stream) {
+    return intersect_device(s, rays, occluded, n, stream, true);
+}
+
+static int intersect_host(lt_scene *s, const lt_ray *rays, void *out, size_t out_each, uint32_t n, bool any) {
+    LT_CHECK(s && rays && out, "lt_intersect: NULL argument");
+    if (n == 0) return LT_OK;
+    hipStream_t st = s->dev->stream;
+    DevBuf dr, dh;
+    int rc;
+    if ((rc = dr.upload(rays, sizeof(lt_ray) * (size_t)n, st)) || (rc = dh.alloc(out_each * (size_t)n))) return rc;
+    if ((rc = intersect_device(s, dr.as<lt_ray>(), dh.p, n, st, any))) return rc;
+    LT_HIP(hipMemcpyAsync(out, dh.p, out_each * (size_t)n, hipMemcpyDeviceToHost, st));
+    LT_HIP(hipStreamSynchronize(st));
+    return LT_OK;
+}
+int lt_intersect_closest(lt_scene *s, const lt_ray *rays, lt_hit *hits, uint32_t n) {
+    return intersect_host(s, rays, hits, sizeof(lt_hit), n, false);
+}
+int lt_intersect_any(lt_scene *s, const lt_ray *rays, uint32_t *occluded, uint32_t n) {
+    return intersect_host(s, rays, occluded, sizeof(uint32_t), n, true);
+}
+
+/* ---- film ------------------------------------------------------------------- */
+int lt_film_create(lt_device *dev, uint32_t w, uint32_t h, lt_film **out) {
+    LT_CHECK(dev && out && w > 0 && h > 0, "lt_film_create: bad argument");
+    lt_film *f = new (std::nothrow) lt_film();
+    if (!f) return set_error(LT_E_NOMEM, "out of host memory");
+    f->dev = dev;
+    f->w = w;
+    f->h = h;
+    size_t n = (size_t)w * h;
+    int rc;
+    if ((rc = f->sum.alloc(n * 12)) || (rc = f->bounces.alloc(n * 4)) || (rc = f->count.alloc(n * 4))) {
+        delete f;
+        return rc;
+    }
+    rc = lt_film_reset(f);
+    if (rc) {
+        delete f;
+        return rc;
+    }
+    *out = f;
+    return LT_OK;
+}
+int lt_film_destroy(lt_film *f) {
+    if (!f) return LT_OK;
+    (void)hipStreamSynchronize(f->dev->stream);
+    delete f;
+    return LT_OK;
+}
+int lt_film_reset(lt_film *f) {
+    LT_CHECK(f, "lt_film_reset: NULL film");
+    size_t n = (size_t)f->w * f->h;
+    hipStream_t st = f->dev->stream;
+    LT_HIP(hipMemsetAsync(f->sum.p, 0, n * 12, st));
+    LT_HIP(hipMemsetAsync(f->bounces.p, 0, n * 4, st));
+    LT_HIP(hipMemsetAsync(f->count.p, 0, n * 4, st));
+    LT_HIP(hipStreamSynchronize(st));
+    return LT_OK;
+}
+int lt_film_download(lt_film *f, float *rgb, uint32_t *bounces, uint32_t *counts) {
+    LT_CHECK(f, "lt_film_download: NULL film");
+    size_t n = (size_t)f->w * f->h;
+    hipStream_t st = f->dev->stream;
+    if (rgb) LT_HIP(hipMemcpyAsync(rgb, f->sum.p, n * 12, hipMemcpyDeviceToHost, st));
+    if (bounces) LT_HIP(hipMemcpyAsync(bounces, f->bounces.p, n * 4, hipMemcpyDeviceToHost, st));
+    if (counts) LT_HIP(hipMemcpyAsync(counts, f->count.p, n * 4, hipMemcpyDeviceToHost, st));
+    LT_HIP(hipStreamSynchronize(st));
+    return LT_OK;
+}
+int lt_film_upload(lt_film *f, const float *rgb, const uint32_t *bounces, const uint32_t *counts) {
+    LT_CHECK(f, "lt_film_upload: NULL film");
+    size_t n = (size_t)f->w * f->h;
+    hipStream_t st = f->dev->stream;
+    if (rgb) LT_HIP(hipMemcpyAsync(f->sum.p, rgb, n * 12, hipMemcpyHostToDevice, st));
+    if (bounces) LT_HIP(hipMemcpyAsync(f->bounces.p, bounces, n * 4, hipMemcpyHostToDevice, st));
+    if (counts) LT_HIP(hipMemcpyAsync(f->count.p, counts, n * 4, hipMemcpyHostToDevice, st));
+    LT_HIP(hipStreamSynchronize(st));
+    return LT_OK;
+}
+int lt_film_resolve(lt_film *f, float *rgb) {
+    LT_CHECK(f && rgb, "lt_film_resolve: NULL argument");
+    size_t n = (size_t)f->w * f->h;
+    std::vector<uint32_t> cnt(n);
+    int rc = lt_film_download(f, rgb, nullptr, cnt.data());
+    if (rc) return rc;
+    for (size_t i = 0; i < n; ++i) {
+        float c = (float)cnt[i];
+        for (int k = 0; k < 3; ++k) rgb[3 * i + k] = cnt[i] ? rgb[3 * i + k] / c : 0.0f;
+    }
+    return LT_OK;
+}
+
+/* ---- rendering -------------------------------------------------------------- */
+int lt_render(lt_scene *s, const lt_camera *cam, const lt_render_params *prm, lt_film *film, lt_render_stats *stats) {
+    LT_CHECK(s && cam && prm && film, "lt_render: NULL argument");
+    if (!s->committed) return set_error(LT_E_STATE, "lt_render: scene is not committed");
+    LT_CHECK(film->w == cam->width && film->h == cam->height, "lt_render: film and camera sizes differ");
+    LT_CHECK(film->dev == s->dev, "lt_render: film belongs to another device");
+    LT_CHECK(cam->filter >= 0 && cam->filter <= 2, "lt_render: unknown filter");
+    LT_HIP(hipSetDevice(s->dev->ordinal));
+    return render_scene(s, *cam, *prm, film, stats);
+}
+
+int lt_render_batch(lt_scene *const *scenes, const lt_camera *cams, lt_film *const *films, uint32_t n,
+                    const lt_render_params *prm, lt_render_stats *stats) {
+    LT_CHECK(scenes && cams && films && prm, "lt_render_batch: NULL argument");
+    lt_render_stats acc;
+    std::memset(&acc, 0, sizeof acc);
+    for (uint32_t i = 0; i < n; ++i) {
+        lt_render_params p = *prm;
+        p.stream = prm->stream + i;
+        lt_render_stats one;
+        int rc = lt_render(scenes[i], &cams[i], &p, films[i], stats ? &one : nullptr);
+        if (rc) return rc;
+        if (stats) {
+            acc.samples += one.samples;
+            acc.closest_rays += one.closest_rays;
+            acc.shadow_rays += one.shadow_rays;
+            acc.path_segments += one.path_segments;
+            acc.total_ms += one.total_ms;
+            acc.extend_ms += one.extend_ms;
+            acc.shade_ms += one.shade_ms;
+            acc.shadow_ms += one.shadow_ms;
+            acc.resolve_ms += one.resolve_ms;
+            acc.generate_ms += one.generate_ms;
+            acc.accumulate_ms += one.accumulate_ms;
+            acc.extend_launches += one.extend_launches;
+            acc.shadow_launches += one.shadow_launches;
+            acc.waves += one.waves;
+        }
+    }
+    if (stats) *stats = acc;
+    return LT_OK;
+}
+
+/* ---- generators ---------------------------------------------------------------- */
+uint32_t lt_geometry_geosphere(float radius, uint32_t n, float *pos, float *nrm, uint32_t *idx, uint32_t *ntris) {
+    if (!pos && !nrm && !idx) {
+        if (ntris) *ntris = 20u << (2 * n);
+        return n == 0 ? 12u : (30u << (2 * n));
+    }
+    return export_mesh(make_geosphere(radius, n), pos, nrm, nullptr, idx, ntris);
+}
+uint32_t lt_geometry_grid(float width, float depth, uint32_t m, uint32_t n, float *pos, float *nrm, float *uv,
+                          uint32_t *idx, uint32_t *ntris) {
+    if (m < 2 || n < 2) {
+        if (ntris) *ntris = 0;
+        return 0;
+    }
+    if (!pos && !nrm && !uv && !idx) {
+        if (ntris) *ntris = 2 * (m - 1) * (n - 1);
+        return m * n;
+    }
+    return export_mesh(make_grid(width, depth, m, n), pos, nrm, uv, idx, ntris);
+}
+uint32_t lt_geometry_box(float w, float h, float d, float *pos, float *nrm, float *uv, uint32_t *idx, uint32_t *ntris) {
+    if (!pos && !nrm && !uv && !idx) {
+        if (ntris) *ntris = 12;
+        return 24;
+    }
+    return export_mesh(make_box(w, h, d), pos, nrm, uv, idx, ntris);
+}
+
+} // extern "C"

@@ -1,0 +1,423 @@
+/*
+ * lt_bvh.hip - GPU BVH build for rtcCommitScene (scene.cpp:86).
+ *
+ * The reference commits a BVH8Triangle4 with Embree's spatial-split SAH
+ * builder (kernels/bvh/bvh8_factory.cpp:936-945, bvh_builder_sah_spatial.cpp;
+ * the builder sources are absent from the reference tree). This file builds a
+ * linear BVH on the GPU instead (T. Karras, "Maximizing Parallelism in the
+ * Construction of BVHs, Octrees, and k-d Trees", HPG 2012):
+ *   1. centroid bounds (wave reduction + ordered-int atomics)
+ *   2. 30-bit Morton code of each centroid, packed over the triangle index into
+ *      a unique 64-bit key
+ *   3. rocprim radix sort of the keys
+ *   4. one thread per internal node finds its key range and split (Karras)
+ *   5. bottom-up boxes and SAH costs in 64 passes: a node completes in pass k
+ *      only from children completed in passes < k, so no thread reads data
+ *      written in the same launch
+ *   6. SAH leaf collapse: a subtree of <= max_leaf triangles becomes one leaf
+ *      when Ci*n <= Ct + (A_l C_l + A_r C_r)/A
+ *   7. node and triangle emission in the traversal layout.
+ *
+ * Traversal layout (64 B per node, Aila & Laine 2009 style):
+ *   n0 = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
+ *   n1 = (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y)
+ *   n2 = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)
+ *   n3 = (child0, child1, -, -) as int: >= 0 internal node, < 0 leaf ~first_tri
+ * Triangles (48 B, BVH order): (v0.xyz, prim), (e1.xyz, last_in_leaf), (e2.xyz, 0)
+ * with e1 = v0 - v1 and e2 = v2 - v0 precomputed as Embree's TriangleM does
+ * (kernels/geometry/triangle.h:53-54).
+ */
+#include "lt_internal.h"
+
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
+#include <vector>
+
+namespace lt {
+
+namespace {
+
+LT_D int ordered_int(float f) {
+    int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7FFFFFFF;
+}
+LT_HD float ordered_float(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7FFFFFFF); }
+
+LT_D void tri_box(const float *pos, const uint32_t *vidx, uint32_t t, float lo[3], float hi[3]) {
+    uint32_t a = vidx[3 * t], b = vidx[3 * t + 1], c = vidx[3 * t + 2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float va = pos[3 * a + k], vb = pos[3 * b + k], vc = pos[3 * c + k];
+        lo[k] = fminf(va, fminf(vb, vc));
+        hi[k] = fmaxf(va, fmaxf(vb, vc));
+    }
+}
+
+__global__ void k_centroid_bounds(const float *pos, const uint32_t *vidx, uint32_t n, int *cb) {
+    float lo[3] = {__builtin_huge_valf(), __builtin_huge_valf(), __builtin_huge_valf()};
+    float hi[3] = {-__builtin_huge_valf(), -__builtin_huge_valf(), -__builtin_huge_valf()};
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        float bl[3], bh[3];
+        tri_box(pos, vidx, t, bl, bh);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float c = 0.5f * (bl[k] + bh[k]);
+            lo[k] = fminf(lo[k], c);
+            hi[k] = fmaxf(hi[k], c);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        for (int off = 32; off > 0; off >>= 1) {
+            lo[k] = fminf(lo[k], __shfl_xor(lo[k], off));
+            hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], off));
+        }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&cb[k], ordered_int(lo[k]));
+            atomicMax(&cb[3 + k], ordered_int(hi[k]));
+        }
+    }
+}
+
+LT_D uint32_t expand_bits(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__global__ void k_morton(const float *pos, const uint32_t *vidx, uint32_t n, const int *cb, uint64_t *keys) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    float lo[3], hi[3];
+    tri_box(pos, vidx, t, lo, hi);
+    uint32_t q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float cmin = ordered_float(cb[k]), cmax = ordered_float(cb[3 + k]);
+        float ext = cmax - cmin;
+        float c = 0.5f * (lo[k] + hi[k]);
+        float f = ext > 0.0f ? (c - cmin) / ext : 0.5f;
+        f = fminf(fmaxf(f * 1024.0f, 0.0f), 1023.0f);
+        q[k] = (uint32_t)f;
+    }
+    uint32_t m = (expand_bits(q[0]) << 2) | (expand_bits(q[1]) << 1) | expand_bits(q[2]);
+    keys[t] = ((uint64_t)m << 32) | (uint64_t)t;
+}
+
+LT_D int kdelta(const uint64_t *k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    return __clzll(k[i] ^ k[j]);
+}
+
+/* Karras 2012, Figure 4. Child encoding: >= 0 internal node, < 0 leaf ~k. */
+__global__ void k_karras(const uint64_t *keys, int n, int2 *child, int *parent_int, int *parent_leaf,
+                         int2 *range) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (kdelta(keys, n, i, i + 1) - kdelta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = kdelta(keys, n, i, i - d);
+    int lmax = 2;
+    while (kdelta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (kdelta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = kdelta(keys, n, i, j);
+    int s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (kdelta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int gamma = i + s * d + min(d, 0);
+    int first = min(i, j), last = max(i, j);
+    int left = (first == gamma) ? ~gamma : gamma;
+    int right = (last == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+    child[i] = make_int2(left, right);
+    range[i] = make_int2(first, last);
+    if (left >= 0) parent_int[left] = i; else parent_leaf[~left] = i;
+    if (right >= 0) parent_int[right] = i; else parent_leaf[~right] = i;
+}
+
+struct BuildArrays {
+    const float *pos;
+    const uint32_t *vidx;
+    const uint64_t *keys;
+    const int2 *child;
+    const int2 *range;
+    const int *parent_int;
+    float4 *blo;      /* internal node boxes: xyz lo, w = SAH cost */
+    float4 *bhi;      /* xyz hi, w = surface area */
+    int *done;        /* pass number (1-based) in which the node completed */
+    int *collapse;
+    int n;
+    int max_leaf;
+};
+
+LT_D float area3(const float lo[3], const float hi[3]) {
+    float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+
+LT_D void child_info(const BuildArrays &a, int c, float lo[3], float hi[3], float &cost, float &area) {
+    if (c < 0) {
+        uint32_t tid = (uint32_t)(a.keys[~c] & 0xFFFFFFFFull);
+        tri_box(a.pos, a.vidx, tid, lo, hi);
+        cost = 1.0f;
+        area = area3(lo, hi);
+    } else {
+        float4 l = a.blo[c], h = a.bhi[c];
+        lo[0] = l.x; lo[1] = l.y; lo[2] = l.z;
+        hi[0] = h.x; hi[1] = h.y; hi[2] = h.z;
+        cost = l.w;
+        area = h.w;
+    }
+}
+
+__global__ void k_refit_pass(BuildArrays a, int pass) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n - 1) return;
+    if (a.done[i]) return;
+    int2 c = a.child[i];
+    bool ok0 = c.x < 0 || (a.done[c.x] != 0 && a.done[c.x] <= pass);
+    bool ok1 = c.y < 0 || (a.done[c.y] != 0 && a.done[c.y] <= pass);
+    if (!(ok0 && ok1)) return;
+    float l0[3], h0[3], l1[3], h1[3], c0, c1, a0, a1;
+    child_info(a, c.x, l0, h0, c0, a0);
+    child_info(a, c.y, l1, h1, c1, a1);
+    float lo[3], hi[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = fminf(l0[k], l1[k]);
+        hi[k] = fmaxf(h0[k], h1[k]);
+    }
+    float ar = area3(lo, hi);
+    int2 r = a.range[i];
+    int cnt = r.y - r.x + 1;
+    float split = 1.0f + (ar > 0.0f ? (a0 * c0 + a1 * c1) / ar : (c0 + c1));
+    float leaf = cnt <= a.max_leaf ? (float)cnt : __builtin_huge_valf();
+    int col = leaf <= split ? 1 : 0;
+    a.blo[i] = make_float4(lo[0], lo[1], lo[2], col ? leaf : split);
+    a.bhi[i] = make_float4(hi[0], hi[1], hi[2], ar);
+    a.collapse[i] = col;
+    a.done[i] = pass + 1;
+}
+
+/* A node is emitted when no proper ancestor collapsed and it did not collapse
+ * itself. depth/leaves are statistics only. */
+__global__ void k_reach(BuildArrays a, int *emit, unsigned int *stats) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n - 1) return;
+    int depth = 0;
+    bool reach = true;
+    int p = i;
+    while (p != 0) {
+        p = a.parent_int[p];
+        ++depth;
+        if (a.collapse[p]) reach = false;
+    }
+    int e = (reach && !a.collapse[i]) ? 1 : 0;
+    emit[i] = e;
+    if (e) {
+        int2 c = a.child[i];
+        unsigned int leaves = (unsigned)((c.x < 0 || a.collapse[c.x]) ? 1 : 0) + (unsigned)((c.y < 0 || a.collapse[c.y]) ? 1 : 0);
+        atomicAdd(&stats[0], 1u);
+        atomicAdd(&stats[1], leaves);
+        atomicMax(&stats[2], (unsigned)(depth + 1));
+    }
+}
+
+__global__ void k_write_tris(const float *pos, const uint32_t *vidx, const uint64_t *keys, uint32_t n, float4 *tris) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint32_t t = (uint32_t)(keys[k] & 0xFFFFFFFFull);
+    uint32_t ia = vidx[3 * t], ib = vidx[3 * t + 1], ic = vidx[3 * t + 2];
+    V3 v0 = v3(pos[3 * ia], pos[3 * ia + 1], pos[3 * ia + 2]);
+    V3 v1 = v3(pos[3 * ib], pos[3 * ib + 1], pos[3 * ib + 2]);
+    V3 v2 = v3(pos[3 * ic], pos[3 * ic + 1], pos[3 * ic + 2]);
+    V3 e1 = v0 - v1, e2 = v2 - v0;
+    tris[3 * k + 0] = make_float4(v0.x, v0.y, v0.z, __uint_as_float(t));
+    tris[3 * k + 1] = make_float4(e1.x, e1.y, e1.z, __uint_as_float(0u));
+    tris[3 * k + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+}
+
+LT_D void emit_child(const BuildArrays &a, int c, float lo[3], float hi[3], int &ptr, int &last, bool &leaf) {
+    float cost, ar;
+    child_info(a, c, lo, hi, cost, ar);
+    if (c < 0) {
+        ptr = c;
+        last = ~c;
+        leaf = true;
+    } else if (a.collapse[c]) {
+        int2 r = a.range[c];
+        ptr = ~r.x;
+        last = r.y;
+        leaf = true;
+    } else {
+        ptr = c;
+        last = -1;
+        leaf = false;
+    }
+}
+
+__global__ void k_emit(BuildArrays a, const int *emit, float4 *nodes, float4 *tris) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n - 1 || !emit[i]) return;
+    int2 c = a.child[i];
+    float l0[3], h0[3], l1[3], h1[3];
+    int p0, p1, last0, last1;
+    bool leaf0, leaf1;
+    emit_child(a, c.x, l0, h0, p0, last0, leaf0);
+    emit_child(a, c.y, l1, h1, p1, last1, leaf1);
+    nodes[4 * i + 0] = make_float4(l0[0], h0[0], l0[1], h0[1]);
+    nodes[4 * i + 1] = make_float4(l1[0], h1[0], l1[1], h1[1]);
+    nodes[4 * i + 2] = make_float4(l0[2], h0[2], l1[2], h1[2]);
+    nodes[4 * i + 3] = make_float4(__int_as_float(p0), __int_as_float(p1), 0.0f, 0.0f);
+    if (leaf0) tris[3 * last0 + 1].w = __uint_as_float(1u);
+    if (leaf1) tris[3 * last1 + 1].w = __uint_as_float(1u);
+}
+
+/* Root for scenes whose whole triangle set is one leaf: both child slots
+ * reference the same leaf (visiting it twice gives the same closest hit). */
+__global__ void k_single_leaf_root(const float *pos, const uint32_t *vidx, const uint64_t *keys, uint32_t n,
+                                   float4 *nodes, float4 *tris) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float lo[3] = {__builtin_huge_valf(), __builtin_huge_valf(), __builtin_huge_valf()};
+    float hi[3] = {-__builtin_huge_valf(), -__builtin_huge_valf(), -__builtin_huge_valf()};
+    for (uint32_t k = 0; k < n; ++k) {
+        float bl[3], bh[3];
+        tri_box(pos, vidx, (uint32_t)(keys[k] & 0xFFFFFFFFull), bl, bh);
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], bl[a]);
+            hi[a] = fmaxf(hi[a], bh[a]);
+        }
+    }
+    nodes[0] = make_float4(lo[0], hi[0], lo[1], hi[1]);
+    nodes[1] = make_float4(lo[0], hi[0], lo[1], hi[1]);
+    nodes[2] = make_float4(lo[2], hi[2], lo[2], hi[2]);
+    nodes[3] = make_float4(__int_as_float(~0), __int_as_float(~0), 0.0f, 0.0f);
+    tris[3 * (n - 1) + 1].w = __uint_as_float(1u);
+}
+
+inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
+
+} // namespace
+
+int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, uint32_t n, uint32_t max_leaf,
+              DevBuf &nodes, DevBuf &tris, BvhStats &stats) {
+    stats = BvhStats();
+    if (n == 0) return LT_OK;
+    const unsigned B = 256;
+    int rc;
+    DevBuf keys, keys_sorted, cb;
+    if ((rc = keys.alloc(sizeof(uint64_t) * n)) || (rc = keys_sorted.alloc(sizeof(uint64_t) * n)) ||
+        (rc = cb.alloc(sizeof(int) * 8)))
+        return rc;
+    int init[8];
+    for (int k = 0; k < 3; ++k) {
+        init[k] = 0x7FFFFFFF;                 /* ordered_int(+huge) upper bound */
+        init[3 + k] = (int)0x80000000;        /* lower bound */
+    }
+    init[6] = init[7] = 0;
+    LT_HIP(hipMemcpyAsync(cb.p, init, sizeof init, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_centroid_bounds, dim3(std::min(grid_for(n, B), 1024u)), dim3(B), 0, st, d_pos, d_vidx, n,
+                       (int *)cb.p);
+    hipLaunchKernelGGL(k_morton, dim3(grid_for(n, B)), dim3(B), 0, st, d_pos, d_vidx, n, (const int *)cb.p,
+                       (uint64_t *)keys.p);
+    LT_HIP(hipGetLastError());
+    size_t temp_bytes = 0;
+    LT_HIP(rocprim::radix_sort_keys(nullptr, temp_bytes, (uint64_t *)keys.p, (uint64_t *)keys_sorted.p, (size_t)n, 0,
+                                    62, st));
+    DevBuf temp;
+    if ((rc = temp.alloc(std::max<size_t>(temp_bytes, 16)))) return rc;
+    LT_HIP(rocprim::radix_sort_keys(temp.p, temp_bytes, (uint64_t *)keys.p, (uint64_t *)keys_sorted.p, (size_t)n, 0,
+                                    62, st));
+    const uint64_t *sk = (const uint64_t *)keys_sorted.p;
+
+    if ((rc = tris.alloc(sizeof(float4) * 3 * (size_t)n))) return rc;
+    uint32_t nnodes = n > 1 ? n - 1 : 1;
+    if ((rc = nodes.alloc(sizeof(float4) * 4 * (size_t)nnodes))) return rc;
+    LT_HIP(hipMemsetAsync(nodes.p, 0, sizeof(float4) * 4 * (size_t)nnodes, st));
+    hipLaunchKernelGGL(k_write_tris, dim3(grid_for(n, B)), dim3(B), 0, st, d_pos, d_vidx, sk, n, (float4 *)tris.p);
+    LT_HIP(hipGetLastError());
+
+    if (n == 1) {
+        hipLaunchKernelGGL(k_single_leaf_root, dim3(1), dim3(64), 0, st, d_pos, d_vidx, sk, n, (float4 *)nodes.p,
+                           (float4 *)tris.p);
+        LT_HIP(hipGetLastError());
+        stats.num_nodes = 1;
+        stats.num_leaves = 1;
+        stats.max_depth = 1;
+        stats.sah_cost = 1.0f;
+        return LT_OK;
+    }
+
+    DevBuf child, range, pint, pleaf, blo, bhi, done, collapse, emit, dstats;
+    size_t ni = n - 1;
+    if ((rc = child.alloc(sizeof(int2) * ni)) || (rc = range.alloc(sizeof(int2) * ni)) ||
+        (rc = pint.alloc(sizeof(int) * ni)) || (rc = pleaf.alloc(sizeof(int) * n)) ||
+        (rc = blo.alloc(sizeof(float4) * ni)) || (rc = bhi.alloc(sizeof(float4) * ni)) ||
+        (rc = done.alloc(sizeof(int) * ni)) || (rc = collapse.alloc(sizeof(int) * ni)) ||
+        (rc = emit.alloc(sizeof(int) * ni)) || (rc = dstats.alloc(sizeof(unsigned) * 4)))
+        return rc;
+    LT_HIP(hipMemsetAsync(done.p, 0, sizeof(int) * ni, st));
+    LT_HIP(hipMemsetAsync(collapse.p, 0, sizeof(int) * ni, st));
+    LT_HIP(hipMemsetAsync(pint.p, 0, sizeof(int) * ni, st));
+    LT_HIP(hipMemsetAsync(dstats.p, 0, sizeof(unsigned) * 4, st));
+    hipLaunchKernelGGL(k_karras, dim3(grid_for(ni, B)), dim3(B), 0, st, sk, (int)n, (int2 *)child.p, (int *)pint.p,
+                       (int *)pleaf.p, (int2 *)range.p);
+    LT_HIP(hipGetLastError());
+    BuildArrays a;
+    a.pos = d_pos;
+    a.vidx = d_vidx;
+    a.keys = sk;
+    a.child = (const int2 *)child.p;
+    a.range = (const int2 *)range.p;
+    a.parent_int = (const int *)pint.p;
+    a.blo = (float4 *)blo.p;
+    a.bhi = (float4 *)bhi.p;
+    a.done = (int *)done.p;
+    a.collapse = (int *)collapse.p;
+    a.n = (int)n;
+    a.max_leaf = (int)max_leaf;
+    /* The key is 62 bits wide, so a node's common prefix grows by at least one
+     * bit per level and the hierarchy is at most 63 levels deep. */
+    for (int pass = 0; pass < 64; ++pass)
+        hipLaunchKernelGGL(k_refit_pass, dim3(grid_for(ni, B)), dim3(B), 0, st, a, pass);
+    LT_HIP(hipGetLastError());
+    int root_done = 0, root_collapse = 0;
+    float4 root_lo;
+    LT_HIP(hipMemcpyAsync(&root_done, done.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    LT_HIP(hipMemcpyAsync(&root_collapse, collapse.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    LT_HIP(hipMemcpyAsync(&root_lo, blo.p, sizeof(float4), hipMemcpyDeviceToHost, st));
+    LT_HIP(hipStreamSynchronize(st));
+    if (!root_done) return set_error(LT_E_STATE, "BVH build: hierarchy deeper than 64 levels");
+    if (root_collapse) {
+        hipLaunchKernelGGL(k_single_leaf_root, dim3(1), dim3(64), 0, st, d_pos, d_vidx, sk, n, (float4 *)nodes.p,
+                           (float4 *)tris.p);
+        LT_HIP(hipGetLastError());
+        stats.num_nodes = 1;
+        stats.num_leaves = 1;
+        stats.max_depth = 1;
+        stats.sah_cost = root_lo.w;
+        return LT_OK;
+    }
+    hipLaunchKernelGGL(k_reach, dim3(grid_for(ni, B)), dim3(B), 0, st, a, (int *)emit.p, (unsigned *)dstats.p);
+    hipLaunchKernelGGL(k_emit, dim3(grid_for(ni, B)), dim3(B), 0, st, a, (const int *)emit.p, (float4 *)nodes.p,
+                       (float4 *)tris.p);
+    LT_HIP(hipGetLastError());
+    unsigned hs[4];
+    LT_HIP(hipMemcpyAsync(hs, dstats.p, sizeof hs, hipMemcpyDeviceToHost, st));
+    LT_HIP(hipStreamSynchronize(st));
+    stats.num_nodes = hs[0];
+    stats.num_leaves = hs[1];
+    stats.max_depth = hs[2];
+    stats.sah_cost = root_lo.w;
+    return LT_OK;
+}
+
+} // namespace lt

@@ -1,0 +1,265 @@
+/*
+ * lt_device.h - device-side math, random numbers and scene tables for the
+ * gfx950 rendering path.
+ *
+ * Arithmetic follows the reference's forms so that results are reproducible
+ * on any IEEE machine that honours these exact operations:
+ *   dot(a,b)   = fma(a.x,b.x, fma(a.y,b.y, a.z*b.z))   embree common/math/vec3.h:201
+ *   cross(a,b) = (fma(a.y,b.z,-(a.z*b.y)), ...)        embree common/math/vec3.h:206
+ *   normalize(a) = a * (1/sqrt(dot(a,a)))  (correctly rounded sqrt and divide;
+ *     Embree's rsqrt estimate is ISA-specific, vec3fa.h:153-164)
+ * The library is compiled with -ffp-contract=off, so every other product and
+ * sum is rounded separately, as the reference's SSE code does.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define LT_HD __host__ __device__ __forceinline__
+#define LT_D __device__ __forceinline__
+
+namespace lt {
+
+/* ------------------------------------------------------------------------ */
+/* float3 helpers                                                            */
+/* ------------------------------------------------------------------------ */
+struct V3 {
+    float x, y, z;
+};
+LT_HD V3 v3(float a, float b, float c) { return V3{a, b, c}; }
+LT_HD V3 v3s(float a) { return V3{a, a, a}; }
+LT_HD V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+LT_HD V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+LT_HD V3 operator-(V3 a) { return V3{-a.x, -a.y, -a.z}; }
+LT_HD V3 operator*(V3 a, V3 b) { return V3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+LT_HD V3 operator*(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
+LT_HD V3 operator*(float s, V3 a) { return V3{s * a.x, s * a.y, s * a.z}; }
+LT_HD V3 operator/(V3 a, float s) { return V3{a.x / s, a.y / s, a.z / s}; }
+LT_HD float dot(V3 a, V3 b) { return __builtin_fmaf(a.x, b.x, __builtin_fmaf(a.y, b.y, a.z * b.z)); }
+LT_HD V3 cross(V3 a, V3 b) {
+    return V3{__builtin_fmaf(a.y, b.z, -(a.z * b.y)), __builtin_fmaf(a.z, b.x, -(a.x * b.z)),
+              __builtin_fmaf(a.x, b.y, -(a.y * b.x))};
+}
+LT_HD float length(V3 a) { return sqrtf(dot(a, a)); }
+LT_HD V3 normalize(V3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+LT_HD bool is_black(V3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
+LT_HD float fmax_ref(float a, float b) { return (a < b) ? b : a; }   /* std::max */
+LT_HD float fmin_ref(float a, float b) { return (b < a) ? b : a; }   /* std::min */
+LT_HD float maxc(V3 a) { return fmax_ref(a.x, fmax_ref(a.y, a.z)); }
+
+/* ------------------------------------------------------------------------ */
+/* Cephes single-precision kernels (S. L. Moshier, Cephes Math Library)      */
+/* ------------------------------------------------------------------------ */
+
+/* sin and cos of 2*pi*u, quadrant reduction in turns. */
+LT_D void sincos_turns(float u, float *s, float *c) {
+    float t = u * 4.0f;
+    float q = __builtin_rintf(t);
+    float d = t - q;
+    float r = d * 1.57079637f;
+    float z = r * r;
+    float ps = __builtin_fmaf(__builtin_fmaf(-1.9515295891e-4f, z, 8.3321608736e-3f), z, -1.6666654611e-1f);
+    float sp = __builtin_fmaf(ps, z * r, r);
+    float pc = __builtin_fmaf(__builtin_fmaf(2.443315711809948e-5f, z, -1.388731625493765e-3f), z, 4.166664568298827e-2f);
+    float cp = pc * (z * z);
+    cp = cp - 0.5f * z;
+    cp = cp + 1.0f;
+    int qi = ((int)q) & 3;
+    float ss = (qi == 0) ? sp : (qi == 1) ? cp : (qi == 2) ? -sp : -cp;
+    float cc = (qi == 0) ? cp : (qi == 1) ? -sp : (qi == 2) ? -cp : sp;
+    *s = ss;
+    *c = cc;
+}
+
+LT_D float log_f(float x) {
+    if (!(x > 0.0f)) return x == 0.0f ? -__builtin_huge_valf() : __builtin_nanf("");
+    if (x == __builtin_huge_valf()) return x;
+    int e;
+    float m = frexpf(x, &e);
+    if (m < 0.707106781186547524f) {
+        e -= 1;
+        m = m + m - 1.0f;
+    } else {
+        m = m - 1.0f;
+    }
+    float z = m * m;
+    float p = 7.0376836292e-2f;
+    p = __builtin_fmaf(p, m, -1.1514610310e-1f);
+    p = __builtin_fmaf(p, m, 1.1676998740e-1f);
+    p = __builtin_fmaf(p, m, -1.2420140846e-1f);
+    p = __builtin_fmaf(p, m, 1.4249322787e-1f);
+    p = __builtin_fmaf(p, m, -1.6668057665e-1f);
+    p = __builtin_fmaf(p, m, 2.0000714765e-1f);
+    p = __builtin_fmaf(p, m, -2.4999993993e-1f);
+    p = __builtin_fmaf(p, m, 3.3333331174e-1f);
+    float y = (p * m) * z;
+    float fe = (float)e;
+    y = __builtin_fmaf(fe, -2.12194440e-4f, y);
+    y = y - 0.5f * z;
+    float r = m + y;
+    r = __builtin_fmaf(fe, 0.693359375f, r);
+    return r;
+}
+
+LT_D float exp_f(float x) {
+    if (x != x) return x;
+    if (x > 88.7228317f) return __builtin_huge_valf();
+    if (x < -87.0f) return 0.0f;
+    float z = floorf(__builtin_fmaf(x, 1.44269504088896341f, 0.5f));
+    float r = __builtin_fmaf(-z, 0.693359375f, x);
+    r = __builtin_fmaf(z, 2.12194440e-4f, r);
+    int n = (int)z;
+    float zz = r * r;
+    float p = 1.9875691500e-4f;
+    p = __builtin_fmaf(p, r, 1.3981999507e-3f);
+    p = __builtin_fmaf(p, r, 8.3334519073e-3f);
+    p = __builtin_fmaf(p, r, 4.1665795894e-2f);
+    p = __builtin_fmaf(p, r, 1.6666665459e-1f);
+    p = __builtin_fmaf(p, r, 5.0000001201e-1f);
+    float y = __builtin_fmaf(p, zz, r) + 1.0f;
+    return ldexpf(y, n);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Random numbers                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11). */
+LT_D void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
+                        uint32_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+/* uniform_sampler.h:45-49: 23 random mantissa bits, [0,1). */
+LT_HD float bits_to_unit(uint32_t v) { return __uint_as_float((v >> 9u) | 0x3F800000u) - 1.0f; }
+
+/* Per-path sequential stream: draw i is word (i & 3) of
+ * Philox(ctr = {x, y, i >> 2, stream}, key = {seed, frame}). The path keeps
+ * only i between kernels; the current block is cached in registers. */
+struct PathRng {
+    uint32_t x, y, stream, seed, frame;
+    uint32_t i;
+    uint32_t block;
+    uint32_t buf[4];
+    LT_D void init(uint32_t px, uint32_t py, uint32_t st, uint32_t sd, uint32_t fr, uint32_t start) {
+        x = px;
+        y = py;
+        stream = st;
+        seed = sd;
+        frame = fr;
+        i = start;
+        block = 0xFFFFFFFFu;
+    }
+    LT_D uint32_t next_u32() {
+        uint32_t b = i >> 2;
+        if (b != block) {
+            philox4x32_10(x, y, b, stream, seed, frame, buf);
+            block = b;
+        }
+        uint32_t w = i & 3u;
+        ++i;
+        return w == 0 ? buf[0] : w == 1 ? buf[1] : w == 2 ? buf[2] : buf[3];
+    }
+    LT_D float next_float() { return bits_to_unit(next_u32()); }
+    LT_D uint32_t next_discrete(uint32_t range) { return (uint32_t)((float)range * next_float()); }
+};
+
+/* ------------------------------------------------------------------------ */
+/* Scene tables (device layout)                                              */
+/* ------------------------------------------------------------------------ */
+enum : int { BSDF_LAMBERT = 0, BSDF_MIRROR = 1, BSDF_DIELECTRIC = 2, BSDF_GGX = 3 };
+enum : int { MEDIUM_NON_SCATTERING = 0, MEDIUM_ISOTROPIC = 1, MEDIUM_GRID = 2 };
+enum : int { TEX_CONSTANT = -1, TEX_UV = -2 };
+enum : int { LOBE_NULL = 0, LOBE_SPEC_REFL = 1, LOBE_SPEC_TRANS = 2, LOBE_DIFFUSE = 4, LOBE_GLOSSY = 8, LOBE_SPECULAR = 3 };
+
+struct DMaterial {
+    int32_t bsdf;
+    float albedo[3];
+    int32_t texture;
+    float ior;
+    float roughness;
+    int32_t medium;
+};
+struct DGeom {
+    uint32_t tbase;      /* first global triangle */
+    int32_t material;
+    int32_t light;
+    uint32_t flags;      /* bit0 normals, bit1 uvs */
+};
+struct DLight {
+    float le[3];
+    float area;
+    float sphere[4];
+    uint32_t geom;
+    uint32_t pad[3];
+};
+struct DMedium {
+    int32_t type;
+    float sigma_a[3];
+    float sigma_s;
+    int32_t grid;
+    float scale, albedo, g;
+    float pad[3];
+};
+struct DTexture {
+    uint32_t w, h;
+    uint32_t offset;     /* float offset into the texel pool */
+    uint32_t pad;
+};
+struct DGrid {
+    uint32_t nx, ny, nz;
+    uint32_t offset;     /* float offset into the voxel pool */
+    float bmin[3], bmax[3];
+    float maxd;
+    float pad;
+};
+
+/* Everything a kernel needs to know about one committed scene. */
+struct DScene {
+    const float4 *nodes;       /* BVH2: 4 x float4 per node */
+    const float4 *tris;        /* 3 x float4 per triangle, BVH order */
+    const uint32_t *tri_geom;  /* per global primitive */
+    const uint32_t *tri_vidx;  /* 3 global vertex indices per global primitive */
+    const float *normals;      /* 3 per global vertex */
+    const float *uvs;          /* 2 per global vertex */
+    const DGeom *geoms;
+    const DMaterial *mats;
+    const DLight *lights;
+    const DMedium *media;
+    const DTexture *textures;
+    const float *texels;
+    const DGrid *grids;
+    const float *voxels;
+    uint32_t ntris;
+    uint32_t nlights;
+    float bg[3];
+    uint32_t root_is_empty;
+};
+
+/* Rays and hits as the public ABI lays them out (32 B + 16 B). */
+struct Ray {
+    float ox, oy, oz, tnear;
+    float dx, dy, dz, tfar;
+};
+struct Hit {
+    float t, u, v;
+    uint32_t prim;   /* global primitive index, 0xFFFFFFFF on a miss */
+};
+
+} // namespace lt

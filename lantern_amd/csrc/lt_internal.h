@@ -1,0 +1,127 @@
+/*
+ * lt_internal.h - shared host-side declarations of liblantern_gpu.so.
+ */
+#pragma once
+
+#include "lt_device.h"
+#include "../../include/lantern_gpu.h"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+namespace lt {
+
+int set_error(int code, const std::string &msg);
+int hip_error(hipError_t e, const char *what, const char *file, int line);
+
+#define LT_HIP(expr)                                                              \
+    do {                                                                          \
+        hipError_t _e = (expr);                                                   \
+        if (_e != hipSuccess) return ::lt::hip_error(_e, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+/* Owning device allocation. */
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    /* Allocates exactly n bytes (frees any previous allocation). */
+    int alloc(size_t n);
+    /* Keeps the allocation if it is already at least n bytes. */
+    int reserve(size_t n) { return n <= bytes && p ? 0 : alloc(n); }
+    int upload(const void *src, size_t n, hipStream_t st);
+    template <class T> T *as() const { return (T *)p; }
+};
+
+struct BvhStats {
+    uint32_t num_nodes = 0, num_leaves = 0, max_depth = 0;
+    float sah_cost = 0.0f;
+};
+
+int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, uint32_t n, uint32_t max_leaf,
+              DevBuf &nodes, DevBuf &tris, BvhStats &stats);
+
+/* Traversal launches (lt_trace.hip). count_ptr (device) overrides count when
+ * non-null; work is a device counter that must be zero at launch. */
+int launch_trace_closest(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, const uint32_t *count_ptr,
+                         uint32_t count, uint32_t *work, int block, int grid, int *spill);
+int launch_trace_any(hipStream_t st, const DScene &sc, const Ray *rays, uint32_t *occluded,
+                     const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block, int grid, int *spill);
+int trace_grid(int num_cus, int block);
+size_t trace_spill_bytes(int grid, int block);
+int trace_default_block();
+
+/* Host-side camera setup shared by the render entry points
+ * (pinhole_camera.cpp:27-40, pinhole_camera.h:101-121,
+ *  reconstruction_filter.cpp:106-142). */
+struct CameraSetup {
+    float origin[3];
+    float mx[3], my[3], mz[3];   /* rows of the basis transpose (m_matrixMul*Axis) */
+    float tanx, tany;
+    uint32_t w, h;
+    int32_t filter;
+    float bin_size;
+    float cdf[33];
+};
+void camera_setup(const lt_camera &c, CameraSetup &out);
+
+} // namespace lt
+
+struct lt_device {
+    int ordinal = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 0;
+    /* wavefront workspace, grown on demand */
+    lt::DevBuf ws;
+    size_t ws_paths = 0;
+    lt::DevBuf ctl;          /* queue counts and work counters */
+    lt::DevBuf trace_work;   /* work counter for the intersect entry points */
+    lt::DevBuf spill;        /* traversal stack overflow area */
+    uint32_t *pinned = nullptr;
+};
+
+struct lt_scene {
+    lt_device *dev = nullptr;
+    /* host copies (scene-order, geometry-major) */
+    std::vector<float> pos, nrm, uv;
+    std::vector<uint32_t> vidx;       /* global vertex indices, 3 per global triangle */
+    std::vector<uint32_t> tri_geom;
+    std::vector<lt::DGeom> geoms;
+    std::vector<uint32_t> geom_nv, geom_vbase;
+    std::vector<lt::DMaterial> mats;
+    std::vector<lt::DMedium> media;
+    std::vector<lt::DTexture> texs;
+    std::vector<float> texels;
+    std::vector<lt::DGrid> grids;
+    std::vector<float> voxels;
+    std::vector<lt::DLight> lights;
+    float bg[3] = {0.0f, 0.0f, 0.0f};
+    bool committed = false;
+    /* device copies */
+    lt::DevBuf d_pos, d_nrm, d_uv, d_vidx, d_tri_geom, d_geoms, d_mats, d_media, d_texs, d_texels, d_grids,
+        d_voxels, d_lights, d_nodes, d_tris;
+    lt::BvhStats bvh;
+    float build_ms = 0.0f;
+    lt::DScene ds;
+};
+
+struct lt_film {
+    lt_device *dev = nullptr;
+    uint32_t w = 0, h = 0;
+    lt::DevBuf sum, bounces, count;
+};

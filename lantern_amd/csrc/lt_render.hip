@@ -1,0 +1,704 @@
+/*
+ * lt_render.hip - wavefront path tracing on gfx950.
+ *
+ * Replaces Integrator::RenderFrame (integrator.cpp:28-43), whose TBB
+ * parallel_for over 8x8 tiles runs RenderPixel (integrator.cpp:98-250) one
+ * path per thread with up to three Embree closest-hit queries per bounce.
+ * Here the bounce loop is split at its three trace points:
+ *
+ *   generate      camera rays for (pixel, sample) paths          pinhole_camera.cpp:88-113
+ *   per bounce b:
+ *     extend      closest hit of every live path's ray             scene.cpp:116-121
+ *     shade       miss/background, medium step, emission, the
+ *                 random decisions of SampleOneLight/EstimateDirect,
+ *                 the continuation BSDF sample and Russian roulette integrator.cpp:119-242, 252-315
+ *                 -> compacted queue of surviving paths (ballot + one atomic per wave)
+ *                 -> two next-event rays per shaded path (light sample, BSDF sample)
+ *     extend      closest hit of the next-event rays               area_light.cpp:58, 100
+ *     resolve     light pdfs at the hits, MIS weights, color += T * N * direct
+ *   accumulate    film += path colors in sample order               integrator.cpp:245-249
+ *
+ * Every random decision of a path is drawn in the order RenderPixel draws it,
+ * from the path's own counter-based stream (Philox keyed by seed and frame,
+ * countered by pixel and draw index), so a path's result does not depend on
+ * which thread or queue slot processed it. The three ray casts consume no
+ * random numbers, which is what makes the split exact.
+ *
+ * Kernels read the live-path count from device memory and loop over it with a
+ * fixed grid, so a whole wave of bounces can be enqueued (or captured in a
+ * hipGraph) without host round trips.
+ */
+#include "lt_internal.h"
+#include "lt_shading.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+namespace lt {
+
+namespace {
+
+/* Control block in device memory. */
+struct Ctl {
+    uint32_t count[2048];      /* live paths entering bounce b */
+    uint32_t nee_count[2048];  /* next-event rays written at bounce b (2 per shaded slot) */
+    uint32_t work[4096];       /* traversal work counters, one per launch */
+    unsigned long long shadow_rays;
+    unsigned long long pad[7];
+};
+
+struct CameraArgs {
+    float origin[3];
+    float mx[3], my[3], mz[3];
+    float tanx, tany;
+    uint32_t w, h;
+    int32_t filter;
+    float bin_size;
+    float cdf[33];
+};
+
+/* Path state, SoA, indexed by path id. */
+struct PathState {
+    float *tx, *ty, *tz;      /* throughput */
+    float *cx, *cy, *cz;      /* radiance accumulated so far */
+    float *iori;
+    int *medium;
+    uint32_t *lobe;
+    uint32_t *rngi;           /* draws consumed */
+    uint32_t *bounces;        /* final bounce count */
+};
+/* One queue of rays (indexed by slot) with their owners. */
+struct Queue {
+    Ray *rays;
+    uint32_t *path;
+};
+/* Next-event record, indexed by the shaded slot. */
+struct NeeRec {
+    float *tx, *ty, *tz;      /* throughput at the shading point */
+    float *f1x, *f1y, *f1z, *p1;
+    float *f2x, *f2y, *f2z, *p2;
+    uint32_t *info;           /* light index | flag bits 30 (light sample) and 31 (BSDF sample) */
+    float *nl;
+};
+
+struct WaveArgs {
+    DScene sc;
+    CameraArgs cam;
+    PathState ps;
+    uint32_t npix;            /* pixels per sample */
+    uint32_t sample_base;     /* frame offset of path id 0 of this wave */
+    uint32_t frame0;
+    uint32_t seed;
+    uint32_t stream;
+    uint32_t max_depth;
+    uint32_t rr_start;
+};
+
+__device__ __forceinline__ void path_coords(const WaveArgs &a, uint32_t p, uint32_t &x, uint32_t &y, uint32_t &frame) {
+    uint32_t s = p / a.npix;
+    uint32_t pix = p - s * a.npix;
+    y = pix / a.cam.w;
+    x = pix - y * a.cam.w;
+    frame = a.frame0 + a.sample_base + s;
+}
+
+/* ReconstructionFilter::Sample (reconstruction_filter.cpp:34-87). */
+__device__ __forceinline__ float filter_sample(const CameraArgs &c, float x) {
+    if (c.filter == 1) return 0.0f;
+    if (c.filter == 2) return x - 0.5f;
+    bool neg = x < 0.5f;
+    x = neg ? x * 2.0f : (x - 0.5f) * 2.0f;
+    uint32_t index = 32;
+    for (uint32_t i = 1; i < 33; ++i) {
+        if (x < c.cdf[i]) {
+            index = i - 1;
+            break;
+        }
+    }
+    float pdf = c.cdf[index + 1] - c.cdf[index];
+    float u = c.bin_size * ((float)index + ((x - c.cdf[index]) / pdf));
+    return neg ? -u : u;
+}
+
+__global__ void k_generate(WaveArgs a, Queue q, uint32_t n, Ctl *ctl) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        uint32_t x, y, frame;
+        path_coords(a, p, x, y, frame);
+        PathRng rng;
+        rng.init(x, y, a.stream, a.seed, frame, 0);
+        float fu = rng.next_float(), fv = rng.next_float();
+        float u = filter_sample(a.cam, fu), v = filter_sample(a.cam, fv);
+        V3 view = v3((((float)x + 0.5f + u) / (float)a.cam.w * 2.0f - 1.0f) * a.cam.tanx,
+                     -((((float)y + 0.5f + v) / (float)a.cam.h) * 2.0f - 1.0f) * a.cam.tany, -1.0f);
+        V3 mx = v3(a.cam.mx[0], a.cam.mx[1], a.cam.mx[2]);
+        V3 my = v3(a.cam.my[0], a.cam.my[1], a.cam.my[2]);
+        V3 mz = v3(a.cam.mz[0], a.cam.mz[1], a.cam.mz[2]);
+        float4 r0 = make_float4(a.cam.origin[0], a.cam.origin[1], a.cam.origin[2], 0.0f);
+        float4 r1 = make_float4(dot(view, mx), dot(view, my), dot(view, mz), __builtin_huge_valf());
+        float4 *rp = (float4 *)(q.rays + p);
+        rp[0] = r0;
+        rp[1] = r1;
+        q.path[p] = p;
+        a.ps.tx[p] = 1.0f; a.ps.ty[p] = 1.0f; a.ps.tz[p] = 1.0f;
+        a.ps.cx[p] = 0.0f; a.ps.cy[p] = 0.0f; a.ps.cz[p] = 0.0f;
+        a.ps.iori[p] = 1.0f;
+        a.ps.medium[p] = -1;
+        a.ps.lobe[p] = LOBE_NULL;
+        a.ps.rngi[p] = rng.i;
+        a.ps.bounces[p] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->count[0] = n;
+}
+
+__device__ __forceinline__ void store_ray(Ray *dst, V3 o, V3 d, float tnear, float tfar) {
+    float4 *rp = (float4 *)dst;
+    rp[0] = make_float4(o.x, o.y, o.z, tnear);
+    rp[1] = make_float4(d.x, d.y, d.z, tfar);
+}
+
+/* One bounce of RenderPixel for every live path (integrator.cpp:119-242). */
+__global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next,
+                                               NeeRec nee, Ray *nee_rays, Ctl *ctl) {
+    const uint32_t count = ctl->count[b];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->nee_count[b] = 2 * count;
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const DScene &sc = a.sc;
+    const float inf = __builtin_huge_valf();
+    for (uint32_t base = wave * 64; base < count; base += nwaves * 64) {
+        const uint32_t q = base + lane;
+        bool alive = false;
+        bool nee1 = false, nee2 = false;
+        uint32_t p = 0;
+        V3 new_o = v3s(0.0f), new_d = v3s(0.0f);
+        if (q < count) {
+            p = cur.path[q];
+            uint32_t x, y, frame;
+            path_coords(a, p, x, y, frame);
+            PathRng rng;
+            rng.init(x, y, a.stream, a.seed, frame, a.ps.rngi[p]);
+            const float4 *rp = (const float4 *)(cur.rays + q);
+            float4 ra = rp[0], rb = rp[1];
+            const Hit h = hits[q];
+            V3 T = v3(a.ps.tx[p], a.ps.ty[p], a.ps.tz[p]);
+            V3 C = v3(a.ps.cx[p], a.ps.cy[p], a.ps.cz[p]);
+            Interaction it;
+            it.iori = a.ps.iori[p];
+            it.lobe = (int)a.ps.lobe[p];
+            int medium = a.ps.medium[p];
+            bool dead = false;
+            uint32_t nee_info = 0;
+            V3 P = v3s(0.0f), wi1 = v3s(0.0f), wi2 = v3s(0.0f);
+            if (h.prim == 0xFFFFFFFFu) {
+                C = C + T * v3(sc.bg[0], sc.bg[1], sc.bg[2]);
+                dead = true;
+            } else {
+                V3 origin = v3(ra.x, ra.y, ra.z);
+                V3 dir = normalize(v3(rb.x, rb.y, rb.z));
+                bool surface = true;
+                if (medium >= 0) {                                                   /* integrator.cpp:131-163 */
+                    const DMedium md = sc.media[medium];
+                    if (md.type == MEDIUM_GRID) {
+                        const DGrid g = sc.grids[md.grid];
+                        float sbar = md.scale * g.maxd;
+                        float t = 0.0f;
+                        bool collided = false;
+                        if (sbar > 0.0f) {
+                            while (true) {
+                                t = t - log_f(1.0f - rng.next_float()) / sbar;
+                                if (!(t < h.t)) break;
+                                V3 qp = origin + dir * t;
+                                float dens = grid_density(sc, g, qp.x, qp.y, qp.z);
+                                if (rng.next_float() * sbar < md.scale * dens) {
+                                    collided = true;
+                                    break;
+                                }
+                            }
+                        }
+                        if (collided) {
+                            surface = false;
+                            T = T * md.albedo;
+                            new_o = origin + dir * t;
+                            new_d = sample_hg(md.g, dir, rng);
+                        }
+                    } else {
+                        float dist = h.t;
+                        if (md.type == MEDIUM_ISOTROPIC) {                           /* isotropic_scattering_medium.h:31-40 */
+                            float d = -log_f(rng.next_float()) / md.sigma_s;
+                            dist = d >= h.t ? h.t : d;
+                        }
+                        V3 na = -v3(md.sigma_a[0], md.sigma_a[1], md.sigma_a[2]);
+                        V3 tr = v3(exp_f(na.x * dist), exp_f(na.y * dist), exp_f(na.z * dist));
+                        T = (T * 1.0f) * tr;
+                        if (dist < h.t) {
+                            surface = false;
+                            new_o = origin + dir * dist;
+                            new_d = uniform_sphere(rng);
+                        }
+                    }
+                }
+                if (surface) {
+                    const uint32_t gi = sc.tri_geom[h.prim];
+                    const DGeom g = sc.geoms[gi];
+                    const DMaterial m = sc.mats[g.material];
+                    const int light = g.light;
+                    if ((b == 0 || (it.lobe & LOBE_SPECULAR) != 0) && light >= 0) {
+                        const DLight L = sc.lights[light];
+                        C = C + T * v3(L.le[0], L.le[1], L.le[2]);
+                    }
+                    it.P = origin + dir * h.t;
+                    if (g.flags & 1u) it.N = normalize(interp_normal(sc, h.prim, h.u, h.v));
+                    else it.N = normalize(v3s(0.0f));
+                    if (g.flags & 2u) interp_uv(sc, h.prim, h.u, h.v, it.tu, it.tv);
+                    else it.tu = it.tv = 0.0f;
+                    it.wo = -dir;
+                    it.ioro = 0.0f;
+                    P = it.P;
+
+                    /* SampleOneLight / EstimateDirect on a copy of the interaction
+                     * (integrator.cpp:252-315); the two casts are deferred. */
+                    const uint32_t nl = sc.nlights;
+                    if (!(nl == 0 || (nl == 1 && light >= 0))) {
+                        uint32_t li;
+                        do {
+                            li = rng.next_discrete(nl);
+                        } while ((int)li == light);
+                        const DLight L = sc.lights[li];
+                        Interaction ic = it;
+                        V3 f1 = v3s(0.0f), f2 = v3s(0.0f);
+                        float p1 = 0.0f, p2 = 0.0f;
+                        if ((supported_lobes(m.bsdf) & ~LOBE_SPECULAR) != 0) {
+                            /* AreaLight::SampleLi up to its cast (area_light.cpp:18-56) */
+                            float dx, dy;
+                            sample_disc(rng, L.sphere[3], dx, dy);
+                            V3 cdir = v3(L.sphere[0], L.sphere[1], L.sphere[2]) - ic.P;
+                            float dist = length(cdir);
+                            V3 cn = normalize(cdir);
+                            V3 pt = rotate_to_world(dx, dy, dist - L.sphere[3], cn);
+                            V3 ldir = normalize(pt);
+                            ic.wi = ldir;
+                            if (dot(ldir, ic.N) > 0.0f) {
+                                f1 = bsdf_eval(sc, m, ic);
+                                p1 = bsdf_pdf(m, ic);
+                                nee1 = true;
+                                wi1 = ldir;
+                            }
+                        }
+                        bsdf_sample(m, ic, rng);
+                        f2 = bsdf_eval(sc, m, ic);
+                        p2 = bsdf_pdf(m, ic);
+                        if (p2 != 0.0f && !is_black(f2) && dot(ic.wi, ic.N) > 0.0f) {  /* AreaLight::PdfLi horizon test */
+                            nee2 = true;
+                            wi2 = ic.wi;
+                        }
+                        if (nee1 || nee2) {
+                            nee.tx[q] = T.x; nee.ty[q] = T.y; nee.tz[q] = T.z;
+                            nee.f1x[q] = f1.x; nee.f1y[q] = f1.y; nee.f1z[q] = f1.z; nee.p1[q] = p1;
+                            nee.f2x[q] = f2.x; nee.f2y[q] = f2.y; nee.f2z[q] = f2.z; nee.p2[q] = p2;
+                            nee.nl[q] = (float)nl;
+                            nee_info = li | (nee1 ? 0x40000000u : 0u) | (nee2 ? 0x80000000u : 0u);
+                        }
+                    }
+                    /* continuation (integrator.cpp:199-231) */
+                    bsdf_sample(m, it, rng);
+                    float pdf = bsdf_pdf(m, it);
+                    if (pdf == 0.0f) {
+                        dead = true;   /* the reference divides 0/0 here */
+                    } else {
+                        T = (T * bsdf_eval(sc, m, it)) / pdf;
+                        if (it.lobe == LOBE_SPEC_TRANS) {
+                            it.iori = it.ioro;
+                            medium = m.medium;
+                        }
+                        new_o = it.P;
+                        new_d = it.wi;
+                    }
+                }
+                if (!dead && b >= a.rr_start) {                                      /* integrator.cpp:235-242 */
+                    float pr = maxc(T);
+                    if (rng.next_float() > pr || !(pr > 0.0f)) dead = true;
+                    else T = T * (1.0f / pr);
+                }
+            }
+            nee.info[q] = nee_info;
+            store_ray(nee_rays + 2 * (size_t)q, P, wi1, 0.001f, nee1 ? inf : -1.0f);
+            store_ray(nee_rays + 2 * (size_t)q + 1, P, wi2, 0.001f, nee2 ? inf : -1.0f);
+            alive = !dead && (b + 1 < a.max_depth);
+            if (!alive) a.ps.bounces[p] = dead ? b : b + 1;
+            a.ps.tx[p] = T.x; a.ps.ty[p] = T.y; a.ps.tz[p] = T.z;
+            a.ps.cx[p] = C.x; a.ps.cy[p] = C.y; a.ps.cz[p] = C.z;
+            a.ps.iori[p] = it.iori;
+            a.ps.medium[p] = medium;
+            a.ps.lobe[p] = (uint32_t)it.lobe;
+            a.ps.rngi[p] = rng.i;
+        }
+        /* wave-level compaction of the survivors */
+        unsigned long long mask = __ballot(alive);
+        unsigned long long nmask = __ballot(nee1);
+        unsigned long long nmask2 = __ballot(nee2);
+        uint32_t slot0 = 0;
+        if (lane == 0) {
+            if (mask) slot0 = atomicAdd(&ctl->count[b + 1], (uint32_t)__popcll(mask));
+            uint32_t ns = (uint32_t)(__popcll(nmask) + __popcll(nmask2));
+            if (ns) atomicAdd(&ctl->shadow_rays, (unsigned long long)ns);
+        }
+        slot0 = __shfl(slot0, 0);
+        if (alive) {
+            uint32_t off = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+            uint32_t s = slot0 + off;
+            next.path[s] = p;
+            store_ray(next.rays + s, new_o, new_d, 0.001f, inf);
+        }
+    }
+}
+
+/* The casts of AreaLight::SampleLi / PdfLi resolved (area_light.cpp:58-71,
+ * 100-109) and the MIS sums of EstimateDirect (integrator.cpp:287-314). */
+__global__ __launch_bounds__(256) void k_resolve(WaveArgs a, uint32_t b, Queue cur, NeeRec nee, const Ray *nee_rays,
+                                                 const Hit *nee_hits, Ctl *ctl) {
+    const uint32_t count = ctl->count[b];
+    const DScene &sc = a.sc;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
+        uint32_t info = nee.info[q];
+        if (!(info & 0xC0000000u)) continue;
+        uint32_t li = info & 0x3FFFFFFFu;
+        const DLight L = sc.lights[li];
+        V3 le = v3(L.le[0], L.le[1], L.le[2]);
+        V3 direct = v3s(0.0f);
+        if (info & 0x40000000u) {
+            const Hit h = nee_hits[2 * (size_t)q];
+            if (h.prim != 0xFFFFFFFFu && sc.tri_geom[h.prim] == L.geom) {
+                const Ray r = nee_rays[2 * (size_t)q];
+                V3 o = v3(r.ox, r.oy, r.oz), d = v3(r.dx, r.dy, r.dz);
+                V3 ip = o + normalize(d) * h.t;
+                V3 dd = ip - o;
+                float d2 = dot(dd, dd);
+                float lpdf = d2 / (fabsf(dot(normalize(interp_normal(sc, h.prim, h.u, h.v)), -d)) * L.area);
+                V3 f1 = v3(nee.f1x[q], nee.f1y[q], nee.f1z[q]);
+                float p1 = nee.p1[q];
+                if (lpdf != 0.0f && !is_black(le) && p1 != 0.0f && !is_black(f1)) {
+                    float w = power_heuristic(lpdf, p1);
+                    direct = direct + ((f1 * le) * w) / lpdf;
+                }
+            }
+        }
+        if (info & 0x80000000u) {
+            const Hit h = nee_hits[2 * (size_t)q + 1];
+            if (h.prim != 0xFFFFFFFFu && sc.tri_geom[h.prim] == L.geom) {
+                const Ray r = nee_rays[2 * (size_t)q + 1];
+                V3 o = v3(r.ox, r.oy, r.oz), d = v3(r.dx, r.dy, r.dz);
+                V3 ip = o + normalize(d) * h.t;
+                V3 dd = ip - o;
+                float d2 = dot(dd, dd);
+                float lpdf = d2 / (fabsf(dot(normalize(interp_normal(sc, h.prim, h.u, h.v)), d)) * L.area);
+                if (lpdf != 0.0f) {
+                    float p2 = nee.p2[q];
+                    V3 f2 = v3(nee.f2x[q], nee.f2y[q], nee.f2z[q]);
+                    float w = power_heuristic(p2, lpdf);
+                    direct = direct + ((f2 * le) * w) / p2;
+                }
+            }
+        }
+        const uint32_t p = cur.path[q];
+        V3 T = v3(nee.tx[q], nee.ty[q], nee.tz[q]);
+        V3 add = T * (nee.nl[q] * direct);
+        a.ps.cx[p] = a.ps.cx[p] + add.x;
+        a.ps.cy[p] = a.ps.cy[p] + add.y;
+        a.ps.cz[p] = a.ps.cz[p] + add.z;
+    }
+}
+
+/* FrameBuffer accumulation in frame order (integrator.cpp:245-249). */
+__global__ void k_accumulate(WaveArgs a, uint32_t nsamples, float *sum, uint32_t *bounces, uint32_t *count) {
+    for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < a.npix; pix += gridDim.x * blockDim.x) {
+        float sx = sum[3 * (size_t)pix], sy = sum[3 * (size_t)pix + 1], sz = sum[3 * (size_t)pix + 2];
+        uint32_t bb = bounces[pix], cc = count[pix];
+        for (uint32_t s = 0; s < nsamples; ++s) {
+            size_t p = (size_t)s * a.npix + pix;
+            sx = sx + a.ps.cx[p];
+            sy = sy + a.ps.cy[p];
+            sz = sz + a.ps.cz[p];
+            bb += a.ps.bounces[p];
+            cc += 1u;
+        }
+        sum[3 * (size_t)pix] = sx;
+        sum[3 * (size_t)pix + 1] = sy;
+        sum[3 * (size_t)pix + 2] = sz;
+        bounces[pix] = bb;
+        count[pix] = cc;
+    }
+}
+
+__global__ void k_reset_ctl(Ctl *ctl) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 2048) {
+        ctl->count[i] = 0;
+        ctl->nee_count[i] = 0;
+    }
+    if (i < 4096) ctl->work[i] = 0;
+    if (i == 0) ctl->shadow_rays = 0;
+}
+
+/* Bytes of workspace per path in flight. */
+constexpr size_t kPathBytes = 4 * 11;                 /* path state */
+constexpr size_t kQueueBytes = sizeof(Ray) + 4;       /* per queue slot */
+constexpr size_t kNeeBytes = 4 * 14 + 2 * sizeof(Ray) + 2 * sizeof(Hit);
+constexpr size_t kBytesPerPath = kPathBytes + 2 * kQueueBytes + sizeof(Hit) + kNeeBytes;
+
+struct Workspace {
+    PathState ps;
+    Queue qa, qb;
+    Hit *hits;
+    NeeRec nee;
+    Ray *nee_rays;
+    Hit *nee_hits;
+};
+
+int carve(lt_device *dev, size_t n, Workspace &w) {
+    size_t bytes = n * kBytesPerPath + 64 * 64;
+    if (dev->ws_paths < n) {
+        int rc = dev->ws.alloc(bytes);
+        if (rc) return rc;
+        dev->ws_paths = n;
+    }
+    char *p = (char *)dev->ws.p;
+    auto take = [&](size_t each) {
+        char *r = p;
+        p += (n * each + 255) & ~(size_t)255;
+        return (void *)r;
+    };
+    w.ps.tx = (float *)take(4); w.ps.ty = (float *)take(4); w.ps.tz = (float *)take(4);
+    w.ps.cx = (float *)take(4); w.ps.cy = (float *)take(4); w.ps.cz = (float *)take(4);
+    w.ps.iori = (float *)take(4);
+    w.ps.medium = (int *)take(4);
+    w.ps.lobe = (uint32_t *)take(4);
+    w.ps.rngi = (uint32_t *)take(4);
+    w.ps.bounces = (uint32_t *)take(4);
+    w.qa.rays = (Ray *)take(sizeof(Ray)); w.qa.path = (uint32_t *)take(4);
+    w.qb.rays = (Ray *)take(sizeof(Ray)); w.qb.path = (uint32_t *)take(4);
+    w.hits = (Hit *)take(sizeof(Hit));
+    w.nee.tx = (float *)take(4); w.nee.ty = (float *)take(4); w.nee.tz = (float *)take(4);
+    w.nee.f1x = (float *)take(4); w.nee.f1y = (float *)take(4); w.nee.f1z = (float *)take(4); w.nee.p1 = (float *)take(4);
+    w.nee.f2x = (float *)take(4); w.nee.f2y = (float *)take(4); w.nee.f2z = (float *)take(4); w.nee.p2 = (float *)take(4);
+    w.nee.info = (uint32_t *)take(4);
+    w.nee.nl = (float *)take(4);
+    w.nee_rays = (Ray *)take(2 * sizeof(Ray));
+    w.nee_hits = (Hit *)take(2 * sizeof(Hit));
+    return LT_OK;
+}
+
+struct EventTimer {
+    bool on;
+    hipStream_t st;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[6];
+    explicit EventTimer(bool enable, hipStream_t s) : on(enable), st(s) {}
+    ~EventTimer() {
+        for (auto &v : ev)
+            for (auto &e : v) {
+                (void)hipEventDestroy(e.first);
+                (void)hipEventDestroy(e.second);
+            }
+    }
+    int begin(int k) {
+        if (!on) return 0;
+        hipEvent_t a, b;
+        LT_HIP(hipEventCreate(&a));
+        LT_HIP(hipEventCreate(&b));
+        ev[k].push_back({a, b});
+        LT_HIP(hipEventRecord(a, st));
+        return 0;
+    }
+    int end(int k) {
+        if (!on) return 0;
+        LT_HIP(hipEventRecord(ev[k].back().second, st));
+        return 0;
+    }
+    double total(int k) {
+        double t = 0;
+        for (auto &e : ev[k]) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) t += ms;
+        }
+        return t;
+    }
+};
+
+enum { T_GEN = 0, T_EXT = 1, T_SHADE = 2, T_SHADOW = 3, T_RESOLVE = 4, T_ACC = 5 };
+
+} // namespace
+
+void camera_setup(const lt_camera &c, CameraSetup &o) {
+    /* pinhole_camera.cpp:27-40 and pinhole_camera.h:101-121 (host libm) */
+    V3 target = v3(c.target[0], c.target[1], c.target[2]);
+    float x = c.radius * sinf(c.phi) * sinf(c.theta);
+    float y = c.radius * cosf(c.phi);
+    float z = c.radius * sinf(c.phi) * cosf(c.theta);
+    V3 origin = target + v3(x, y, z);
+    V3 za = normalize(origin - target);
+    V3 xa;
+    if (za.x == 0.0f && (za.y == 1.0f || za.y == -1.0f) && za.z == 0.0f) xa = normalize(cross(v3(0.0f, 0.0f, 1.0f), za));
+    else xa = normalize(cross(v3(0.0f, 1.0f, 0.0f), za));
+    V3 ya = cross(za, xa);
+    o.origin[0] = origin.x; o.origin[1] = origin.y; o.origin[2] = origin.z;
+    o.mx[0] = xa.x; o.mx[1] = ya.x; o.mx[2] = za.x;
+    o.my[0] = xa.y; o.my[1] = ya.y; o.my[2] = za.y;
+    o.mz[0] = xa.z; o.mz[1] = ya.z; o.mz[2] = za.z;
+    o.tanx = tanf(c.fov * 0.5f);
+    o.tany = tanf(c.fov * 0.5f * (float)c.height / (float)c.width);
+    o.w = c.width;
+    o.h = c.height;
+    o.filter = c.filter;
+    /* ReconstructionFilter::PreCompute, tent of width 1 (reconstruction_filter.cpp:106-142) */
+    const float width = 1.0f;
+    o.bin_size = width / (32 - 1);
+    float pdf[32], norm = 0.0f;
+    for (uint32_t i = 0; i < 31; ++i) {
+        float t = width - fabsf((float)i * o.bin_size);
+        pdf[i] = t;
+        norm += t;
+    }
+    pdf[31] = 0.0f;
+    for (uint32_t i = 0; i < 31; ++i) pdf[i] /= norm;
+    o.cdf[0] = 0.0f;
+    for (uint32_t i = 1; i < 32; ++i) o.cdf[i] = o.cdf[i - 1] + pdf[i - 1];
+    o.cdf[32] = 1.0f;
+}
+
+/* Renders one scene; see the file comment for the kernel sequence. */
+int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &prm, lt_film *film,
+                 lt_render_stats *stats) {
+    lt_device *dev = scene->dev;
+    hipStream_t st = dev->stream;
+    const uint32_t npix = cam.width * cam.height;
+    if (npix == 0 || prm.spp == 0) return LT_OK;
+    if (prm.max_depth == 0 || prm.max_depth > 2000) return set_error(LT_E_INVALID, "max_depth must be in 1..2000");
+    size_t budget = prm.wave_paths ? prm.wave_paths : (size_t)16 << 20;
+    uint32_t spw = (uint32_t)std::max<size_t>(1, std::min<size_t>(prm.spp, budget / npix));
+    size_t nmax = (size_t)spw * npix;
+    if (nmax > 0xFFFFFFFFull / 2) return set_error(LT_E_INVALID, "wave too large");
+    Workspace w;
+    int rc = carve(dev, nmax, w);
+    if (rc) return rc;
+    if ((rc = dev->ctl.reserve(sizeof(Ctl)))) return rc;
+    Ctl *ctl = (Ctl *)dev->ctl.p;
+    const int tblock = prm.block_size ? (int)prm.block_size : trace_default_block();
+    const int tgrid = trace_grid(dev->num_cus, tblock);
+    if ((rc = dev->spill.reserve(trace_spill_bytes(tgrid, tblock)))) return rc;
+    int *spill = (int *)dev->spill.p;
+
+    CameraSetup cs;
+    camera_setup(cam, cs);
+    WaveArgs a;
+    a.sc = scene->ds;
+    std::memcpy(&a.cam, &cs, sizeof(CameraArgs));
+    a.ps = w.ps;
+    a.npix = npix;
+    a.frame0 = prm.frame0;
+    a.seed = prm.seed;
+    a.stream = prm.stream;
+    a.max_depth = prm.max_depth;
+    a.rr_start = prm.rr_start;
+
+    const int sgrid = std::max(1, dev->num_cus) * 8;
+    EventTimer tm(stats != nullptr, st);
+    hipEvent_t e0, e1;
+    LT_HIP(hipEventCreate(&e0));
+    LT_HIP(hipEventCreate(&e1));
+    LT_HIP(hipEventRecord(e0, st));
+    uint64_t closest = 0, shadow = 0, segments = 0;
+    uint32_t ext_launches = 0, sh_launches = 0, waves = 0;
+    for (uint32_t s0 = 0; s0 < prm.spp; s0 += spw) {
+        uint32_t ns = std::min(spw, prm.spp - s0);
+        uint32_t n = ns * npix;
+        a.sample_base = s0;
+        ++waves;
+        hipLaunchKernelGGL(k_reset_ctl, dim3(16), dim3(256), 0, st, ctl);
+        tm.begin(T_GEN);
+        hipLaunchKernelGGL(k_generate, dim3(std::min<uint32_t>((n + 255) / 256, 65535)), dim3(256), 0, st, a, w.qa, n,
+                           ctl);
+        tm.end(T_GEN);
+        LT_HIP(hipGetLastError());
+        Queue cur = w.qa, nxt = w.qb;
+        uint32_t wslot = 0;
+        for (uint32_t b = 0; b < prm.max_depth; ++b) {
+            if (b > 0 && (b % 8) == 0) {
+                /* stop early once every path of the wave has terminated */
+                uint32_t live = 0;
+                LT_HIP(hipMemcpyAsync(&live, &ctl->count[b], sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+                LT_HIP(hipStreamSynchronize(st));
+                if (live == 0) break;
+            }
+            if (wslot + 2 > 4096) return set_error(LT_E_STATE, "too many traversal launches in one wave");
+            tm.begin(T_EXT);
+            if ((rc = launch_trace_closest(st, a.sc, cur.rays, w.hits, &ctl->count[b], 0, &ctl->work[wslot++], tblock,
+                                           tgrid, spill)))
+                return rc;
+            tm.end(T_EXT);
+            ++ext_launches;
+            tm.begin(T_SHADE);
+            hipLaunchKernelGGL(k_shade, dim3(sgrid), dim3(256), 0, st, a, b, cur, (const Hit *)w.hits, nxt, w.nee,
+                               w.nee_rays, ctl);
+            tm.end(T_SHADE);
+            LT_HIP(hipGetLastError());
+            /* next-event rays: two per shaded slot; inactive ones have tfar < 0 */
+            tm.begin(T_SHADOW);
+            if ((rc = launch_trace_closest(st, a.sc, w.nee_rays, w.nee_hits, &ctl->nee_count[b], 0,
+                                           &ctl->work[wslot++], tblock, tgrid, spill)))
+                return rc;
+            tm.end(T_SHADOW);
+            ++sh_launches;
+            tm.begin(T_RESOLVE);
+            hipLaunchKernelGGL(k_resolve, dim3(sgrid), dim3(256), 0, st, a, b, cur, w.nee, (const Ray *)w.nee_rays,
+                               (const Hit *)w.nee_hits, ctl);
+            tm.end(T_RESOLVE);
+            LT_HIP(hipGetLastError());
+            std::swap(cur, nxt);
+        }
+        tm.begin(T_ACC);
+        hipLaunchKernelGGL(k_accumulate, dim3(std::min<uint32_t>((npix + 255) / 256, 65535)), dim3(256), 0, st, a, ns,
+                           film->sum.as<float>(), film->bounces.as<uint32_t>(), film->count.as<uint32_t>());
+        tm.end(T_ACC);
+        LT_HIP(hipGetLastError());
+        if (stats) {
+            std::vector<uint32_t> counts(prm.max_depth + 1);
+            unsigned long long sh = 0;
+            LT_HIP(hipMemcpyAsync(counts.data(), ctl->count, sizeof(uint32_t) * (prm.max_depth + 1),
+                                  hipMemcpyDeviceToHost, st));
+            LT_HIP(hipMemcpyAsync(&sh, &ctl->shadow_rays, sizeof sh, hipMemcpyDeviceToHost, st));
+            LT_HIP(hipStreamSynchronize(st));
+            for (uint32_t b = 0; b < prm.max_depth; ++b) {
+                closest += counts[b];
+                segments += counts[b];
+            }
+            shadow += sh;
+        }
+    }
+    LT_HIP(hipEventRecord(e1, st));
+    LT_HIP(hipEventSynchronize(e1));
+    float total = 0;
+    LT_HIP(hipEventElapsedTime(&total, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        stats->samples = (uint64_t)npix * prm.spp;
+        stats->closest_rays = closest;
+        stats->shadow_rays = shadow;
+        stats->path_segments = segments;
+        stats->total_ms = total;
+        stats->generate_ms = tm.total(T_GEN);
+        stats->extend_ms = tm.total(T_EXT);
+        stats->shade_ms = tm.total(T_SHADE);
+        stats->shadow_ms = tm.total(T_SHADOW);
+        stats->resolve_ms = tm.total(T_RESOLVE);
+        stats->accumulate_ms = tm.total(T_ACC);
+        stats->extend_launches = ext_launches;
+        stats->shadow_launches = sh_launches;
+        stats->waves = waves;
+    }
+    return LT_OK;
+}
+
+} // namespace lt

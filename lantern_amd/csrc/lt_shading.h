@@ -1,0 +1,223 @@
+/*
+ * lt_shading.h - device-side restatement of Lantern's per-hit estimator
+ * pieces: sampling (math/sampling.h), vector helpers (math/vector_math.h/.cpp),
+ * BSDFs (the headers under materials/bsdfs), textures (materials/textures and
+ * scene/image_cache.cpp), media (materials/media) and attribute interpolation
+ * (embree kernels/common/scene_triangle_mesh.cpp:214-260).
+ */
+#pragma once
+
+#include "lt_device.h"
+
+namespace lt {
+
+/* SurfaceInteraction (integrator/surface_interaction.h:16-31). */
+struct Interaction {
+    V3 P, N, wi, wo;
+    float tu, tv;
+    int lobe;
+    float iori, ioro;
+};
+
+/* vector_math.cpp:12-33 */
+LT_D V3 rotate_to_world(float x, float y, float z, const V3 &n) {
+    V3 major;
+    const float k = 0.57735026919f;
+    if (fabsf(n.x) < k) major = v3(1.0f, 0.0f, 0.0f);
+    else if (fabsf(n.y) < k) major = v3(0.0f, 1.0f, 0.0f);
+    else major = v3(0.0f, 0.0f, 1.0f);
+    V3 u = normalize(cross(n, major));
+    V3 v = cross(n, u);
+    return (u * x + v * y) + n * z;
+}
+
+/* sampling.h:31-38 */
+LT_D void sample_disc(PathRng &s, float radius, float &x, float &y) {
+    float r = sqrtf(s.next_float()) * radius;
+    float sn, cs;
+    sincos_turns(s.next_float(), &sn, &cs);
+    x = r * cs;
+    y = r * sn;
+}
+/* sampling.h:50-60 */
+LT_D V3 cosine_hemisphere(const V3 &n, PathRng &s) {
+    float x, y;
+    sample_disc(s, 1.0f, x, y);
+    float z = sqrtf(1.0f - x * x - y * y);
+    return normalize(rotate_to_world(x, y, z, n));
+}
+/* sampling.h:74-84 */
+LT_D V3 uniform_sphere(PathRng &s) {
+    float cp = 2.0f * s.next_float() - 1.0f;
+    float sp = sqrtf(1.0f - cp * cp);
+    float sn, cs;
+    sincos_turns(s.next_float(), &sn, &cs);
+    return v3(sp * sn, cp, sp * cs);
+}
+/* sampling.h:16-21 with numf = numg = 1 */
+LT_D float power_heuristic(float f, float g) { return (f * f) / (f * f + g * g); }
+/* vector_math.h:33-56 */
+LT_D V3 reflect(const V3 &V, const V3 &N) { return (2.0f * dot(V, N)) * N - V; }
+LT_D V3 refract(const V3 &V, const V3 &N, float VdotN, float eta, float s2) {
+    return (eta * VdotN - sqrtf(1.0f - s2)) * N - eta * V;
+}
+LT_D float sin2_t(float VdotN, float eta) { return eta * eta * (1.0f - VdotN * VdotN); }
+LT_D float fresnel(float ni, float no, float VdotN, float s2) {
+    if (s2 > 1.0f) return 1.0f;
+    float ct = sqrtf(1.0f - s2);
+    float rs = (ni * VdotN - no * ct) / (ni * VdotN + no * ct);
+    float rp = (no * VdotN - ni * ct) / (no * VdotN + ni * ct);
+    return 0.5f * (rs * rs + rp * rp);
+}
+
+LT_D int supported_lobes(int bsdf) {
+    return bsdf == BSDF_LAMBERT ? LOBE_DIFFUSE
+         : bsdf == BSDF_MIRROR ? LOBE_SPEC_REFL
+         : bsdf == BSDF_DIELECTRIC ? LOBE_SPECULAR
+         : LOBE_GLOSSY;
+}
+
+/* ConstantTexture / UVTexture / ImageTexture (+ ImageCache::SampleImage:
+ * nearest texel by std::round, v flipped; indices clamped to the image). */
+LT_D V3 albedo(const DScene &sc, const DMaterial &m, const Interaction &it) {
+    if (m.texture == TEX_CONSTANT) return v3(m.albedo[0], m.albedo[1], m.albedo[2]);
+    if (m.texture == TEX_UV) return v3(it.tu, it.tv, 0.0f);
+    const DTexture t = sc.textures[m.texture];
+    float u = it.tu, v = 1.0f - it.tv;
+    long x = (long)roundf(u * (float)t.w);
+    long y = (long)roundf(v * (float)t.h);
+    x = x < 0 ? 0 : (x > (long)t.w - 1 ? (long)t.w - 1 : x);
+    y = y < 0 ? 0 : (y > (long)t.h - 1 ? (long)t.h - 1 : y);
+    const float *p = sc.texels + t.offset + ((size_t)y * t.w + (size_t)x) * 3;
+    return v3(p[0], p[1], p[2]);
+}
+
+/* GGX conductor (no reference counterpart): D, separable Smith G1, Schlick F
+ * with F0 = albedo; Eval includes cos(wi) like the reference BSDFs. */
+LT_D float ggx_d(float a2, float c) {
+    float t = __builtin_fmaf(c * c, a2 - 1.0f, 1.0f);
+    return a2 / ((float)M_PI * t * t);
+}
+LT_D float ggx_g1(float a2, float c) { return (2.0f * c) / (c + sqrtf(__builtin_fmaf(1.0f - a2, c * c, a2))); }
+
+LT_D V3 bsdf_eval(const DScene &sc, const DMaterial &m, const Interaction &it) {
+    if (m.bsdf == BSDF_LAMBERT) return (albedo(sc, m, it) * (float)M_1_PI) * dot(it.wi, it.N);   /* lambert_bsdf.h:26-28 */
+    if (m.bsdf == BSDF_MIRROR || m.bsdf == BSDF_DIELECTRIC) return albedo(sc, m, it);            /* mirror_bsdf.h:27-29 */
+    float co = dot(it.N, it.wo), ci = dot(it.N, it.wi);
+    if (!(co > 0.0f) || !(ci > 0.0f)) return v3s(0.0f);
+    V3 h = normalize(it.wo + it.wi);
+    float ch = dot(it.N, h);
+    float a = m.roughness, a2 = a * a;
+    float oh = fmax_ref(dot(it.wi, h), 0.0f);
+    float om = 1.0f - oh, om2 = om * om, om5 = om2 * om2 * om;
+    V3 f0 = albedo(sc, m, it);
+    V3 F = f0 + (v3s(1.0f) - f0) * om5;
+    float k = (ggx_d(a2, ch) * ggx_g1(a2, co) * ggx_g1(a2, ci)) / (4.0f * co);
+    return F * k;
+}
+
+LT_D float bsdf_pdf(const DMaterial &m, const Interaction &it) {
+    if (m.bsdf == BSDF_LAMBERT) return dot(it.wi, it.N) * (float)M_1_PI;                         /* lambert_bsdf.h:36-38 */
+    if (m.bsdf == BSDF_MIRROR || m.bsdf == BSDF_DIELECTRIC) return 1.0f;
+    float co = dot(it.N, it.wo), ci = dot(it.N, it.wi);
+    if (!(co > 0.0f) || !(ci > 0.0f)) return 0.0f;
+    V3 h = normalize(it.wo + it.wi);
+    float ch = dot(it.N, h);
+    float a = m.roughness, a2 = a * a;
+    return (ggx_d(a2, ch) * ch) / (4.0f * fabsf(dot(it.wo, h)));
+}
+
+LT_D void bsdf_sample(const DMaterial &m, Interaction &it, PathRng &s) {
+    if (m.bsdf == BSDF_LAMBERT) {                                                                /* lambert_bsdf.h:30-34 */
+        it.wi = cosine_hemisphere(it.N, s);
+        it.lobe = LOBE_DIFFUSE;
+    } else if (m.bsdf == BSDF_MIRROR) {                                                          /* mirror_bsdf.h:31-34 */
+        it.wi = reflect(it.wo, it.N);
+        it.lobe = LOBE_SPEC_REFL;
+    } else if (m.bsdf == BSDF_DIELECTRIC) {                                                      /* ideal_specular_dielectric.h:35-64 */
+        float VdotN = dot(it.wo, it.N);
+        float ioro = m.ior;
+        if (VdotN < 0.0f) {
+            ioro = 1.0f;
+            it.N = -it.N;
+            VdotN = -VdotN;
+        }
+        float eta = it.iori / ioro;
+        float s2 = sin2_t(VdotN, eta);
+        float F = fresnel(it.iori, ioro, VdotN, s2);
+        float r = s.next_float();
+        if (r <= F) {
+            it.wi = reflect(it.wo, it.N);
+            it.lobe = LOBE_SPEC_REFL;
+            it.ioro = it.iori;
+        } else {
+            it.wi = refract(it.wo, it.N, VdotN, eta, s2);
+            it.lobe = LOBE_SPEC_TRANS;
+            it.ioro = ioro;
+        }
+    } else {                                                                                     /* GGX: h ~ D(h)cos(h) */
+        float a = m.roughness, a2 = a * a;
+        float u1 = s.next_float(), u2 = s.next_float();
+        float ct = sqrtf((1.0f - u1) / __builtin_fmaf(a2 - 1.0f, u1, 1.0f));
+        float st = sqrtf(fmax_ref(0.0f, 1.0f - ct * ct));
+        float sn, cs;
+        sincos_turns(u2, &sn, &cs);
+        V3 h = rotate_to_world(st * cs, st * sn, ct, it.N);
+        it.wi = reflect(it.wo, h);
+        it.lobe = LOBE_GLOSSY;
+    }
+}
+
+/* rtcInterpolate1: P = fma(w,p0, fma(u,p1, v*p2)), w = 1-u-v
+ * (scene_triangle_mesh.cpp:241-247). */
+LT_D V3 interp_normal(const DScene &sc, uint32_t gprim, float u, float v) {
+    const uint32_t *ix = sc.tri_vidx + 3 * (size_t)gprim;
+    uint32_t a = ix[0], b = ix[1], c = ix[2];
+    const float *n = sc.normals;
+    float w = 1.0f - u - v;
+    return v3(__builtin_fmaf(w, n[3 * a], __builtin_fmaf(u, n[3 * b], v * n[3 * c])),
+              __builtin_fmaf(w, n[3 * a + 1], __builtin_fmaf(u, n[3 * b + 1], v * n[3 * c + 1])),
+              __builtin_fmaf(w, n[3 * a + 2], __builtin_fmaf(u, n[3 * b + 2], v * n[3 * c + 2])));
+}
+LT_D void interp_uv(const DScene &sc, uint32_t gprim, float u, float v, float &ou, float &ov) {
+    const uint32_t *ix = sc.tri_vidx + 3 * (size_t)gprim;
+    uint32_t a = ix[0], b = ix[1], c = ix[2];
+    const float *t = sc.uvs;
+    float w = 1.0f - u - v;
+    ou = __builtin_fmaf(w, t[2 * a], __builtin_fmaf(u, t[2 * b], v * t[2 * c]));
+    ov = __builtin_fmaf(w, t[2 * a + 1], __builtin_fmaf(u, t[2 * b + 1], v * t[2 * c + 1]));
+}
+
+/* Nearest-voxel density of a grid medium (no reference counterpart). */
+LT_D float grid_density(const DScene &sc, const DGrid &g, float px, float py, float pz) {
+    float p[3] = {px, py, pz};
+    uint32_t n[3] = {g.nx, g.ny, g.nz};
+    int ix[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (!(p[a] >= g.bmin[a] && p[a] <= g.bmax[a])) return 0.0f;
+        float f = (p[a] - g.bmin[a]) / (g.bmax[a] - g.bmin[a]) * (float)n[a];
+        int i = (int)f;
+        ix[a] = i < 0 ? 0 : (i > (int)n[a] - 1 ? (int)n[a] - 1 : i);
+    }
+    return sc.voxels[g.offset + ((size_t)ix[2] * g.ny + ix[1]) * g.nx + ix[0]];
+}
+
+/* Henyey-Greenstein about the propagation direction d (no reference counterpart). */
+LT_D V3 sample_hg(float g, const V3 &d, PathRng &s) {
+    float u1 = s.next_float(), u2 = s.next_float();
+    float ct;
+    if (fabsf(g) < 1e-3f) {
+        ct = 1.0f - 2.0f * u1;
+    } else {
+        float q = (1.0f - g * g) / (1.0f - g + 2.0f * g * u1);
+        ct = (1.0f + g * g - q * q) / (2.0f * g);
+    }
+    ct = fmin_ref(1.0f, fmax_ref(-1.0f, ct));
+    float st = sqrtf(fmax_ref(0.0f, 1.0f - ct * ct));
+    float sn, cs;
+    sincos_turns(u2, &sn, &cs);
+    return normalize(rotate_to_world(st * cs, st * sn, ct, d));
+}
+
+} // namespace lt

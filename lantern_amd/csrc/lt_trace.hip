@@ -1,0 +1,237 @@
+/*
+ * lt_trace.hip - closest-hit and any-hit traversal for gfx950.
+ *
+ * Replaces rtcIntersect1 -> BVHNIntersector1::intersect
+ * (kernels/bvh/bvh_intersector1.cpp:44-132) and its triangle test
+ * MoellerTrumboreIntersector1::intersect (kernels/geometry/
+ * triangle_intersector_moeller.h:75-113) with Intersect1EpilogM
+ * (kernels/geometry/intersector_epilog.h:241-305).
+ *
+ * Design for CDNA4:
+ *  - one ray per lane, 64-lane waves fetch 64 rays at a time from an atomic
+ *    work counter (persistent threads, Aila & Laine 2009), so waves that drew
+ *    short rays go back for more instead of idling;
+ *  - rays (32 B) and hits (16 B) are read/written as float4 pairs, so a wave
+ *    touches 2 KB of contiguous HBM per batch;
+ *  - the traversal stack lives in LDS, laid out [depth][lane] so the 64 lanes
+ *    of a push hit 64 consecutive dwords (no bank conflicts); entries beyond
+ *    LDS_STACK spill to a per-thread slice of global memory (the LBVH is at
+ *    most 64 levels deep, so the total capacity STACK_CAP cannot overflow);
+ *  - a malformed hierarchy cannot hang or overrun: traversal steps, stack
+ *    depth and leaf walks are bounded, and a ray that hits a bound reports
+ *    t = -1 with no primitive;
+ *  - box tests are (lo - o) * (1/d) with the far distance scaled by
+ *    (1 + 2^-21), so rounding never culls a box that contains a hit; the
+ *    closest hit is decided only by the exact triangle test, which makes the
+ *    result independent of the BVH except for exact t ties.
+ */
+#include "lt_internal.h"
+
+#include <algorithm>
+
+namespace lt {
+
+namespace {
+
+constexpr int LDS_STACK = 24;
+constexpr int STACK_CAP = 96;
+constexpr uint32_t MAX_STEPS = 1u << 22;
+
+struct TravRay {
+    V3 o, d, idir;
+    float tnear, tmin, tfar;
+};
+
+/* rcp_safe (embree common/math/vec3fa.h:171-174): |d| < 1e-18 -> +-1e-18. */
+LT_D float safe_rcp(float d) {
+    const float m = 1e-18f;
+    if (fabsf(d) < m) d = signbit(d) ? -m : m;
+    return 1.0f / d;
+}
+
+/* Slab test of one child box against [tmin, tfar]. */
+LT_D bool slab(float lox, float hix, float loy, float hiy, float loz, float hiz, const TravRay &r, float &tentry) {
+    float t0x = (lox - r.o.x) * r.idir.x, t1x = (hix - r.o.x) * r.idir.x;
+    float t0y = (loy - r.o.y) * r.idir.y, t1y = (hiy - r.o.y) * r.idir.y;
+    float t0z = (loz - r.o.z) * r.idir.z, t1z = (hiz - r.o.z) * r.idir.z;
+    float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
+    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), r.tfar));
+    tf = tf * 1.0000005f;
+    tentry = tn;
+    return tn <= tf;
+}
+
+/* Embree's Moeller-Trumbore with Ng = cross(e2, e1)
+ * (triangle_intersector_moeller.h:75-113, :115-124) and the finalize of
+ * MoellerTrumboreHitM (:43-48) as a correctly rounded division. */
+LT_D bool moeller(const V3 &O, const V3 &D, float tnear, float tfar, const V3 &v0, const V3 &e1, const V3 &e2, float &t,
+                  float &u, float &v) {
+    V3 Ng = cross(e2, e1);
+    V3 C = v0 - O;
+    V3 R = cross(C, D);
+    float den = dot(Ng, D);
+    float absDen = fabsf(den);
+    bool neg = signbit(den);
+    float U = dot(R, e2);
+    float V = dot(R, e1);
+    if (neg) {
+        U = -U;
+        V = -V;
+    }
+    if (!(den != 0.0f && U >= 0.0f && V >= 0.0f && U + V <= absDen)) return false;
+    float T = dot(Ng, C);
+    if (neg) T = -T;
+    if (!((absDen * tnear < T) && (T <= absDen * tfar))) return false;
+    t = T / absDen;
+    u = U / absDen;
+    v = V / absDen;
+    return true;
+}
+
+template <bool ANY, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, const Ray *__restrict__ rays, Hit *__restrict__ hits,
+                                                 uint32_t *__restrict__ occluded, const uint32_t *count_ptr,
+                                                 uint32_t count_imm, uint32_t *work, int *spill) {
+    __shared__ int lds[LDS_STACK * BLOCK];
+    const uint32_t n = count_ptr ? *count_ptr : count_imm;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    int *gstack = spill + (size_t)(blockIdx.x * BLOCK + tid) * (STACK_CAP - LDS_STACK);
+    const float4 *__restrict__ nodes = sc.nodes;
+    const float4 *__restrict__ tris = sc.tris;
+
+    while (true) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(work, 64u);
+        base = __shfl(base, 0);
+        if (base >= n) break;
+        const uint32_t i = base + lane;
+        if (i >= n) continue;
+
+        const float4 *rp = (const float4 *)(rays + i);
+        float4 ra = rp[0], rb = rp[1];
+        TravRay r;
+        r.o = v3(ra.x, ra.y, ra.z);
+        r.tnear = ra.w;
+        r.d = v3(rb.x, rb.y, rb.z);
+        r.tfar = rb.w;
+        r.tmin = fmaxf(r.tnear, 0.0f);
+        r.idir = v3(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
+        float best_t = r.tfar, best_u = 0.0f, best_v = 0.0f;
+        uint32_t best_prim = 0xFFFFFFFFu;
+        bool done = !(r.tfar >= 0.0f) || sc.ntris == 0;
+
+        int sp = 0;
+        int node = 0;
+        uint32_t steps = 0;
+        while (!done) {
+            /* guards against a malformed hierarchy: bounded work and stack */
+            if (++steps > MAX_STEPS) {
+                best_prim = 0xFFFFFFFFu;
+                best_t = -1.0f;
+                break;
+            }
+            if (node >= 0) {
+                const float4 *np = nodes + 4 * node;
+                float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+                float ta, tb;
+                bool ha = slab(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, r, ta);
+                bool hb = slab(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, r, tb);
+                int ca = __float_as_int(n3.x), cb = __float_as_int(n3.y);
+                if (ha && hb) {
+                    int nearc = ca, farc = cb;
+                    if (tb < ta) {
+                        nearc = cb;
+                        farc = ca;
+                    }
+                    if (sp >= STACK_CAP) {
+                        best_prim = 0xFFFFFFFFu;
+                        best_t = -1.0f;
+                        break;
+                    }
+                    if (sp < LDS_STACK) lds[sp * BLOCK + tid] = farc;
+                    else gstack[sp - LDS_STACK] = farc;
+                    ++sp;
+                    node = nearc;
+                    continue;
+                } else if (ha) {
+                    node = ca;
+                    continue;
+                } else if (hb) {
+                    node = cb;
+                    continue;
+                }
+            } else {
+                int k = ~node;
+                while (true) {
+                    const float4 *tp = tris + 3 * k;
+                    float4 a = tp[0], b = tp[1], c = tp[2];
+                    float t, u, v;
+                    if (moeller(r.o, r.d, r.tnear, r.tfar, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t,
+                                u, v)) {
+                        r.tfar = t;
+                        best_t = t;
+                        best_u = u;
+                        best_v = v;
+                        best_prim = __float_as_uint(a.w);
+                        if (ANY) {
+                            done = true;
+                            break;
+                        }
+                    }
+                    if (__float_as_uint(b.w) & 1u) break;
+                    if (++k >= (int)sc.ntris) break;
+                }
+                if (done) break;
+            }
+            if (sp == 0) break;
+            --sp;
+            node = sp < LDS_STACK ? lds[sp * BLOCK + tid] : gstack[sp - LDS_STACK];
+        }
+        if (ANY) {
+            occluded[i] = best_prim != 0xFFFFFFFFu ? 1u : 0u;
+        } else {
+            float4 o = make_float4(best_t, best_u, best_v, __uint_as_float(best_prim));
+            *(float4 *)(hits + i) = o;
+        }
+    }
+}
+
+template <bool ANY>
+int launch(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, uint32_t *occl, const uint32_t *count_ptr,
+           uint32_t count, uint32_t *work, int block, int grid, int *spill) {
+    switch (block) {
+    case 64: hipLaunchKernelGGL((k_trace<ANY, 64>), dim3(grid), dim3(64), 0, st, sc, rays, hits, occl, count_ptr, count, work, spill); break;
+    case 128: hipLaunchKernelGGL((k_trace<ANY, 128>), dim3(grid), dim3(128), 0, st, sc, rays, hits, occl, count_ptr, count, work, spill); break;
+    case 256: hipLaunchKernelGGL((k_trace<ANY, 256>), dim3(grid), dim3(256), 0, st, sc, rays, hits, occl, count_ptr, count, work, spill); break;
+    default: return set_error(LT_E_INVALID, "traversal block size must be 64, 128 or 256");
+    }
+    LT_HIP(hipGetLastError());
+    return LT_OK;
+}
+
+} // namespace
+
+int trace_default_block() { return 128; }
+
+/* Resident blocks: LDS per block is LDS_STACK * block * 4 B of the CU's
+ * 160 KB, and at most 32 waves per CU are worth scheduling. */
+int trace_grid(int num_cus, int block) {
+    int waves_per_block = block / 64;
+    int by_lds = (160 * 1024) / (LDS_STACK * block * 4);
+    int by_waves = std::max(1, 32 / waves_per_block);
+    return std::max(1, num_cus) * std::max(1, std::min(by_lds, by_waves));
+}
+
+size_t trace_spill_bytes(int grid, int block) { return (size_t)grid * block * (STACK_CAP - LDS_STACK) * sizeof(int); }
+
+int launch_trace_closest(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, const uint32_t *count_ptr,
+                         uint32_t count, uint32_t *work, int block, int grid, int *spill) {
+    return launch<false>(st, sc, rays, hits, nullptr, count_ptr, count, work, block, grid, spill);
+}
+int launch_trace_any(hipStream_t st, const DScene &sc, const Ray *rays, uint32_t *occluded,
+                     const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block, int grid, int *spill) {
+    return launch<true>(st, sc, rays, nullptr, occluded, count_ptr, count, work, block, grid, spill);
+}
+
+} // namespace lt

@@ -1,0 +1,281 @@
+"""Scene descriptions for the rendering path.
+
+SceneDesc is plain data (numpy arrays and dicts), so the same description can
+be handed to the GPU library and, in tests, to the CPU oracle. The synthetic
+workloads are built from the host generators of liblantern_gpu.so
+(CreateGeosphere / CreateGrid / CreateBox, source/core/scene/geometry_generator.cpp)
+plus a seeded gradient-noise fBm. Every free choice (light height, ground
+albedo, camera placement, noise frequency) is recorded in SceneDesc.meta.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _capi
+
+DEFAULT_SEED = 0x4C414E54
+
+
+class SceneDesc:
+    def __init__(self):
+        self.meshes = []      # dict(pos, idx, nrm, uv, material, emission)
+        self.materials = []   # dict(bsdf, albedo, texture, ior, roughness, medium)
+        self.media = []
+        self.textures = []    # (h, w, 3) float32
+        self.grids = []       # dict(density (nz,ny,nx), bmin, bmax)
+        self.background = (0.0, 0.0, 0.0)
+        self.camera = None    # dict(phi, theta, radius, width, height, target, fov, filter)
+        self.params = None    # dict(spp, max_depth, rr_start, seed)
+        self.meta = {}
+
+    def add_material(self, **m):
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+    def add_mesh(self, pos, idx, nrm=None, uv=None, material=0, emission=None):
+        self.meshes.append(dict(pos=np.ascontiguousarray(pos, np.float32), idx=np.ascontiguousarray(idx, np.uint32),
+                                nrm=None if nrm is None else np.ascontiguousarray(nrm, np.float32),
+                                uv=None if uv is None else np.ascontiguousarray(uv, np.float32),
+                                material=material, emission=emission))
+        return len(self.meshes) - 1
+
+    @property
+    def num_triangles(self):
+        return int(sum(m["idx"].shape[0] * (2 if m["idx"].shape[1] == 4 else 1) for m in self.meshes))
+
+
+# ---- generators (through the C ABI) --------------------------------------------
+
+def geosphere(radius, subdiv):
+    L = _capi.lib()
+    nt = C.c_uint32()
+    nv = L.lt_geometry_geosphere(radius, subdiv, None, None, None, C.byref(nt))
+    pos = np.zeros((nv, 3), np.float32)
+    nrm = np.zeros((nv, 3), np.float32)
+    idx = np.zeros((nt.value, 3), np.uint32)
+    L.lt_geometry_geosphere(radius, subdiv, pos.ctypes.data_as(C.POINTER(C.c_float)),
+                            nrm.ctypes.data_as(C.POINTER(C.c_float)), idx.ctypes.data_as(C.POINTER(C.c_uint32)),
+                            C.byref(nt))
+    return pos, nrm, idx
+
+
+def grid(width, depth, m, n):
+    L = _capi.lib()
+    nt = C.c_uint32()
+    nv = L.lt_geometry_grid(width, depth, m, n, None, None, None, None, C.byref(nt))
+    pos = np.zeros((nv, 3), np.float32)
+    nrm = np.zeros((nv, 3), np.float32)
+    uv = np.zeros((nv, 2), np.float32)
+    idx = np.zeros((nt.value, 3), np.uint32)
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    L.lt_geometry_grid(width, depth, m, n, fp(pos), fp(nrm), fp(uv), idx.ctypes.data_as(C.POINTER(C.c_uint32)),
+                       C.byref(nt))
+    return pos, nrm, uv, idx
+
+
+def box(w, h, d):
+    L = _capi.lib()
+    nt = C.c_uint32()
+    nv = L.lt_geometry_box(w, h, d, None, None, None, None, C.byref(nt))
+    pos = np.zeros((nv, 3), np.float32)
+    nrm = np.zeros((nv, 3), np.float32)
+    uv = np.zeros((nv, 2), np.float32)
+    idx = np.zeros((nt.value, 3), np.uint32)
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    L.lt_geometry_box(w, h, d, fp(pos), fp(nrm), fp(uv), idx.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(nt))
+    return pos, nrm, uv, idx
+
+
+def transform(pos, nrm, m4):
+    """AddMesh applies a row-major 4x4 to positions (scene.cpp:456-461); normals are
+    transformed by the inverse transpose of the upper 3x3 (the reference keeps them
+    as given; all transforms used here are rigid or axis flips)."""
+    m = np.asarray(m4, np.float64).reshape(4, 4)
+    p = np.c_[pos.astype(np.float64), np.ones(len(pos))] @ m.T
+    out_p = p[:, :3].astype(np.float32)
+    out_n = None
+    if nrm is not None:
+        nm = np.linalg.inv(m[:3, :3]).T
+        n = nrm.astype(np.float64) @ nm.T
+        n /= np.linalg.norm(n, axis=1, keepdims=True)
+        out_n = n.astype(np.float32)
+    return out_p, out_n
+
+
+def translate(x, y, z):
+    return [1, 0, 0, x, 0, 1, 0, y, 0, 0, 1, z, 0, 0, 0, 1]
+
+
+# ---- seeded gradient noise (improved Perlin noise, K. Perlin, SIGGRAPH 2002) ----
+
+_GRAD = np.array([[1, 1, 0], [-1, 1, 0], [1, -1, 0], [-1, -1, 0], [1, 0, 1], [-1, 0, 1], [1, 0, -1], [-1, 0, -1],
+                  [0, 1, 1], [0, -1, 1], [0, 1, -1], [0, -1, -1]], np.float64)
+
+
+class GradientNoise:
+    def __init__(self, seed):
+        rng = np.random.Generator(np.random.PCG64(seed))
+        p = rng.permutation(256)
+        self.perm = np.concatenate([p, p]).astype(np.int64)
+        self.gi = rng.integers(0, 12, 256)
+
+    def __call__(self, pts):
+        pts = np.asarray(pts, np.float64)
+        fl = np.floor(pts)
+        i = fl.astype(np.int64) & 255
+        f = pts - fl
+        u = f * f * f * (f * (f * 6 - 15) + 10)
+        P = self.perm
+        out = 0.0
+        res = np.zeros(len(pts))
+        corners = []
+        for dz in (0, 1):
+            for dy in (0, 1):
+                for dx in (0, 1):
+                    h = P[P[P[i[:, 0] + dx] + i[:, 1] + dy] + i[:, 2] + dz]
+                    g = _GRAD[self.gi[h & 255]]
+                    d = f - np.array([dx, dy, dz], np.float64)
+                    corners.append(((dx, dy, dz), np.sum(g * d, axis=1)))
+        for (dx, dy, dz), val in corners:
+            wx = u[:, 0] if dx else 1 - u[:, 0]
+            wy = u[:, 1] if dy else 1 - u[:, 1]
+            wz = u[:, 2] if dz else 1 - u[:, 2]
+            res += wx * wy * wz * val
+        del out
+        return res
+
+
+def fbm(noise, pts, octaves, freq=2.0, lacunarity=2.0, gain=0.5):
+    pts = np.asarray(pts, np.float64)
+    total = np.zeros(len(pts))
+    amp, f = 1.0, freq
+    for _ in range(octaves):
+        total += amp * noise(pts * f)
+        amp *= gain
+        f *= lacunarity
+    return total
+
+
+def displaced_geosphere(subdiv, amplitude, octaves, seed, freq=2.0):
+    """CreateGeosphere(1, subdiv) displaced radially by 1 + amplitude * fBm(n).
+    Normals are the analytic normals of the displaced surface (finite-difference
+    tangential gradient of the fBm field)."""
+    pos, nrm, idx = geosphere(1.0, subdiv)
+    noise = GradientNoise(seed)
+    n = nrm.astype(np.float64)
+    f0 = fbm(noise, n, octaves, freq)
+    r = 1.0 + amplitude * f0
+    eps = 1e-4
+    grad = np.zeros_like(n)
+    for a in range(3):
+        e = np.zeros(3)
+        e[a] = eps
+        grad[:, a] = (fbm(noise, n + e, octaves, freq) - fbm(noise, n - e, octaves, freq)) / (2 * eps)
+    tang = grad - np.sum(grad * n, axis=1, keepdims=True) * n
+    sn = n - amplitude * tang / r[:, None]
+    sn /= np.linalg.norm(sn, axis=1, keepdims=True)
+    p = (n * r[:, None]).astype(np.float32)
+    return p, sn.astype(np.float32), idx
+
+
+def quad_light(size, height, le):
+    """CreateGrid(size, size, 2, 2) flipped to face -y at y = height. Returns the
+    mesh and the radiant power that gives radiance le (area_light.h:17)."""
+    pos, nrm, uv, idx = grid(size, size, 2, 2)
+    flip = [1, 0, 0, 0, 0, -1, 0, height, 0, 0, -1, 0, 0, 0, 0, 1]
+    p, n = transform(pos, nrm, flip)
+    power = le * math.pi * size * size
+    return p, n, uv, idx, power
+
+
+# ---- the benchmark workloads ---------------------------------------------------
+
+def nine_sphere(width=512, height=512, spp=16, seed=DEFAULT_SEED, subdiv=4, max_depth=8):
+    """9 icospheres (subdiv 4, radius 1) on a 3x3 grid with spacing 2.5 over a
+    20x20 ground quad, one 2x2 quad light (radiance 10), seeded Lambertian
+    albedo U[0.2,0.8]^3 per sphere, 45 deg pinhole camera."""
+    d = SceneDesc()
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ground = d.add_material(bsdf=_capi.LT_BSDF_LAMBERT, albedo=(0.8, 0.8, 0.8))
+    light_mat = d.add_material(bsdf=_capi.LT_BSDF_LAMBERT, albedo=(0.0, 0.0, 0.0))
+    spos, snrm, sidx = geosphere(1.0, subdiv)
+    for ix, x in enumerate((-2.5, 0.0, 2.5)):
+        for iz, z in enumerate((-2.5, 0.0, 2.5)):
+            alb = tuple(float(v) for v in rng.uniform(0.2, 0.8, 3))
+            m = d.add_material(bsdf=_capi.LT_BSDF_LAMBERT, albedo=alb)
+            p, n = transform(spos, snrm, translate(x, 1.0, z))
+            d.add_mesh(p, sidx, n, None, m)
+    gp, gn, guv, gi = grid(20.0, 20.0, 2, 2)
+    d.add_mesh(gp, gi, gn, guv, ground)
+    lp, ln, luv, li, power = quad_light(2.0, 5.0, 10.0)
+    d.add_mesh(lp, li, ln, luv, light_mat, emission=((1.0, 1.0, 1.0), power))
+    d.background = (0.0, 0.0, 0.0)
+    d.camera = dict(phi=1.1, theta=0.35, radius=12.0, width=width, height=height, target=(0.0, 1.0, 0.0),
+                    fov=math.pi / 4, filter=0)
+    d.params = dict(spp=spp, max_depth=max_depth, rr_start=4, seed=seed)
+    d.meta = dict(workload="nine_sphere", seed=seed, light_height=5.0, ground_albedo=0.8, camera_phi=1.1,
+                  camera_theta=0.35, camera_radius=12.0, rr="bounces > 3 (reference)")
+    return d
+
+
+def triangle_heavy(width=1280, height=720, spp=64, seed=DEFAULT_SEED, subdiv=7, max_depth=10):
+    """Icosphere subdiv 7 (327,680 tris) with seeded 6-octave fBm radial
+    displacement (amplitude 0.15), GGX conductor roughness 0.3 plus a Lambertian
+    floor, constant environment 1.0 (background) plus one quad light,
+    Russian roulette from depth 3."""
+    d = SceneDesc()
+    p, n, idx = displaced_geosphere(subdiv, 0.15, 6, seed)
+    metal = d.add_material(bsdf=_capi.LT_BSDF_GGX, albedo=(0.95, 0.64, 0.54), roughness=0.3)
+    floor = d.add_material(bsdf=_capi.LT_BSDF_LAMBERT, albedo=(0.8, 0.8, 0.8))
+    light_mat = d.add_material(bsdf=_capi.LT_BSDF_LAMBERT, albedo=(0.0, 0.0, 0.0))
+    d.add_mesh(p, idx, n, None, metal)
+    gp, gn, guv, gi = grid(20.0, 20.0, 2, 2)
+    gp2, gn2 = transform(gp, gn, translate(0.0, -1.3, 0.0))
+    d.add_mesh(gp2, gi, gn2, guv, floor)
+    lp, ln, luv, li, power = quad_light(2.0, 4.0, 10.0)
+    d.add_mesh(lp, li, ln, luv, light_mat, emission=((1.0, 1.0, 1.0), power))
+    d.background = (1.0, 1.0, 1.0)
+    d.camera = dict(phi=1.25, theta=0.5, radius=5.0, width=width, height=height, target=(0.0, 0.0, 0.0),
+                    fov=math.pi / 4, filter=0)
+    d.params = dict(spp=spp, max_depth=max_depth, rr_start=3, seed=seed)
+    d.meta = dict(workload="triangle_heavy", seed=seed, fbm_freq=2.0, fbm_octaves=6, amplitude=0.15,
+                  floor_y=-1.3, light_height=4.0, light_le=10.0, ggx_f0=(0.95, 0.64, 0.54), camera_phi=1.25,
+                  camera_theta=0.5, camera_radius=5.0)
+    return d
+
+
+def nine_balls_reference(width=128, height=72, spp=4, seed=DEFAULT_SEED, max_depth=1500):
+    """The content of the reference's scenes/nine_balls_scene.json: a 50x50 gray
+    floor at y=-6, an emissive black geosphere (radiant power 800) at the origin
+    and eight n=3 geospheres of radius 2 at (+-4, +-4, +-4) with green, blue,
+    orange, mirror and glass (IOR 1.35) BSDFs, background (0.846, 0.933, 0.949),
+    camera phi=pi/2, theta=0, radius 40, fov 0.7853981. The resolution is a
+    parameter (the file uses 1280x720)."""
+    d = SceneDesc()
+    L = _capi
+    mats = {}
+    for name, bsdf, alb, ior in [("green", L.LT_BSDF_LAMBERT, (0.408, 0.741, 0.467), 1.0),
+                                 ("blue", L.LT_BSDF_LAMBERT, (0.392, 0.584, 0.929), 1.0),
+                                 ("orange", L.LT_BSDF_LAMBERT, (1.0, 0.498, 0.314), 1.0),
+                                 ("black", L.LT_BSDF_LAMBERT, (0.0, 0.0, 0.0), 1.0),
+                                 ("gray", L.LT_BSDF_LAMBERT, (0.9, 0.9, 0.9), 1.0),
+                                 ("mirror", L.LT_BSDF_MIRROR, (0.95, 0.95, 0.95), 1.0),
+                                 ("glass", L.LT_BSDF_DIELECTRIC, (0.95, 0.95, 0.95), 1.35)]:
+        mats[name] = d.add_material(bsdf=bsdf, albedo=alb, ior=ior)
+    gp, gn, guv, gi = grid(50.0, 50.0, 2, 2)
+    p, n = transform(gp, gn, translate(0, -6, 0))
+    d.add_mesh(p, gi, n, guv, mats["gray"])
+    sp, sn, si = geosphere(2.0, 3)
+    d.add_mesh(sp, si, sn, None, mats["black"], emission=((1.0, 1.0, 1.0), 800.0))
+    for (x, y, z), m in [((-4, 4, 4), "green"), ((4, 4, 4), "glass"), ((-4, -4, 4), "orange"), ((4, -4, 4), "blue"),
+                         ((-4, 4, -4), "blue"), ((4, 4, -4), "orange"), ((-4, -4, -4), "mirror"),
+                         ((4, -4, -4), "green")]:
+        p, n = transform(sp, sn, translate(x, y, z))
+        d.add_mesh(p, si, n, None, mats[m])
+    d.background = (0.846, 0.933, 0.949)
+    d.camera = dict(phi=1.57079632679489661923, theta=0.0, radius=40.0, width=width, height=height,
+                    target=(0.0, 0.0, 0.0), fov=0.7853981, filter=0)
+    d.params = dict(spp=spp, max_depth=max_depth, rr_start=4, seed=seed)
+    d.meta = dict(workload="nine_balls_reference", seed=seed)
+    return d

@@ -1,0 +1,220 @@
+"""ctypes binding of liblt_oracle.so (oracle/lt_oracle.h).
+
+TEST INFRASTRUCTURE ONLY: the checker. Consumes the same plain-data scene
+descriptions as the product (numpy arrays and dicts) and never calls into
+liblantern_gpu.so.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblt_oracle.so")
+
+ORC_RNG_PCG_TILE, ORC_RNG_PHILOX = 0, 1
+
+
+class orc_camera_desc(C.Structure):
+    _fields_ = [("phi", C.c_float), ("theta", C.c_float), ("radius", C.c_float), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("target", C.c_float * 3), ("fov", C.c_float), ("filter", C.c_int32)]
+
+
+class orc_material_desc(C.Structure):
+    _fields_ = [("bsdf", C.c_int32), ("albedo", C.c_float * 3), ("texture", C.c_int32), ("ior", C.c_float),
+                ("roughness", C.c_float), ("medium", C.c_int32)]
+
+
+class orc_medium_desc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("absorption_color", C.c_float * 3), ("absorption_at_distance", C.c_float),
+                ("scattering_distance", C.c_float), ("grid", C.c_int32), ("sigma_t_scale", C.c_float),
+                ("albedo", C.c_float), ("hg_g", C.c_float)]
+
+
+class orc_render_params(C.Structure):
+    _fields_ = [("spp", C.c_uint32), ("max_depth", C.c_uint32), ("rr_start", C.c_uint32), ("seed", C.c_uint32),
+                ("frame0", C.c_uint32), ("rng", C.c_int32), ("threads", C.c_int32), ("stream", C.c_uint32),
+                ("x0", C.c_int32), ("y0", C.c_int32), ("x1", C.c_int32), ("y1", C.c_int32)]
+
+
+_lib = None
+_pf = C.POINTER(C.c_float)
+_pd = C.POINTER(C.c_double)
+_pu = C.POINTER(C.c_uint32)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run make -C oracle")
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            "orc_pcg32": (None, [C.c_uint64, C.c_uint64, C.c_uint32, _pu]),
+            "orc_pcg32_float": (None, [C.c_uint64, C.c_uint64, C.c_uint32, _pf]),
+            "orc_hash_mix": (C.c_uint32, [C.c_uint32, C.c_uint32]),
+            "orc_hash_finalize": (C.c_uint32, [C.c_uint32]),
+            "orc_tile_seed": (C.c_uint32, [C.c_uint32, C.c_uint32]),
+            "orc_philox4x32_10": (None, [_pu, _pu, _pu]),
+            "orc_sincos_turns": (None, [C.c_float, _pf, _pf]),
+            "orc_logf": (C.c_float, [C.c_float]),
+            "orc_expf": (C.c_float, [C.c_float]),
+            "orc_filter_cdf": (None, [_pf]),
+            "orc_filter_sample": (C.c_float, [C.c_int32, C.c_float]),
+            "orc_camera_basis": (None, [C.POINTER(orc_camera_desc), _pf]),
+            "orc_camera_rays": (None, [C.POINTER(orc_camera_desc), C.c_uint32, _pu, _pu, _pf, _pf, _pf]),
+            "orc_geosphere": (C.c_uint32, [C.c_float, C.c_uint32, _pf, _pf, _pu, _pu]),
+            "orc_grid": (C.c_uint32, [C.c_float, C.c_float, C.c_uint32, C.c_uint32, _pf, _pf, _pf, _pu, _pu]),
+            "orc_box": (C.c_uint32, [C.c_float, C.c_float, C.c_float, _pf, _pf, _pf, _pu, _pu]),
+            "orc_bounding_sphere": (None, [_pf, C.c_uint32, _pf]),
+            "orc_surface_area": (C.c_float, [_pf, _pu, C.c_uint32]),
+            "orc_scene_create": (C.c_void_p, []),
+            "orc_scene_destroy": (None, [C.c_void_p]),
+            "orc_scene_add_mesh": (C.c_int32, [C.c_void_p, _pf, C.c_uint32, _pu, C.c_uint32, _pf, _pf]),
+            "orc_scene_add_material": (C.c_int32, [C.c_void_p, C.POINTER(orc_material_desc)]),
+            "orc_scene_add_medium": (C.c_int32, [C.c_void_p, C.POINTER(orc_medium_desc)]),
+            "orc_scene_add_texture": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, _pf]),
+            "orc_scene_add_grid": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, _pf, _pf, _pf]),
+            "orc_scene_set_material": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
+            "orc_scene_set_emission": (C.c_int32, [C.c_void_p, C.c_int32, _pf, C.c_float]),
+            "orc_scene_set_background": (None, [C.c_void_p, _pf]),
+            "orc_scene_commit": (C.c_int32, [C.c_void_p]),
+            "orc_scene_lights": (C.c_uint32, [C.c_void_p, _pf]),
+            "orc_intersect": (None, [C.c_void_p, _pf, C.c_uint32, C.c_int32, _pf, _pu, _pu, _pu]),
+            "orc_intersect_f64": (None, [C.c_void_p, _pd, C.c_uint32, _pd, _pu, _pu]),
+            "orc_render_f32": (C.c_double, [C.c_void_p, C.POINTER(orc_camera_desc), C.POINTER(orc_render_params), _pf,
+                                            _pu, _pu, C.POINTER(C.c_uint64)]),
+            "orc_render_f64": (C.c_double, [C.c_void_p, C.POINTER(orc_camera_desc), C.POINTER(orc_render_params), _pd,
+                                            _pu, _pu, C.POINTER(C.c_uint64)]),
+            "orc_trace_path_f32": (C.c_uint32, [C.c_void_p, C.POINTER(orc_camera_desc), C.c_uint32, C.c_uint32,
+                                                C.POINTER(orc_render_params), _pf]),
+            "orc_hardware_threads": (C.c_uint32, []),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def default_threads():
+    """Host threads for the oracle: the machine's cores, capped by
+    OMP_NUM_THREADS (the CPU share of a shared GPU host) or 16."""
+    cap = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return max(1, min(lib().orc_hardware_threads(), cap))
+
+
+def fp(a):
+    return None if a is None else a.ctypes.data_as(_pf)
+
+
+def up(a):
+    return None if a is None else a.ctypes.data_as(_pu)
+
+
+def camera(cam):
+    return orc_camera_desc(cam["phi"], cam["theta"], cam["radius"], cam["width"], cam["height"],
+                           (C.c_float * 3)(*cam.get("target", (0.0, 0.0, 0.0))), cam["fov"], cam.get("filter", 0))
+
+
+class OracleScene:
+    """The oracle's copy of a SceneDesc (triangles only; quads are split (0,1,2),(0,2,3))."""
+
+    def __init__(self, desc):
+        L = lib()
+        self.h = L.orc_scene_create()
+        for m in desc.materials:
+            md = orc_material_desc(m["bsdf"], (C.c_float * 3)(*m.get("albedo", (0.0, 0.0, 0.0))),
+                                   m.get("texture", -1), m.get("ior", 1.0), m.get("roughness", 0.0),
+                                   m.get("medium", -1))
+            L.orc_scene_add_material(self.h, C.byref(md))
+        for t in desc.textures:
+            t = np.ascontiguousarray(t, np.float32)
+            L.orc_scene_add_texture(self.h, t.shape[1], t.shape[0], fp(t))
+        for g in desc.grids:
+            dens = np.ascontiguousarray(g["density"], np.float32)
+            nz, ny, nx = dens.shape
+            L.orc_scene_add_grid(self.h, nx, ny, nz, fp(dens), fp(np.asarray(g["bmin"], np.float32)),
+                                 fp(np.asarray(g["bmax"], np.float32)))
+        for m in desc.media:
+            md = orc_medium_desc(m["type"], (C.c_float * 3)(*m.get("absorption_color", (1.0, 1.0, 1.0))),
+                                 m.get("absorption_at_distance", 1.0), m.get("scattering_distance", 1.0),
+                                 m.get("grid", -1), m.get("sigma_t_scale", 0.0), m.get("albedo", 1.0),
+                                 m.get("hg_g", 0.0))
+            L.orc_scene_add_medium(self.h, C.byref(md))
+        for mesh in desc.meshes:
+            pos = np.ascontiguousarray(mesh["pos"], np.float32)
+            idx = np.ascontiguousarray(mesh["idx"], np.uint32)
+            if idx.shape[1] == 4:
+                idx = np.ascontiguousarray(np.concatenate([idx[:, [0, 1, 2]], idx[:, [0, 2, 3]]], axis=1).reshape(-1, 3))
+            nrm = mesh.get("nrm")
+            uv = mesh.get("uv")
+            nrm = None if nrm is None else np.ascontiguousarray(nrm, np.float32)
+            uv = None if uv is None else np.ascontiguousarray(uv, np.float32)
+            g = L.orc_scene_add_mesh(self.h, fp(pos), pos.shape[0], up(idx), idx.shape[0], fp(nrm), fp(uv))
+            if g < 0:
+                raise RuntimeError("orc_scene_add_mesh failed")
+            L.orc_scene_set_material(self.h, g, mesh["material"])
+            if mesh.get("emission") is not None:
+                col, power = mesh["emission"]
+                L.orc_scene_set_emission(self.h, g, (C.c_float * 3)(*col), power)
+        L.orc_scene_set_background(self.h, (C.c_float * 3)(*desc.background))
+        if L.orc_scene_commit(self.h) != 0:
+            raise RuntimeError("orc_scene_commit failed")
+
+    def close(self):
+        if self.h:
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def lights(self):
+        n = lib().orc_scene_lights(self.h, None)
+        out = np.zeros((max(n, 1), 9), np.float32)
+        lib().orc_scene_lights(self.h, fp(out))
+        return out[:n]
+
+    def intersect(self, rays, use_bvh=True, ties=False):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        tuv = np.zeros((n, 3), np.float32)
+        geom = np.zeros(n, np.uint32)
+        prim = np.zeros(n, np.uint32)
+        tie = np.zeros(n, np.uint32) if ties else None
+        lib().orc_intersect(self.h, fp(rays), n, 1 if use_bvh else 0, fp(tuv), up(geom), up(prim), up(tie))
+        return (tuv, geom, prim, tie) if ties else (tuv, geom, prim)
+
+    def render(self, cam, params, precision=32, rng=ORC_RNG_PHILOX, threads=0, window=None, frame0=0, stream=0):
+        """Returns (sum (h,w,3), bounces (h,w), count (h,w), seconds, rays)."""
+        L = lib()
+        w, h = cam["width"], cam["height"]
+        p = orc_render_params()
+        p.spp = params["spp"]
+        p.max_depth = params.get("max_depth", 1500)
+        p.rr_start = params.get("rr_start", 4)
+        p.seed = params.get("seed", 0x4C414E54)
+        p.frame0 = frame0
+        p.rng = rng
+        p.threads = threads if threads > 0 else default_threads()
+        p.stream = stream
+        if window:
+            p.x0, p.y0, p.x1, p.y1 = window
+        else:
+            p.x0 = p.y0 = p.x1 = p.y1 = 0
+        b = np.zeros((h, w), np.uint32)
+        c = np.zeros((h, w), np.uint32)
+        rays = C.c_uint64()
+        if precision == 64:
+            s = np.zeros((h, w, 3), np.float64)
+            sec = L.orc_render_f64(self.h, C.byref(camera(cam)), C.byref(p), s.ctypes.data_as(_pd), up(b), up(c),
+                                   C.byref(rays))
+        else:
+            s = np.zeros((h, w, 3), np.float32)
+            sec = L.orc_render_f32(self.h, C.byref(camera(cam)), C.byref(p), fp(s), up(b), up(c), C.byref(rays))
+        return s, b, c, sec, rays.value

@@ -1,0 +1,1586 @@
+/*
+ * lt_oracle.cpp - CPU oracle for the Lantern rendering hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see lt_oracle.h). This is a restatement of the
+ * reference algorithm, written for checking; nothing in the product links it.
+ *
+ * Arithmetic conventions (both precisions):
+ *   - dot(a,b)  = fma(a.x,b.x, fma(a.y,b.y, a.z*b.z))      embree common/math/vec3.h:201
+ *   - cross(a,b)= (fma(a.y,b.z,-(a.z*b.y)), ...)           embree common/math/vec3.h:206
+ *   - normalize(a) = a * (1/sqrt(dot(a,a)))  (Embree uses an rsqrt estimate plus
+ *     one Newton step, vec3fa.h:153-164; its bits are ISA-specific, so the
+ *     correctly rounded form is used here and in the GPU path)
+ *   - every other operation is a separately rounded IEEE op (-ffp-contract=off).
+ * FP32 transcendental functions are the Cephes single-precision kernels
+ * (S. L. Moshier, Cephes Math Library), restated below; FP64 uses libm.
+ */
+#include "lt_oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <thread>
+#include <vector>
+#include <xmmintrin.h>
+#include <pmmintrin.h>
+
+namespace {
+
+typedef uint32_t u32;
+typedef uint64_t u64;
+
+/* ------------------------------------------------------------------------ */
+/* Scalar kernels                                                            */
+/* ------------------------------------------------------------------------ */
+
+inline float fmaR(float a, float b, float c) { return std::fmaf(a, b, c); }
+inline double fmaR(double a, double b, double c) { return std::fma(a, b, c); }
+
+/* sin and cos of 2*pi*u. Quadrant reduction in turns (exact), then the Cephes
+ * sinf/cosf polynomials on |r| <= pi/4. */
+inline void sincos_turns_f(float u, float *s, float *c) {
+    float t = u * 4.0f;
+    float q = std::nearbyint(t);
+    float d = t - q;
+    float r = d * 1.57079637f;
+    float z = r * r;
+    float ps = fmaR(fmaR(-1.9515295891e-4f, z, 8.3321608736e-3f), z, -1.6666654611e-1f);
+    float sp = fmaR(ps, z * r, r);
+    float pc = fmaR(fmaR(2.443315711809948e-5f, z, -1.388731625493765e-3f), z, 4.166664568298827e-2f);
+    float cp = pc * (z * z);
+    cp = cp - 0.5f * z;
+    cp = cp + 1.0f;
+    int qi = ((int)q) & 3;
+    switch (qi) {
+    case 0: *s = sp; *c = cp; break;
+    case 1: *s = cp; *c = -sp; break;
+    case 2: *s = -sp; *c = -cp; break;
+    default: *s = -cp; *c = sp; break;
+    }
+}
+
+/* Cephes logf. */
+inline float log_f(float x) {
+    if (!(x > 0.0f)) return x == 0.0f ? -std::numeric_limits<float>::infinity()
+                                      : std::numeric_limits<float>::quiet_NaN();
+    if (x == std::numeric_limits<float>::infinity()) return x;
+    int e;
+    float m = std::frexp(x, &e);
+    if (m < 0.707106781186547524f) {
+        e -= 1;
+        m = m + m - 1.0f;
+    } else {
+        m = m - 1.0f;
+    }
+    float z = m * m;
+    float p = 7.0376836292e-2f;
+    p = fmaR(p, m, -1.1514610310e-1f);
+    p = fmaR(p, m, 1.1676998740e-1f);
+    p = fmaR(p, m, -1.2420140846e-1f);
+    p = fmaR(p, m, 1.4249322787e-1f);
+    p = fmaR(p, m, -1.6668057665e-1f);
+    p = fmaR(p, m, 2.0000714765e-1f);
+    p = fmaR(p, m, -2.4999993993e-1f);
+    p = fmaR(p, m, 3.3333331174e-1f);
+    float y = (p * m) * z;
+    float fe = (float)e;
+    y = fmaR(fe, -2.12194440e-4f, y);
+    y = y - 0.5f * z;
+    float r = m + y;
+    r = fmaR(fe, 0.693359375f, r);
+    return r;
+}
+
+/* Cephes expf. Results below FLT_MIN flush to zero (FTZ). */
+inline float exp_f(float x) {
+    if (x != x) return x;
+    if (x > 88.7228317f) return std::numeric_limits<float>::infinity();
+    if (x < -87.0f) return 0.0f;
+    float z = std::floor(fmaR(x, 1.44269504088896341f, 0.5f));
+    float r = fmaR(-z, 0.693359375f, x);
+    r = fmaR(z, 2.12194440e-4f, r);
+    int n = (int)z;
+    float zz = r * r;
+    float p = 1.9875691500e-4f;
+    p = fmaR(p, r, 1.3981999507e-3f);
+    p = fmaR(p, r, 8.3334519073e-3f);
+    p = fmaR(p, r, 4.1665795894e-2f);
+    p = fmaR(p, r, 1.6666665459e-1f);
+    p = fmaR(p, r, 5.0000001201e-1f);
+    float y = fmaR(p, zz, r) + 1.0f;
+    return std::ldexp(y, n);
+}
+
+template <class R> struct MathK;
+template <> struct MathK<float> {
+    static void sincos_turns(float u, float *s, float *c) { sincos_turns_f(u, s, c); }
+    static float log(float x) { return log_f(x); }
+    static float exp(float x) { return exp_f(x); }
+    static float sqrt(float x) { return std::sqrt(x); }
+};
+template <> struct MathK<double> {
+    static void sincos_turns(double u, double *s, double *c) {
+        double th = u * 2.0 * M_PI;
+        *s = std::sin(th);
+        *c = std::cos(th);
+    }
+    static double log(double x) { return std::log(x); }
+    static double exp(double x) { return std::exp(x); }
+    static double sqrt(double x) { return std::sqrt(x); }
+};
+
+/* ------------------------------------------------------------------------ */
+/* Vectors                                                                   */
+/* ------------------------------------------------------------------------ */
+template <class R> struct V3 {
+    R x, y, z;
+    V3() : x(0), y(0), z(0) {}
+    V3(R a) : x(a), y(a), z(a) {}
+    V3(R a, R b, R c) : x(a), y(b), z(c) {}
+    R operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+template <class R> inline V3<R> operator+(V3<R> a, V3<R> b) { return V3<R>(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <class R> inline V3<R> operator-(V3<R> a, V3<R> b) { return V3<R>(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <class R> inline V3<R> operator-(V3<R> a) { return V3<R>(-a.x, -a.y, -a.z); }
+template <class R> inline V3<R> operator*(V3<R> a, V3<R> b) { return V3<R>(a.x * b.x, a.y * b.y, a.z * b.z); }
+template <class R> inline V3<R> operator*(V3<R> a, R s) { return V3<R>(a.x * s, a.y * s, a.z * s); }
+template <class R> inline V3<R> operator*(R s, V3<R> a) { return V3<R>(s * a.x, s * a.y, s * a.z); }
+template <class R> inline V3<R> operator/(V3<R> a, R s) { return V3<R>(a.x / s, a.y / s, a.z / s); }
+template <class R> inline R dot(V3<R> a, V3<R> b) { return fmaR(a.x, b.x, fmaR(a.y, b.y, a.z * b.z)); }
+template <class R> inline V3<R> cross(V3<R> a, V3<R> b) {
+    return V3<R>(fmaR(a.y, b.z, -(a.z * b.y)), fmaR(a.z, b.x, -(a.x * b.z)), fmaR(a.x, b.y, -(a.y * b.x)));
+}
+template <class R> inline R length(V3<R> a) { return MathK<R>::sqrt(dot(a, a)); }
+template <class R> inline V3<R> normalize(V3<R> a) { return a * (R(1) / MathK<R>::sqrt(dot(a, a))); }
+template <class R> inline bool is_black(V3<R> a) { return a.x == R(0) && a.y == R(0) && a.z == R(0); }
+template <class R> inline R maxc(V3<R> a) { return std::max(a.x, std::max(a.y, a.z)); }
+template <class R, class S> inline V3<R> cvt(V3<S> a) { return V3<R>((R)a.x, (R)a.y, (R)a.z); }
+
+/* ------------------------------------------------------------------------ */
+/* Random numbers                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* uniform_sampler.h:17-57 */
+struct PcgSampler {
+    u64 state, seq;
+    PcgSampler(u64 seed, u64 sequence) : state(seed), seq(sequence) { next_u32(); }
+    u32 next_u32() {
+        u64 old = state;
+        state = old * 6364136223846793005ULL + (seq | 1ULL);
+        u32 xs = (u32)(((old >> 18u) ^ old) >> 27u);
+        u32 rot = (u32)(old >> 59u);
+        return (xs >> rot) | (xs << ((0u - rot) & 31u));
+    }
+};
+
+inline float bits_to_unit(u32 v) {
+    u32 b = (v >> 9u) | 0x3F800000u;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f - 1.0f;
+}
+
+/* integrator.cpp:45-73 */
+inline u32 hash_mix(u32 hash, u32 k) {
+    const u32 c1 = 0xcc9e2d51u, c2 = 0x1b873593u;
+    k *= c1;
+    k = (k << 15) | (k >> 17);
+    k *= c2;
+    hash ^= k;
+    hash = ((hash << 13) | (hash >> 19)) * 5u + 0xe6546b64u;
+    return hash;
+}
+inline u32 hash_finalize(u32 h) {
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+/* Philox-4x32-10: J. K. Salmon, M. A. Moraes, R. O. Dror, D. E. Shaw,
+ * "Parallel random numbers: as easy as 1, 2, 3", SC'11. */
+inline void philox(const u32 c_in[4], const u32 k_in[2], u32 out[4]) {
+    u32 c0 = c_in[0], c1 = c_in[1], c2 = c_in[2], c3 = c_in[3];
+    u32 k0 = k_in[0], k1 = k_in[1];
+    for (int r = 0; r < 10; ++r) {
+        u64 p0 = (u64)0xD2511F53u * c0;
+        u64 p1 = (u64)0xCD9E8D57u * c2;
+        u32 hi0 = (u32)(p0 >> 32), lo0 = (u32)p0;
+        u32 hi1 = (u32)(p1 >> 32), lo1 = (u32)p1;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* Counter-based sampler: the i-th draw of a path is word (i & 3) of
+ * Philox(ctr = {x, y, i >> 2, stream}, key = {seed, frame}). */
+struct PhiloxSampler {
+    u32 key[2];
+    u32 x, y, stream;
+    u32 i;
+    u32 block;
+    u32 buf[4];
+    PhiloxSampler(u32 seed, u32 frame, u32 px, u32 py, u32 st)
+        : x(px), y(py), stream(st), i(0), block(0xFFFFFFFFu) {
+        key[0] = seed;
+        key[1] = frame;
+    }
+    u32 next_u32() {
+        u32 b = i >> 2;
+        if (b != block) {
+            u32 c[4] = {x, y, b, stream};
+            philox(c, key, buf);
+            block = b;
+        }
+        return buf[(i++) & 3u];
+    }
+};
+
+/* Type-erased sampler interface used by the estimator. */
+struct Sampler {
+    int kind;
+    PcgSampler *pcg;
+    PhiloxSampler *phx;
+    u32 next_u32() { return kind == 0 ? pcg->next_u32() : phx->next_u32(); }
+    float next_float() { return bits_to_unit(next_u32()); }                  /* uniform_sampler.h:45-49 */
+    u32 next_discrete(u32 range) { return (u32)((float)range * next_float()); } /* :55-57 */
+};
+
+/* ------------------------------------------------------------------------ */
+/* Reconstruction filter and camera                                          */
+/* ------------------------------------------------------------------------ */
+
+/* reconstruction_filter.cpp:106-142 (tent, width 1, 32 bins). */
+struct TentFilter {
+    float width, bin_size, cdf[33];
+    TentFilter() {
+        width = 1.0f;
+        bin_size = width / (32 - 1);
+        float pdf[32];
+        float norm = 0.0f;
+        for (u32 i = 0; i < 31; ++i) {
+            float t = width - std::fabs(i * bin_size);
+            pdf[i] = t;
+            norm += t;
+        }
+        pdf[31] = 0.0f;
+        for (u32 i = 0; i < 31; ++i) pdf[i] /= norm;
+        cdf[0] = 0.0f;
+        for (u32 i = 1; i < 32; ++i) cdf[i] = cdf[i - 1] + pdf[i - 1];
+        cdf[32] = 1.0f;
+    }
+    /* reconstruction_filter.cpp:34-87 */
+    float sample(int type, float x) const {
+        if (type == 1) return 0.0f;          /* Dirac */
+        if (type == 2) return x - 0.5f;      /* Box, width 0.5 */
+        bool neg = x < 0.5f;
+        x = neg ? x * 2.0f : (x - 0.5f) * 2.0f;
+        u32 index = 32;
+        for (u32 i = 1; i < 33; ++i) {
+            if (x < cdf[i]) {
+                index = i - 1;
+                break;
+            }
+        }
+        /* index == 32 cannot occur for x < 1 because cdf[32] == 1. */
+        float pdf = cdf[index + 1] - cdf[index];
+        float u = bin_size * ((float)index + ((x - cdf[index]) / pdf));
+        return neg ? -u : u;
+    }
+};
+const TentFilter &tent() {
+    static TentFilter f;
+    return f;
+}
+
+struct Camera {
+    V3<float> origin, xa, ya, za;
+    float tanx, tany;
+    u32 w, h;
+    int filter;
+    /* pinhole_camera.cpp:27-40 and pinhole_camera.h:101-121 */
+    explicit Camera(const orc_camera_desc &d) {
+        w = d.width;
+        h = d.height;
+        filter = d.filter;
+        V3<float> target(d.target[0], d.target[1], d.target[2]);
+        float x = d.radius * sinf(d.phi) * sinf(d.theta);
+        float y = d.radius * cosf(d.phi);
+        float z = d.radius * sinf(d.phi) * cosf(d.theta);
+        origin = target + V3<float>(x, y, z);
+        za = normalize(origin - target);
+        if (za.x == 0.0f && (za.y == 1.0f || za.y == -1.0f) && za.z == 0.0f)
+            xa = normalize(cross(V3<float>(0.0f, 0.0f, 1.0f), za));
+        else
+            xa = normalize(cross(V3<float>(0.0f, 1.0f, 0.0f), za));
+        ya = cross(za, xa);
+        tanx = tanf(d.fov * 0.5f);
+        tany = tanf(d.fov * 0.5f * (float)d.height / (float)d.width);
+    }
+    /* pinhole_camera.cpp:88-113: unnormalized direction, tnear 0. */
+    void ray(u32 px, u32 py, float fu, float fv, V3<float> &o, V3<float> &dir) const {
+        float u = tent().sample(filter, fu);
+        float v = tent().sample(filter, fv);
+        V3<float> view((((float)px + 0.5f + u) / (float)w * 2.0f - 1.0f) * tanx,
+                       -((((float)py + 0.5f + v) / (float)h) * 2.0f - 1.0f) * tany, -1.0f);
+        V3<float> mx(xa.x, ya.x, za.x), my(xa.y, ya.y, za.y), mz(xa.z, ya.z, za.z);
+        o = origin;
+        dir = V3<float>(dot(view, mx), dot(view, my), dot(view, mz));
+    }
+};
+
+/* ------------------------------------------------------------------------ */
+/* Geometry generators (geometry_generator.cpp)                              */
+/* ------------------------------------------------------------------------ */
+struct MeshData {
+    std::vector<V3<float>> pos, nrm;
+    std::vector<float> uv;
+    std::vector<u32> idx;
+};
+
+/* geometry_generator.cpp:214-282 */
+void subdivide(MeshData &m) {
+    MeshData in = m;
+    m.pos.clear();
+    m.idx.clear();
+    size_t nt = in.idx.size() / 3;
+    for (size_t i = 0; i < nt; ++i) {
+        V3<float> v0 = in.pos[in.idx[i * 3 + 0]], v1 = in.pos[in.idx[i * 3 + 1]], v2 = in.pos[in.idx[i * 3 + 2]];
+        V3<float> m0(0.5f * (v0.x + v1.x), 0.5f * (v0.y + v1.y), 0.5f * (v0.z + v1.z));
+        V3<float> m1(0.5f * (v1.x + v2.x), 0.5f * (v1.y + v2.y), 0.5f * (v1.z + v2.z));
+        V3<float> m2(0.5f * (v0.x + v2.x), 0.5f * (v0.y + v2.y), 0.5f * (v0.z + v2.z));
+        m.pos.push_back(v0); m.pos.push_back(v1); m.pos.push_back(v2);
+        m.pos.push_back(m0); m.pos.push_back(m1); m.pos.push_back(m2);
+        u32 b = (u32)(i * 6);
+        u32 t[12] = {b + 0, b + 3, b + 5, b + 3, b + 4, b + 5, b + 5, b + 4, b + 2, b + 3, b + 1, b + 4};
+        m.idx.insert(m.idx.end(), t, t + 12);
+    }
+}
+
+/* geometry_generator.cpp:306-356 */
+MeshData geosphere(float radius, u32 n) {
+    const float X = 0.525731f, Z = 0.850651f;
+    V3<float> p[12] = {{-X, 0, -Z}, {X, 0, -Z}, {-X, 0, Z}, {X, 0, Z}, {0, Z, -X}, {0, Z, X},
+                       {0, -Z, -X}, {0, -Z, X}, {Z, X, 0}, {-Z, X, 0}, {Z, -X, 0}, {-Z, -X, 0}};
+    u32 k[60] = {1, 4, 0, 4, 9, 0, 4, 5, 9, 8, 5, 4, 1, 8, 4, 1, 10, 8, 10, 3, 8, 8, 3, 5, 3, 2, 5, 3, 7, 2,
+                 3, 10, 7, 10, 6, 7, 6, 11, 7, 6, 0, 11, 6, 1, 0, 10, 1, 6, 11, 0, 9, 2, 11, 9, 5, 2, 9, 11, 2, 7};
+    MeshData m;
+    m.pos.assign(p, p + 12);
+    m.idx.assign(k, k + 60);
+    for (u32 i = 0; i < n; ++i) subdivide(m);
+    m.nrm.resize(m.pos.size());
+    for (size_t i = 0; i < m.pos.size(); ++i) {
+        V3<float> nn = normalize(m.pos[i]);
+        m.pos[i] = radius * nn;
+        m.nrm[i] = nn;
+    }
+    return m;
+}
+
+/* geometry_generator.cpp:515-566 */
+MeshData grid(float width, float depth, u32 mm, u32 nn) {
+    MeshData m;
+    float hw = 0.5f * width, hd = 0.5f * depth;
+    float dx = width / (float)(nn - 1), dz = depth / (float)(mm - 1);
+    float du = 1.0f / (float)(nn - 1), dv = 1.0f / (float)(mm - 1);
+    m.pos.resize(mm * nn);
+    m.nrm.resize(mm * nn);
+    m.uv.resize(mm * nn * 2);
+    for (u32 i = 0; i < mm; ++i) {
+        float z = hd - (float)i * dz;
+        for (u32 j = 0; j < nn; ++j) {
+            float x = -hw + (float)j * dx;
+            m.pos[i * nn + j] = V3<float>(x, 0.0f, -z);
+            m.nrm[i * nn + j] = V3<float>(0.0f, 1.0f, 0.0f);
+            m.uv[(i * nn + j) * 2 + 0] = (float)j * du;
+            m.uv[(i * nn + j) * 2 + 1] = (float)i * dv;
+        }
+    }
+    for (u32 i = 0; i + 1 < mm; ++i)
+        for (u32 j = 0; j + 1 < nn; ++j) {
+            u32 t[6] = {i * nn + j, i * nn + j + 1, (i + 1) * nn + j, (i + 1) * nn + j, i * nn + j + 1, (i + 1) * nn + j + 1};
+            m.idx.insert(m.idx.end(), t, t + 6);
+        }
+    return m;
+}
+
+/* geometry_generator.cpp:22-121 */
+MeshData box(float width, float height, float depth) {
+    MeshData m;
+    float w2 = 0.5f * width, h2 = 0.5f * height, d2 = 0.5f * depth;
+    struct Vtx { float p[3], n[3], t[2]; };
+    const Vtx v[24] = {
+        {{-w2, -h2, +d2}, {0, 0, 1}, {0, 1}}, {{-w2, +h2, +d2}, {0, 0, 1}, {0, 0}}, {{+w2, +h2, +d2}, {0, 0, 1}, {1, 0}}, {{+w2, -h2, +d2}, {0, 0, 1}, {1, 1}},
+        {{-w2, -h2, -d2}, {0, 0, -1}, {1, 1}}, {{+w2, -h2, -d2}, {0, 0, -1}, {0, 1}}, {{+w2, +h2, -d2}, {0, 0, -1}, {0, 0}}, {{-w2, +h2, -d2}, {0, 0, -1}, {1, 0}},
+        {{-w2, +h2, +d2}, {0, 1, 0}, {0, 1}}, {{-w2, +h2, -d2}, {0, 1, 0}, {0, 0}}, {{+w2, +h2, -d2}, {0, 1, 0}, {1, 0}}, {{+w2, +h2, +d2}, {0, 1, 0}, {1, 1}},
+        {{-w2, -h2, +d2}, {0, -1, 0}, {1, 1}}, {{+w2, -h2, +d2}, {0, -1, 0}, {0, 1}}, {{+w2, -h2, -d2}, {0, -1, 0}, {0, 0}}, {{-w2, -h2, -d2}, {0, -1, 0}, {1, 0}},
+        {{-w2, -h2, -d2}, {-1, 0, 0}, {0, 1}}, {{-w2, +h2, -d2}, {-1, 0, 0}, {0, 0}}, {{-w2, +h2, +d2}, {-1, 0, 0}, {1, 0}}, {{-w2, -h2, +d2}, {-1, 0, 0}, {1, 1}},
+        {{+w2, -h2, +d2}, {1, 0, 0}, {0, 1}}, {{+w2, +h2, +d2}, {1, 0, 0}, {0, 0}}, {{+w2, +h2, -d2}, {1, 0, 0}, {1, 0}}, {{+w2, -h2, -d2}, {1, 0, 0}, {1, 1}},
+    };
+    for (int i = 0; i < 24; ++i) {
+        m.pos.push_back(V3<float>(v[i].p[0], v[i].p[1], v[i].p[2]));
+        m.nrm.push_back(V3<float>(v[i].n[0], v[i].n[1], v[i].n[2]));
+        m.uv.push_back(v[i].t[0]);
+        m.uv.push_back(v[i].t[1]);
+    }
+    for (u32 f = 0; f < 6; ++f) {
+        u32 b = f * 4;
+        u32 t[6] = {b, b + 1, b + 2, b, b + 2, b + 3};
+        m.idx.insert(m.idx.end(), t, t + 6);
+    }
+    return m;
+}
+
+/* scene.cpp:49-80 (Ritter-style sphere with two refinement passes). */
+void bounding_sphere(const V3<float> *p, size_t n, float out[4]) {
+    V3<float> c = p[0];
+    float r = 0.0001f;
+    for (int it = 0; it < 2; ++it)
+        for (size_t j = 0; j < n; ++j) {
+            V3<float> d = p[j] - c;
+            float len = length(d);
+            if (len > r) {
+                float a = len / r, a2 = a * a;
+                r = 0.5f * (a + 1 / a) * r;
+                c = 0.5f * ((1 + 1 / a2) * c + (1 - 1 / a2) * p[j]);
+            }
+        }
+    for (size_t j = 0; j < n; ++j) {
+        V3<float> d = p[j] - c;
+        float len = length(d);
+        if (len > r) {
+            r = (r + len) / 2.0f;
+            c = c + ((len - r) / len * d);
+        }
+    }
+    out[0] = c.x; out[1] = c.y; out[2] = c.z; out[3] = r;
+}
+
+/* scene.cpp:464-471 */
+float surface_area(const V3<float> *p, const u32 *idx, size_t ntris) {
+    float a = 0.0f;
+    for (size_t i = 0; i < ntris; ++i) {
+        V3<float> v0 = p[idx[3 * i]], v1 = p[idx[3 * i + 1]], v2 = p[idx[3 * i + 2]];
+        a += 0.5f * length(cross(v0 - v1, v0 - v2));
+    }
+    return a;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Scene                                                                     */
+/* ------------------------------------------------------------------------ */
+struct Geom {
+    u32 vbase, nv, tbase, nt;
+    bool has_nrm, has_uv;
+    int material;
+    int light;
+};
+struct Light {
+    V3<float> le;
+    float area;
+    float sphere[4];
+    u32 geom;
+};
+struct Texture { u32 w, h; std::vector<float> rgb; };
+struct Grid {
+    u32 nx, ny, nz;
+    std::vector<float> d;
+    float bmin[3], bmax[3];
+    float maxd;
+};
+struct Medium {
+    int type;
+    V3<float> sigma_a;   /* -log(color)/distance, medium.h:19-23 */
+    float sigma_s;       /* 1/scattering_distance, isotropic_scattering_medium.h:22-25 */
+    int grid;
+    float scale, albedo, g;
+};
+struct Tri { V3<float> v0, e1, e2, v1, v2; u32 geom, prim; };
+
+struct BvhNode { float lo[3], hi[3]; int left; int count; }; /* count>0: leaf, left = first tri */
+
+} // namespace
+
+struct orc_scene {
+    std::vector<V3<float>> pos, nrm;
+    std::vector<float> uv;
+    std::vector<u32> idx; /* global vertex indices */
+    std::vector<Geom> geoms;
+    std::vector<orc_material_desc> mats;
+    std::vector<Medium> media;
+    std::vector<Texture> texs;
+    std::vector<Grid> grids;
+    std::vector<Light> lights;
+    V3<float> background;
+    std::vector<Tri> tris;       /* in scene order: geom-major, prim-minor */
+    std::vector<u32> bvh_tris;   /* permutation for the BVH */
+    std::vector<BvhNode> nodes;
+    bool committed = false;
+};
+
+namespace {
+
+/* ------------------------------------------------------------------------ */
+/* Intersection: Embree's Moeller-Trumbore (triangle_intersector_moeller.h)  */
+/* ------------------------------------------------------------------------ */
+template <class R> struct RayR {
+    V3<R> o, d;
+    R tnear, tfar;
+};
+template <class R> struct HitR {
+    R t, u, v;
+    u32 geom, prim;
+};
+
+/* MoellerTrumboreIntersector1::intersect (triangle_intersector_moeller.h:75-113)
+ * with Ng = cross(e2,e1) (:122) and MoellerTrumboreHitM::finalize (:43-48).
+ * The rcp of finalize is taken as a correctly rounded division. Returns true
+ * and writes t,u,v when the triangle is accepted against [tnear, tfar]. */
+template <class R>
+inline bool moeller(const V3<R> &O, const V3<R> &D, R tnear, R tfar, const V3<R> &v0, const V3<R> &e1,
+                    const V3<R> &e2, R &t, R &u, R &v) {
+    V3<R> Ng = cross(e2, e1);
+    V3<R> C = v0 - O;
+    V3<R> Rv = cross(C, D);
+    R den = dot(Ng, D);
+    R absDen = std::fabs(den);
+    bool neg = std::signbit(den);
+    R U = dot(Rv, e2);
+    R V = dot(Rv, e1);
+    if (neg) { U = -U; V = -V; }
+    if (!(den != R(0) && U >= R(0) && V >= R(0) && U + V <= absDen)) return false;
+    R T = dot(Ng, C);
+    if (neg) T = -T;
+    if (!((absDen * tnear < T) && (T <= absDen * tfar))) return false;
+    t = T / absDen;
+    u = U / absDen;
+    v = V / absDen;
+    return true;
+}
+
+template <class R> inline void tri_edges(const Tri &tr, V3<R> &v0, V3<R> &e1, V3<R> &e2) {
+    v0 = cvt<R>(tr.v0);
+    e1 = cvt<R>(tr.e1);
+    e2 = cvt<R>(tr.e2);
+}
+/* FP64 mode rebuilds the edges in double from the float vertices. */
+template <> inline void tri_edges<double>(const Tri &tr, V3<double> &v0, V3<double> &e1, V3<double> &e2) {
+    v0 = cvt<double>(tr.v0);
+    e1 = v0 - cvt<double>(tr.v1);
+    e2 = cvt<double>(tr.v2) - v0;
+}
+
+template <class R>
+bool intersect_brute(const orc_scene &s, const RayR<R> &ray, HitR<R> &hit, u32 *ties) {
+    R tfar = ray.tfar;
+    bool found = false;
+    for (size_t i = 0; i < s.tris.size(); ++i) {
+        V3<R> v0, e1, e2;
+        tri_edges<R>(s.tris[i], v0, e1, e2);
+        R t, u, v;
+        if (moeller(ray.o, ray.d, ray.tnear, tfar, v0, e1, e2, t, u, v)) {
+            tfar = t;
+            hit.t = t; hit.u = u; hit.v = v;
+            hit.geom = s.tris[i].geom;
+            hit.prim = s.tris[i].prim;
+            found = true;
+        }
+    }
+    if (ties) {
+        u32 n = 0;
+        if (found)
+            for (size_t i = 0; i < s.tris.size(); ++i) {
+                V3<R> v0, e1, e2;
+                tri_edges<R>(s.tris[i], v0, e1, e2);
+                R t, u, v;
+                if (moeller(ray.o, ray.d, ray.tnear, ray.tfar, v0, e1, e2, t, u, v) && t == hit.t) ++n;
+            }
+        *ties = n;
+    }
+    return found;
+}
+
+/* Binned SAH BVH (16 bins, leaves of at most 4 triangles). */
+struct BuildItem { float lo[3], hi[3], c[3]; };
+
+void bvh_build(orc_scene &s) {
+    size_t n = s.tris.size();
+    std::vector<BuildItem> items(n);
+    for (size_t i = 0; i < n; ++i) {
+        const Tri &t = s.tris[i];
+        V3<float> v1 = s.pos[s.idx[3 * i + 1]], v2 = s.pos[s.idx[3 * i + 2]], v0 = s.pos[s.idx[3 * i]];
+        for (int a = 0; a < 3; ++a) {
+            items[i].lo[a] = std::min(v0[a], std::min(v1[a], v2[a]));
+            items[i].hi[a] = std::max(v0[a], std::max(v1[a], v2[a]));
+            items[i].c[a] = 0.5f * (items[i].lo[a] + items[i].hi[a]);
+        }
+        (void)t;
+    }
+    s.bvh_tris.resize(n);
+    for (size_t i = 0; i < n; ++i) s.bvh_tris[i] = (u32)i;
+    s.nodes.clear();
+    s.nodes.reserve(2 * n + 1);
+    struct Job { int node; u32 first, count; };
+    std::vector<Job> stack;
+    s.nodes.push_back(BvhNode());
+    stack.push_back({0, 0, (u32)n});
+    auto area = [](const float lo[3], const float hi[3]) {
+        float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (dx < 0) return 0.0f;
+        return 2.0f * (dx * dy + dy * dz + dz * dx);
+    };
+    while (!stack.empty()) {
+        Job j = stack.back();
+        stack.pop_back();
+        BvhNode nd;
+        float clo[3] = {1e30f, 1e30f, 1e30f}, chi[3] = {-1e30f, -1e30f, -1e30f};
+        for (int a = 0; a < 3; ++a) { nd.lo[a] = 1e30f; nd.hi[a] = -1e30f; }
+        for (u32 i = j.first; i < j.first + j.count; ++i) {
+            const BuildItem &it = items[s.bvh_tris[i]];
+            for (int a = 0; a < 3; ++a) {
+                nd.lo[a] = std::min(nd.lo[a], it.lo[a]);
+                nd.hi[a] = std::max(nd.hi[a], it.hi[a]);
+                clo[a] = std::min(clo[a], it.c[a]);
+                chi[a] = std::max(chi[a], it.c[a]);
+            }
+        }
+        int best_axis = -1, best_split = -1;
+        float best_cost = (float)j.count;
+        if (j.count > 1) {
+            const int NB = 16;
+            for (int a = 0; a < 3; ++a) {
+                float ext = chi[a] - clo[a];
+                if (ext <= 0) continue;
+                float blo[NB][3], bhi[NB][3];
+                u32 bc[NB] = {0};
+                for (int b = 0; b < NB; ++b)
+                    for (int k = 0; k < 3; ++k) { blo[b][k] = 1e30f; bhi[b][k] = -1e30f; }
+                for (u32 i = j.first; i < j.first + j.count; ++i) {
+                    const BuildItem &it = items[s.bvh_tris[i]];
+                    int b = std::min(NB - 1, (int)((it.c[a] - clo[a]) / ext * NB));
+                    bc[b]++;
+                    for (int k = 0; k < 3; ++k) {
+                        blo[b][k] = std::min(blo[b][k], it.lo[k]);
+                        bhi[b][k] = std::max(bhi[b][k], it.hi[k]);
+                    }
+                }
+                float ra[NB], rlo[3] = {1e30f, 1e30f, 1e30f}, rhi[3] = {-1e30f, -1e30f, -1e30f};
+                u32 rc[NB], cnt = 0;
+                for (int b = NB - 1; b > 0; --b) {
+                    for (int k = 0; k < 3; ++k) { rlo[k] = std::min(rlo[k], blo[b][k]); rhi[k] = std::max(rhi[k], bhi[b][k]); }
+                    cnt += bc[b];
+                    ra[b] = area(rlo, rhi);
+                    rc[b] = cnt;
+                }
+                float llo[3] = {1e30f, 1e30f, 1e30f}, lhi[3] = {-1e30f, -1e30f, -1e30f};
+                u32 lc = 0;
+                float pa = area(nd.lo, nd.hi);
+                for (int b = 0; b < NB - 1; ++b) {
+                    for (int k = 0; k < 3; ++k) { llo[k] = std::min(llo[k], blo[b][k]); lhi[k] = std::max(lhi[k], bhi[b][k]); }
+                    lc += bc[b];
+                    if (lc == 0 || rc[b + 1] == 0) continue;
+                    float cost = 1.0f + (area(llo, lhi) * lc + ra[b + 1] * rc[b + 1]) / std::max(pa, 1e-30f);
+                    if (cost < best_cost) { best_cost = cost; best_axis = a; best_split = b; }
+                }
+            }
+        }
+        bool make_leaf = j.count <= 4 && best_axis < 0;
+        if (j.count <= 1) make_leaf = true;
+        if (best_axis < 0 && !make_leaf) {
+            /* no useful SAH split (all centroids equal, or a leaf would be too big): median split */
+            best_axis = -2;
+        }
+        if (make_leaf) {
+            nd.left = (int)j.first;
+            nd.count = (int)j.count;
+            s.nodes[j.node] = nd;
+            continue;
+        }
+        u32 mid;
+        if (best_axis >= 0) {
+            float ext = chi[best_axis] - clo[best_axis];
+            auto it = std::partition(s.bvh_tris.begin() + j.first, s.bvh_tris.begin() + j.first + j.count, [&](u32 ti) {
+                int b = std::min(15, (int)((items[ti].c[best_axis] - clo[best_axis]) / ext * 16));
+                return b <= best_split;
+            });
+            mid = (u32)(it - s.bvh_tris.begin());
+        } else {
+            mid = j.first + j.count / 2;
+        }
+        if (mid == j.first || mid == j.first + j.count) mid = j.first + j.count / 2;
+        int l = (int)s.nodes.size();
+        s.nodes.push_back(BvhNode());
+        s.nodes.push_back(BvhNode());
+        nd.left = l;
+        nd.count = 0;
+        s.nodes[j.node] = nd;
+        stack.push_back({l, j.first, mid - j.first});
+        stack.push_back({l + 1, mid, j.first + j.count - mid});
+    }
+}
+
+template <class R>
+inline bool slab(const BvhNode &n, const V3<R> &o, const V3<R> &idir, R tmin, R tmax, R &tentry) {
+    R t0x = (R(n.lo[0]) - o.x) * idir.x, t1x = (R(n.hi[0]) - o.x) * idir.x;
+    R t0y = (R(n.lo[1]) - o.y) * idir.y, t1y = (R(n.hi[1]) - o.y) * idir.y;
+    R t0z = (R(n.lo[2]) - o.z) * idir.z, t1z = (R(n.hi[2]) - o.z) * idir.z;
+    R tn = std::max(std::max(std::min(t0x, t1x), std::min(t0y, t1y)), std::max(std::min(t0z, t1z), tmin));
+    R tf = std::min(std::min(std::max(t0x, t1x), std::max(t0y, t1y)), std::min(std::max(t0z, t1z), tmax));
+    tf = tf * R(1.0000005);
+    tentry = tn;
+    return tn <= tf;
+}
+
+template <class R> inline R safe_rcp(R d) {
+    const R m = R(1e-18);
+    if (std::fabs(d) < m) d = std::signbit(d) ? -m : m;
+    return R(1) / d;
+}
+
+template <class R>
+bool intersect_bvh(const orc_scene &s, const RayR<R> &ray, HitR<R> &hit) {
+    if (s.nodes.empty()) return false;
+    V3<R> idir(safe_rcp(ray.d.x), safe_rcp(ray.d.y), safe_rcp(ray.d.z));
+    R tfar = ray.tfar;
+    bool found = false;
+    int stack[128];
+    int sp = 0;
+    int node = 0;
+    R te;
+    if (!slab(s.nodes[0], ray.o, idir, std::max(ray.tnear, R(0)), tfar, te)) return false;
+    while (true) {
+        const BvhNode &n = s.nodes[node];
+        if (n.count > 0) {
+            for (int k = 0; k < n.count; ++k) {
+                u32 ti = s.bvh_tris[n.left + k];
+                V3<R> v0, e1, e2;
+                tri_edges<R>(s.tris[ti], v0, e1, e2);
+                R t, u, v;
+                if (moeller(ray.o, ray.d, ray.tnear, tfar, v0, e1, e2, t, u, v)) {
+                    tfar = t;
+                    hit.t = t; hit.u = u; hit.v = v;
+                    hit.geom = s.tris[ti].geom;
+                    hit.prim = s.tris[ti].prim;
+                    found = true;
+                }
+            }
+        } else {
+            R ta, tb;
+            bool ha = slab(s.nodes[n.left], ray.o, idir, std::max(ray.tnear, R(0)), tfar, ta);
+            bool hb = slab(s.nodes[n.left + 1], ray.o, idir, std::max(ray.tnear, R(0)), tfar, tb);
+            if (ha && hb) {
+                int nearc = n.left, farc = n.left + 1;
+                if (tb < ta) std::swap(nearc, farc);
+                stack[sp++] = farc;
+                node = nearc;
+                continue;
+            } else if (ha) {
+                node = n.left;
+                continue;
+            } else if (hb) {
+                node = n.left + 1;
+                continue;
+            }
+        }
+        if (sp == 0) break;
+        node = stack[--sp];
+    }
+    return found;
+}
+
+template <class R> inline bool intersect(const orc_scene &s, const RayR<R> &r, HitR<R> &h) {
+    return intersect_bvh<R>(s, r, h);
+}
+
+/* rtcInterpolate1 / TriangleMesh::interpolate (scene_triangle_mesh.cpp:241-247):
+ * P = fma(w,p0, fma(u,p1, v*p2)), w = 1-u-v. */
+template <class R> inline V3<R> interp_normal(const orc_scene &s, u32 geom, u32 prim, R u, R v) {
+    const Geom &g = s.geoms[geom];
+    const u32 *ix = &s.idx[3 * (g.tbase + prim)];
+    V3<R> p0 = cvt<R>(s.nrm[ix[0]]), p1 = cvt<R>(s.nrm[ix[1]]), p2 = cvt<R>(s.nrm[ix[2]]);
+    R w = R(1) - u - v;
+    return V3<R>(fmaR(w, p0.x, fmaR(u, p1.x, v * p2.x)), fmaR(w, p0.y, fmaR(u, p1.y, v * p2.y)),
+                 fmaR(w, p0.z, fmaR(u, p1.z, v * p2.z)));
+}
+template <class R> inline void interp_uv(const orc_scene &s, u32 geom, u32 prim, R u, R v, R &a, R &b) {
+    const Geom &g = s.geoms[geom];
+    const u32 *ix = &s.idx[3 * (g.tbase + prim)];
+    R w = R(1) - u - v;
+    a = fmaR(w, (R)s.uv[2 * ix[0]], fmaR(u, (R)s.uv[2 * ix[1]], v * (R)s.uv[2 * ix[2]]));
+    b = fmaR(w, (R)s.uv[2 * ix[0] + 1], fmaR(u, (R)s.uv[2 * ix[1] + 1], v * (R)s.uv[2 * ix[2] + 1]));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sampling helpers (sampling.h, vector_math.h/.cpp)                         */
+/* ------------------------------------------------------------------------ */
+enum { LOBE_NULL = 0, LOBE_SPEC_REFL = 1, LOBE_SPEC_TRANS = 2, LOBE_DIFFUSE = 4, LOBE_GLOSSY = 8, LOBE_SPECULAR = 3 };
+
+template <class R> inline R uni(Sampler &s) { return (R)s.next_float(); }
+
+/* vector_math.cpp:12-33 */
+template <class R> inline V3<R> rotate_to_world(R x, R y, R z, const V3<R> &n) {
+    V3<R> major;
+    const R k = R(0.57735026919f);
+    if (std::fabs(n.x) < k) major = V3<R>(1, 0, 0);
+    else if (std::fabs(n.y) < k) major = V3<R>(0, 1, 0);
+    else major = V3<R>(0, 0, 1);
+    V3<R> u = normalize(cross(n, major));
+    V3<R> v = cross(n, u);
+    return (u * x + v * y) + n * z;
+}
+/* sampling.h:31-38 */
+template <class R> inline void sample_disc(Sampler &s, R radius, R &x, R &y) {
+    R r = MathK<R>::sqrt(uni<R>(s)) * radius;
+    R sn, cs;
+    MathK<R>::sincos_turns(uni<R>(s), &sn, &cs);
+    x = r * cs;
+    y = r * sn;
+}
+/* sampling.h:50-60 */
+template <class R> inline V3<R> cosine_hemisphere(const V3<R> &n, Sampler &s) {
+    R x, y;
+    sample_disc<R>(s, R(1), x, y);
+    R z = MathK<R>::sqrt(R(1) - x * x - y * y);
+    return normalize(rotate_to_world(x, y, z, n));
+}
+/* sampling.h:74-84 */
+template <class R> inline V3<R> uniform_sphere(Sampler &s) {
+    R cp = R(2) * uni<R>(s) - R(1);
+    R sp = MathK<R>::sqrt(R(1) - cp * cp);
+    R sn, cs;
+    MathK<R>::sincos_turns(uni<R>(s), &sn, &cs);
+    return V3<R>(sp * sn, cp, sp * cs);
+}
+/* sampling.h:16-21 */
+template <class R> inline R power_heuristic(R f, R g) { return (f * f) / (f * f + g * g); }
+/* vector_math.h:33-56 */
+template <class R> inline V3<R> reflect(const V3<R> &V, const V3<R> &N) { return (R(2) * dot(V, N)) * N - V; }
+template <class R> inline V3<R> refract(const V3<R> &V, const V3<R> &N, R VdotN, R eta, R s2) {
+    return (eta * VdotN - MathK<R>::sqrt(R(1) - s2)) * N - eta * V;
+}
+template <class R> inline R sin2_t(R VdotN, R eta) { return eta * eta * (R(1) - VdotN * VdotN); }
+template <class R> inline R fresnel(R ni, R no, R VdotN, R s2) {
+    if (s2 > R(1)) return R(1);
+    R ct = MathK<R>::sqrt(R(1) - s2);
+    R rs = (ni * VdotN - no * ct) / (ni * VdotN + no * ct);
+    R rp = (no * VdotN - ni * ct) / (no * VdotN + ni * ct);
+    return R(0.5) * (rs * rs + rp * rp);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Interaction, textures and BSDFs                                           */
+/* ------------------------------------------------------------------------ */
+template <class R> struct Interaction {                 /* surface_interaction.h:16-31 */
+    V3<R> P, N, wi, wo;
+    R tu, tv;
+    int lobe = LOBE_NULL;
+    R iori = 0, ioro = 0;
+};
+
+template <class R> inline V3<R> albedo(const orc_scene &s, const orc_material_desc &m, const Interaction<R> &it) {
+    if (m.texture == ORC_TEX_CONSTANT) return V3<R>(m.albedo[0], m.albedo[1], m.albedo[2]);
+    if (m.texture == ORC_TEX_UV) return V3<R>(it.tu, it.tv, R(0));               /* uv_texture.h:16-18 */
+    /* image_texture.h:32-39 + image_cache.cpp:42-56: nearest texel, v flipped.
+     * Indices are clamped to the image (the reference reads past the row). */
+    const Texture &t = s.texs[m.texture];
+    R u = it.tu, v = R(1) - it.tv;
+    long x = (long)std::round(u * (R)t.w);
+    long y = (long)std::round(v * (R)t.h);
+    x = std::min<long>(std::max<long>(x, 0), (long)t.w - 1);
+    y = std::min<long>(std::max<long>(y, 0), (long)t.h - 1);
+    const float *p = &t.rgb[((size_t)y * t.w + (size_t)x) * 3];
+    return V3<R>(p[0], p[1], p[2]);
+}
+
+inline int supported_lobes(int bsdf) {
+    switch (bsdf) {
+    case ORC_BSDF_LAMBERT: return LOBE_DIFFUSE;
+    case ORC_BSDF_MIRROR: return LOBE_SPEC_REFL;
+    case ORC_BSDF_DIELECTRIC: return LOBE_SPECULAR;
+    default: return LOBE_GLOSSY;
+    }
+}
+
+/* GGX (Trowbridge-Reitz) microfacet conductor, no reference counterpart.
+ * D = a^2/(pi (c^2 (a^2-1) + 1)^2); separable Smith G1(c) = 2c/(c + sqrt(a^2 + (1-a^2)c^2));
+ * Schlick Fresnel with F0 = albedo; Eval carries cos(wi) like the reference
+ * BSDFs (lambert_bsdf.h:26-28). Sampling draws h from D(h)cos(h) (Walter et al. 2007). */
+template <class R> inline R ggx_d(R a2, R c) {
+    R t = fmaR(c * c, a2 - R(1), R(1));
+    return a2 / (R(M_PI) * t * t);
+}
+template <class R> inline R ggx_g1(R a2, R c) {
+    return (R(2) * c) / (c + MathK<R>::sqrt(fmaR(R(1) - a2, c * c, a2)));
+}
+
+template <class R>
+V3<R> bsdf_eval(const orc_scene &s, const orc_material_desc &m, const Interaction<R> &it) {
+    switch (m.bsdf) {
+    case ORC_BSDF_LAMBERT:                                                         /* lambert_bsdf.h:26-28 */
+        return (albedo<R>(s, m, it) * R(M_1_PI)) * dot(it.wi, it.N);
+    case ORC_BSDF_MIRROR:                                                          /* mirror_bsdf.h:27-29 */
+    case ORC_BSDF_DIELECTRIC:                                                      /* ideal_specular_dielectric.h:31-33 */
+        return albedo<R>(s, m, it);
+    default: {
+        R co = dot(it.N, it.wo), ci = dot(it.N, it.wi);
+        if (!(co > R(0)) || !(ci > R(0))) return V3<R>(R(0));
+        V3<R> h = normalize(it.wo + it.wi);
+        R ch = dot(it.N, h);
+        R a = (R)m.roughness, a2 = a * a;
+        R oh = std::max(dot(it.wi, h), R(0));
+        R om = R(1) - oh, om2 = om * om, om5 = om2 * om2 * om;
+        V3<R> f0 = albedo<R>(s, m, it);
+        V3<R> F = f0 + (V3<R>(R(1)) - f0) * om5;
+        R k = (ggx_d(a2, ch) * ggx_g1(a2, co) * ggx_g1(a2, ci)) / (R(4) * co);
+        return F * k;
+    }
+    }
+}
+
+template <class R>
+R bsdf_pdf(const orc_material_desc &m, const Interaction<R> &it) {
+    switch (m.bsdf) {
+    case ORC_BSDF_LAMBERT: return dot(it.wi, it.N) * R(M_1_PI);                   /* lambert_bsdf.h:36-38 */
+    case ORC_BSDF_MIRROR:
+    case ORC_BSDF_DIELECTRIC: return R(1);
+    default: {
+        R co = dot(it.N, it.wo), ci = dot(it.N, it.wi);
+        if (!(co > R(0)) || !(ci > R(0))) return R(0);
+        V3<R> h = normalize(it.wo + it.wi);
+        R ch = dot(it.N, h);
+        R a = (R)m.roughness, a2 = a * a;
+        return (ggx_d(a2, ch) * ch) / (R(4) * std::fabs(dot(it.wo, h)));
+    }
+    }
+}
+
+template <class R>
+void bsdf_sample(const orc_material_desc &m, Interaction<R> &it, Sampler &s) {
+    switch (m.bsdf) {
+    case ORC_BSDF_LAMBERT:                                                         /* lambert_bsdf.h:30-34 */
+        it.wi = cosine_hemisphere(it.N, s);
+        it.lobe = LOBE_DIFFUSE;
+        return;
+    case ORC_BSDF_MIRROR:                                                          /* mirror_bsdf.h:31-34 */
+        it.wi = reflect(it.wo, it.N);
+        it.lobe = LOBE_SPEC_REFL;
+        return;
+    case ORC_BSDF_DIELECTRIC: {                                                    /* ideal_specular_dielectric.h:35-64 */
+        R VdotN = dot(it.wo, it.N);
+        R ioro = (R)m.ior;
+        if (VdotN < R(0)) {
+            ioro = R(1);
+            it.N = -it.N;
+            VdotN = -VdotN;
+        }
+        R eta = it.iori / ioro;
+        R s2 = sin2_t(VdotN, eta);
+        R F = fresnel(it.iori, ioro, VdotN, s2);
+        R r = uni<R>(s);
+        if (r <= F) {
+            it.wi = reflect(it.wo, it.N);
+            it.lobe = LOBE_SPEC_REFL;
+            it.ioro = it.iori;
+        } else {
+            it.wi = refract(it.wo, it.N, VdotN, eta, s2);
+            it.lobe = LOBE_SPEC_TRANS;
+            it.ioro = ioro;
+        }
+        return;
+    }
+    default: {
+        R a = (R)m.roughness, a2 = a * a;
+        R u1 = uni<R>(s), u2 = uni<R>(s);
+        R ct = MathK<R>::sqrt((R(1) - u1) / fmaR(a2 - R(1), u1, R(1)));
+        R st = MathK<R>::sqrt(std::max(R(0), R(1) - ct * ct));
+        R sn, cs;
+        MathK<R>::sincos_turns(u2, &sn, &cs);
+        V3<R> h = rotate_to_world(st * cs, st * sn, ct, it.N);
+        it.wi = reflect(it.wo, h);
+        it.lobe = LOBE_GLOSSY;
+        return;
+    }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Area light (area_light.cpp)                                               */
+/* ------------------------------------------------------------------------ */
+template <class R> inline RayR<R> make_ray(V3<R> o, V3<R> d, R tnear) {
+    RayR<R> r;
+    r.o = o;
+    r.d = d;
+    r.tnear = tnear;
+    r.tfar = std::numeric_limits<R>::infinity();
+    return r;
+}
+
+struct Stats { u64 rays = 0; };
+
+/* AreaLight::SampleLi (area_light.cpp:18-72). */
+template <class R>
+V3<R> light_sample_li(const orc_scene &s, const Light &L, Sampler &smp, Interaction<R> &it, R &pdf, Stats &st) {
+    R x, y;
+    sample_disc<R>(smp, (R)L.sphere[3], x, y);
+    V3<R> cdir = V3<R>(L.sphere[0], L.sphere[1], L.sphere[2]) - it.P;
+    R dist = length(cdir);
+    V3<R> cn = normalize(cdir);
+    V3<R> pt = rotate_to_world(x, y, dist - (R)L.sphere[3], cn);
+    V3<R> dir = normalize(pt);
+    it.wi = dir;
+    if (dot(dir, it.N) <= R(0)) {
+        pdf = 0;
+        return V3<R>(R(0));
+    }
+    RayR<R> ray = make_ray(it.P, dir, R(0.001f));
+    HitR<R> h;
+    st.rays++;
+    if (!intersect(s, ray, h) || h.geom != L.geom) {
+        pdf = 0;
+        return V3<R>(R(0));
+    }
+    V3<R> ip = ray.o + normalize(ray.d) * h.t;
+    V3<R> dd = ip - it.P;
+    R d2 = dot(dd, dd);
+    pdf = d2 / (std::fabs(dot(normalize(interp_normal<R>(s, h.geom, h.prim, h.u, h.v)), -dir)) * (R)L.area);
+    return cvt<R>(L.le);
+}
+
+/* AreaLight::PdfLi (area_light.cpp:74-110). */
+template <class R>
+R light_pdf_li(const orc_scene &s, const Light &L, const Interaction<R> &it, Stats &st) {
+    if (dot(it.wi, it.N) <= R(0)) return R(0);
+    RayR<R> ray = make_ray(it.P, it.wi, R(0.001f));
+    HitR<R> h;
+    st.rays++;
+    if (!intersect(s, ray, h) || h.geom != L.geom) return R(0);
+    V3<R> ip = ray.o + normalize(ray.d) * h.t;
+    V3<R> dd = ip - it.P;
+    R d2 = dot(dd, dd);
+    return d2 / (std::fabs(dot(normalize(interp_normal<R>(s, h.geom, h.prim, h.u, h.v)), it.wi)) * (R)L.area);
+}
+
+/* Integrator::EstimateDirect (integrator.cpp:273-315). */
+template <class R>
+V3<R> estimate_direct(const orc_scene &s, const Light &L, Sampler &smp, Interaction<R> &it,
+                      const orc_material_desc &m, Stats &st) {
+    V3<R> direct(R(0));
+    R lpdf, spdf;
+    V3<R> f;
+    if ((supported_lobes(m.bsdf) & ~LOBE_SPECULAR) != 0) {
+        V3<R> Li = light_sample_li<R>(s, L, smp, it, lpdf, st);
+        if (lpdf != R(0) && !is_black(Li)) {
+            f = bsdf_eval<R>(s, m, it);
+            spdf = bsdf_pdf<R>(m, it);
+            if (spdf != R(0) && !is_black(f)) {
+                R w = power_heuristic(lpdf, spdf);
+                direct = direct + ((f * Li) * w) / lpdf;
+            }
+        }
+    }
+    bsdf_sample<R>(m, it, smp);
+    f = bsdf_eval<R>(s, m, it);
+    spdf = bsdf_pdf<R>(m, it);
+    if (spdf != R(0) && !is_black(f)) {
+        lpdf = light_pdf_li<R>(s, L, it, st);
+        if (lpdf == R(0)) return direct;
+        R w = power_heuristic(spdf, lpdf);
+        direct = direct + ((f * cvt<R>(L.le)) * w) / spdf;
+    }
+    return direct;
+}
+
+/* Integrator::SampleOneLight (integrator.cpp:252-271): interaction by value. */
+template <class R>
+V3<R> sample_one_light(const orc_scene &s, Sampler &smp, Interaction<R> it, const orc_material_desc &m,
+                       int hit_light, Stats &st) {
+    u32 nl = (u32)s.lights.size();
+    if (nl == 0 || (nl == 1 && hit_light >= 0)) return V3<R>(R(0));
+    u32 li;
+    do {
+        li = smp.next_discrete(nl);
+    } while ((int)li == hit_light);
+    return R((float)nl) * estimate_direct<R>(s, s.lights[li], smp, it, m, st);
+}
+
+/* Heterogeneous grid: nearest-voxel density (no reference counterpart). */
+inline float grid_density(const Grid &g, float px, float py, float pz) {
+    float p[3] = {px, py, pz};
+    int ix[3];
+    u32 n[3] = {g.nx, g.ny, g.nz};
+    for (int a = 0; a < 3; ++a) {
+        if (!(p[a] >= g.bmin[a] && p[a] <= g.bmax[a])) return 0.0f;
+        float f = (p[a] - g.bmin[a]) / (g.bmax[a] - g.bmin[a]) * (float)n[a];
+        int i = (int)f;
+        ix[a] = std::min(std::max(i, 0), (int)n[a] - 1);
+    }
+    return g.d[((size_t)ix[2] * g.ny + ix[1]) * g.nx + ix[0]];
+}
+
+/* Henyey-Greenstein direction about the propagation direction d (no reference
+ * counterpart): cos = (1 + g^2 - ((1-g^2)/(1-g+2gU))^2)/(2g). */
+template <class R> inline V3<R> sample_hg(R g, const V3<R> &d, Sampler &s) {
+    R u1 = uni<R>(s), u2 = uni<R>(s);
+    R ct;
+    if (std::fabs(g) < R(1e-3)) {
+        ct = R(1) - R(2) * u1;
+    } else {
+        R q = (R(1) - g * g) / (R(1) - g + R(2) * g * u1);
+        ct = (R(1) + g * g - q * q) / (R(2) * g);
+    }
+    ct = std::min(R(1), std::max(R(-1), ct));
+    R st = MathK<R>::sqrt(std::max(R(0), R(1) - ct * ct));
+    R sn, cs;
+    MathK<R>::sincos_turns(u2, &sn, &cs);
+    return normalize(rotate_to_world(st * cs, st * sn, ct, d));
+}
+
+/* Integrator::RenderPixel (integrator.cpp:98-250) for one path. */
+template <class R>
+u32 render_pixel(const orc_scene &s, const Camera &cam, u32 px, u32 py, Sampler &smp, const orc_render_params &p,
+                 V3<R> &color_out, Stats &st) {
+    float fu = smp.next_float(), fv = smp.next_float();
+    V3<float> o32, d32;
+    cam.ray(px, py, fu, fv, o32, d32);
+    RayR<R> ray = make_ray(cvt<R>(o32), cvt<R>(d32), R(0));
+    V3<R> color(R(0)), T(R(1));
+    Interaction<R> it;
+    it.iori = R(1);
+    int medium = -1;
+    u32 b = 0;
+    for (; b < p.max_depth; ++b) {
+        HitR<R> h;
+        st.rays++;
+        if (!intersect(s, ray, h)) {
+            color = color + T * cvt<R>(s.background);
+            break;
+        }
+        V3<R> origin = ray.o;
+        V3<R> dir = normalize(ray.d);
+        bool surface = true;
+        if (medium >= 0) {                                                          /* integrator.cpp:131-163 */
+            const Medium &md = s.media[medium];
+            if (md.type == ORC_MEDIUM_GRID) {
+                /* delta tracking through the majorant (no reference counterpart) */
+                const Grid &g = s.grids[md.grid];
+                R sbar = (R)md.scale * (R)g.maxd;
+                R t = 0;
+                bool collided = false;
+                if (sbar > R(0)) {
+                    while (true) {
+                        t = t - MathK<R>::log(R(1) - uni<R>(smp)) / sbar;
+                        if (!(t < h.t)) break;
+                        V3<R> q = origin + dir * t;
+                        R dens = (R)grid_density(g, (float)q.x, (float)q.y, (float)q.z);
+                        if (uni<R>(smp) * sbar < (R)md.scale * dens) {
+                            collided = true;
+                            break;
+                        }
+                    }
+                }
+                if (collided) {
+                    surface = false;
+                    T = T * (R)md.albedo;
+                    ray = make_ray(origin + dir * t, sample_hg<R>((R)md.g, dir, smp), R(0.001f));
+                }
+            } else {
+                R dist = h.t;
+                if (md.type == ORC_MEDIUM_ISOTROPIC) {                              /* isotropic_scattering_medium.h:31-40 */
+                    R d = -MathK<R>::log(uni<R>(smp)) / (R)md.sigma_s;
+                    dist = d >= h.t ? h.t : d;
+                }
+                V3<R> na = -cvt<R>(md.sigma_a);
+                V3<R> tr(MathK<R>::exp(na.x * dist), MathK<R>::exp(na.y * dist), MathK<R>::exp(na.z * dist));
+                T = (T * R(1)) * tr;
+                if (dist < h.t) {
+                    surface = false;
+                    ray = make_ray(origin + dir * dist, uniform_sphere<R>(smp), R(0.001f));
+                }
+            }
+        }
+        if (surface) {
+            const Geom &g = s.geoms[h.geom];
+            const orc_material_desc &m = s.mats[g.material];
+            int light = g.light;
+            if ((b == 0 || (it.lobe & LOBE_SPECULAR) != 0) && light >= 0)
+                color = color + T * cvt<R>(s.lights[light].le);
+            it.P = origin + dir * h.t;
+            if (g.has_nrm)
+                it.N = normalize(interp_normal<R>(s, h.geom, h.prim, h.u, h.v));
+            else
+                it.N = normalize(V3<R>(R(0)));
+            if (g.has_uv)
+                interp_uv<R>(s, h.geom, h.prim, h.u, h.v, it.tu, it.tv);
+            else
+                it.tu = it.tv = R(0);
+            it.wo = -dir;
+            it.ioro = R(0);
+            color = color + T * sample_one_light<R>(s, smp, it, m, light, st);
+            bsdf_sample<R>(m, it, smp);
+            R pdf = bsdf_pdf<R>(m, it);
+            if (pdf == R(0)) break; /* guard: the reference divides 0/0 here (measure zero for its BSDFs) */
+            T = (T * bsdf_eval<R>(s, m, it)) / pdf;
+            if (it.lobe == LOBE_SPEC_TRANS) {
+                it.iori = it.ioro;
+                medium = m.medium;
+            }
+            ray = make_ray(it.P, it.wi, R(0.001f));
+        }
+        if (b >= p.rr_start) {              /* integrator.cpp:235-242: `bounces > 3` is rr_start = 4 */
+            R pr = maxc(T);
+            if (uni<R>(smp) > pr || !(pr > R(0))) break;
+            T = T * (R(1) / pr);
+        }
+    }
+    color_out = color;
+    return b;
+}
+
+
+template <class R, class Acc>
+double render_impl(const orc_scene &s, const orc_camera_desc &cd, const orc_render_params &p, Acc *sum, u32 *bounces,
+                   u32 *count, u64 *rays) {
+    Camera cam(cd);
+    const u32 W = cd.width, H = cd.height;
+    int x0 = p.x0, y0 = p.y0, x1 = p.x1, y1 = p.y1;
+    if (x1 <= 0) { x0 = 0; y0 = 0; x1 = (int)W; y1 = (int)H; }
+    const u32 TS = 8;                                                              /* integrator.h:34 */
+    const u32 ntx = (W + TS - 1) / TS, nty = (H + TS - 1) / TS;
+    u32 nthreads = p.threads > 0 ? (u32)p.threads : std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<u64> total_rays(0);
+    unsigned int csr = _mm_getcsr();
+    auto t0 = std::chrono::steady_clock::now();
+    for (u32 fi = 0; fi < p.spp; ++fi) {
+        u32 frame = p.frame0 + fi;
+        std::atomic<u32> next(0);
+        auto worker = [&]() {
+            _mm_setcsr(_mm_getcsr() | 0x8040);  /* FTZ | DAZ, main.cpp:22-23 */
+            Stats st;
+            while (true) {
+                u32 ti = next.fetch_add(1);
+                if (ti >= ntx * nty) break;
+                u32 ty = ti / ntx, tx = ti - ty * ntx;
+                u32 bx0 = tx * TS, bx1 = std::min(bx0 + TS, W), by0 = ty * TS, by1 = std::min(by0 + TS, H);
+                if ((int)bx1 <= x0 || (int)bx0 >= x1 || (int)by1 <= y0 || (int)by0 >= y1) continue;
+                u32 hsh = hash_finalize(hash_mix(hash_mix(0u, ti), frame));     /* integrator.cpp:84-87 */
+                PcgSampler pcg(hsh, frame);
+                for (u32 y = by0; y < by1; ++y)
+                    for (u32 x = bx0; x < bx1; ++x) {
+                        if (p.rng != ORC_RNG_PCG_TILE && ((int)x < x0 || (int)x >= x1 || (int)y < y0 || (int)y >= y1)) continue;
+                        PhiloxSampler phx(p.seed, frame, x, y, p.stream);
+                        Sampler smp{p.rng == ORC_RNG_PCG_TILE ? 0 : 1, &pcg, &phx};
+                        V3<R> c;
+                        u32 nb = render_pixel<R>(s, cam, x, y, smp, p, c, st);
+                        if ((int)x < x0 || (int)x >= x1 || (int)y < y0 || (int)y >= y1) continue;
+                        size_t k = (size_t)y * W + x;
+                        sum[3 * k] += (Acc)c.x;
+                        sum[3 * k + 1] += (Acc)c.y;
+                        sum[3 * k + 2] += (Acc)c.z;
+                        bounces[k] += nb;
+                        count[k] += 1;
+                    }
+            }
+            total_rays += st.rays;
+        };
+        std::vector<std::thread> th;
+        for (u32 i = 0; i < nthreads; ++i) th.emplace_back(worker);
+        for (auto &t : th) t.join();
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    _mm_setcsr(csr);
+    if (rays) *rays = total_rays.load();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+} // namespace
+
+/* ======================================================================== */
+/* C interface                                                               */
+/* ======================================================================== */
+extern "C" {
+
+void orc_pcg32(uint64_t seed, uint64_t sequence, uint32_t n, uint32_t *out) {
+    PcgSampler s(seed, sequence);
+    for (u32 i = 0; i < n; ++i) out[i] = s.next_u32();
+}
+void orc_pcg32_float(uint64_t seed, uint64_t sequence, uint32_t n, float *out) {
+    PcgSampler s(seed, sequence);
+    for (u32 i = 0; i < n; ++i) out[i] = bits_to_unit(s.next_u32());
+}
+uint32_t orc_hash_mix(uint32_t h, uint32_t k) { return hash_mix(h, k); }
+uint32_t orc_hash_finalize(uint32_t h) { return hash_finalize(h); }
+uint32_t orc_tile_seed(uint32_t tile, uint32_t frame) { return hash_finalize(hash_mix(hash_mix(0u, tile), frame)); }
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { philox(ctr, key, out); }
+
+void orc_sincos_turns(float u, float *s, float *c) { sincos_turns_f(u, s, c); }
+float orc_logf(float x) { return log_f(x); }
+float orc_expf(float x) { return exp_f(x); }
+
+void orc_filter_cdf(float out[33]) { std::memcpy(out, tent().cdf, sizeof(float) * 33); }
+float orc_filter_sample(int32_t filter, float x) { return tent().sample(filter, x); }
+
+void orc_camera_basis(const orc_camera_desc *cd, float out[14]) {
+    Camera c(*cd);
+    float v[14] = {c.origin.x, c.origin.y, c.origin.z, c.xa.x, c.xa.y, c.xa.z, c.ya.x,
+                   c.ya.y, c.ya.z, c.za.x, c.za.y, c.za.z, c.tanx, c.tany};
+    std::memcpy(out, v, sizeof v);
+}
+void orc_camera_rays(const orc_camera_desc *cd, uint32_t n, const uint32_t *px, const uint32_t *py, const float *r0,
+                     const float *r1, float *rays) {
+    Camera c(*cd);
+    for (u32 i = 0; i < n; ++i) {
+        V3<float> o, d;
+        c.ray(px[i], py[i], r0[i], r1[i], o, d);
+        float v[8] = {o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, std::numeric_limits<float>::infinity()};
+        std::memcpy(rays + 8 * i, v, sizeof v);
+    }
+}
+
+static uint32_t export_mesh(const MeshData &m, float *pos, float *nrm, float *uv, uint32_t *idx, uint32_t *ntris) {
+    if (ntris) *ntris = (uint32_t)(m.idx.size() / 3);
+    for (size_t i = 0; i < m.pos.size(); ++i) {
+        if (pos) { pos[3 * i] = m.pos[i].x; pos[3 * i + 1] = m.pos[i].y; pos[3 * i + 2] = m.pos[i].z; }
+        if (nrm) { nrm[3 * i] = m.nrm[i].x; nrm[3 * i + 1] = m.nrm[i].y; nrm[3 * i + 2] = m.nrm[i].z; }
+    }
+    if (uv && !m.uv.empty()) std::memcpy(uv, m.uv.data(), m.uv.size() * sizeof(float));
+    if (idx) std::memcpy(idx, m.idx.data(), m.idx.size() * sizeof(u32));
+    return (uint32_t)m.pos.size();
+}
+uint32_t orc_geosphere(float radius, uint32_t subdiv, float *pos, float *nrm, uint32_t *idx, uint32_t *ntris) {
+    if (!pos && !nrm && !idx) {
+        if (ntris) *ntris = 20u << (2 * subdiv);
+        return subdiv == 0 ? 12u : (30u << (2 * subdiv));
+    }
+    return export_mesh(geosphere(radius, subdiv), pos, nrm, nullptr, idx, ntris);
+}
+uint32_t orc_grid(float width, float depth, uint32_t m, uint32_t n, float *pos, float *nrm, float *uv, uint32_t *idx,
+                  uint32_t *ntris) {
+    return export_mesh(grid(width, depth, m, n), pos, nrm, uv, idx, ntris);
+}
+uint32_t orc_box(float w, float h, float d, float *pos, float *nrm, float *uv, uint32_t *idx, uint32_t *ntris) {
+    return export_mesh(box(w, h, d), pos, nrm, uv, idx, ntris);
+}
+void orc_bounding_sphere(const float *pos, uint32_t nv, float out[4]) {
+    std::vector<V3<float>> p(nv);
+    for (u32 i = 0; i < nv; ++i) p[i] = V3<float>(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    bounding_sphere(p.data(), nv, out);
+}
+float orc_surface_area(const float *pos, const uint32_t *idx, uint32_t ntris) {
+    uint32_t nv = 0;
+    for (u32 i = 0; i < 3 * ntris; ++i) nv = std::max(nv, idx[i] + 1);
+    std::vector<V3<float>> p(nv);
+    for (u32 i = 0; i < nv; ++i) p[i] = V3<float>(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    return surface_area(p.data(), idx, ntris);
+}
+
+orc_scene *orc_scene_create(void) {
+    orc_scene *s = new orc_scene();
+    s->background = V3<float>(0.0f);
+    return s;
+}
+void orc_scene_destroy(orc_scene *s) { delete s; }
+
+int32_t orc_scene_add_mesh(orc_scene *s, const float *pos, uint32_t nv, const uint32_t *idx, uint32_t ntris,
+                           const float *nrm, const float *uv) {
+    Geom g;
+    g.vbase = (u32)s->pos.size();
+    g.nv = nv;
+    g.tbase = (u32)(s->idx.size() / 3);
+    g.nt = ntris;
+    g.has_nrm = nrm != nullptr;
+    g.has_uv = uv != nullptr;
+    g.material = -1;
+    g.light = -1;
+    for (u32 i = 0; i < nv; ++i) {
+        s->pos.push_back(V3<float>(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]));
+        s->nrm.push_back(nrm ? V3<float>(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]) : V3<float>(0.0f));
+        s->uv.push_back(uv ? uv[2 * i] : 0.0f);
+        s->uv.push_back(uv ? uv[2 * i + 1] : 0.0f);
+    }
+    for (u32 i = 0; i < 3 * ntris; ++i) {
+        if (idx[i] >= nv) return -1;
+        s->idx.push_back(g.vbase + idx[i]);
+    }
+    s->geoms.push_back(g);
+    s->committed = false;
+    return (int32_t)s->geoms.size() - 1;
+}
+int32_t orc_scene_add_material(orc_scene *s, const orc_material_desc *m) {
+    s->mats.push_back(*m);
+    return (int32_t)s->mats.size() - 1;
+}
+int32_t orc_scene_add_medium(orc_scene *s, const orc_medium_desc *d) {
+    Medium m;
+    m.type = d->type;
+    /* medium.h:19-23: sigma_a = -log(color)/distance (host libm) */
+    m.sigma_a = V3<float>(-logf(d->absorption_color[0]) / d->absorption_at_distance,
+                          -logf(d->absorption_color[1]) / d->absorption_at_distance,
+                          -logf(d->absorption_color[2]) / d->absorption_at_distance);
+    m.sigma_s = d->type == ORC_MEDIUM_ISOTROPIC ? 1.0f / d->scattering_distance : 0.0f;
+    m.grid = d->grid;
+    m.scale = d->sigma_t_scale;
+    m.albedo = d->albedo;
+    m.g = d->hg_g;
+    s->media.push_back(m);
+    return (int32_t)s->media.size() - 1;
+}
+int32_t orc_scene_add_texture(orc_scene *s, uint32_t w, uint32_t h, const float *rgb) {
+    Texture t;
+    t.w = w;
+    t.h = h;
+    t.rgb.assign(rgb, rgb + (size_t)w * h * 3);
+    s->texs.push_back(std::move(t));
+    return (int32_t)s->texs.size() - 1;
+}
+int32_t orc_scene_add_grid(orc_scene *s, uint32_t nx, uint32_t ny, uint32_t nz, const float *density,
+                           const float bmin[3], const float bmax[3]) {
+    Grid g;
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    g.d.assign(density, density + (size_t)nx * ny * nz);
+    g.maxd = 0.0f;
+    for (float v : g.d) g.maxd = std::max(g.maxd, v);
+    for (int a = 0; a < 3; ++a) { g.bmin[a] = bmin[a]; g.bmax[a] = bmax[a]; }
+    s->grids.push_back(std::move(g));
+    return (int32_t)s->grids.size() - 1;
+}
+int32_t orc_scene_set_material(orc_scene *s, int32_t geom, int32_t material) {
+    if (geom < 0 || geom >= (int32_t)s->geoms.size() || material < 0 || material >= (int32_t)s->mats.size()) return -1;
+    s->geoms[geom].material = material;
+    return 0;
+}
+int32_t orc_scene_set_emission(orc_scene *s, int32_t geom, const float color[3], float radiant_power) {
+    if (geom < 0 || geom >= (int32_t)s->geoms.size()) return -1;
+    const Geom &g = s->geoms[geom];
+    Light L;
+    std::vector<u32> local(3 * g.nt);
+    for (u32 i = 0; i < 3 * g.nt; ++i) local[i] = s->idx[3 * g.tbase + i] - g.vbase;
+    float area = surface_area(&s->pos[g.vbase], local.data(), g.nt);
+    bounding_sphere(&s->pos[g.vbase], g.nv, L.sphere);
+    /* area_light.h:16-21: Le = color * power * (1/pi) / area */
+    L.le = V3<float>(color[0] * radiant_power * (float)M_1_PI / area, color[1] * radiant_power * (float)M_1_PI / area,
+                     color[2] * radiant_power * (float)M_1_PI / area);
+    L.area = area;
+    L.geom = (u32)geom;
+    s->lights.push_back(L);
+    s->geoms[geom].light = (int)s->lights.size() - 1;
+    return (int32_t)s->lights.size() - 1;
+}
+void orc_scene_set_background(orc_scene *s, const float rgb[3]) { s->background = V3<float>(rgb[0], rgb[1], rgb[2]); }
+
+int32_t orc_scene_commit(orc_scene *s) {
+    for (const Geom &g : s->geoms)
+        if (g.material < 0) return -1;
+    s->tris.clear();
+    for (u32 gi = 0; gi < s->geoms.size(); ++gi) {
+        const Geom &g = s->geoms[gi];
+        for (u32 p = 0; p < g.nt; ++p) {
+            const u32 *ix = &s->idx[3 * (g.tbase + p)];
+            V3<float> v0 = s->pos[ix[0]], v1 = s->pos[ix[1]], v2 = s->pos[ix[2]];
+            Tri t;
+            t.v0 = v0;
+            t.e1 = v0 - v1;   /* triangle.h:53-54 */
+            t.e2 = v2 - v0;
+            t.v1 = v1;
+            t.v2 = v2;
+            t.geom = gi;
+            t.prim = p;
+            s->tris.push_back(t);
+        }
+    }
+    bvh_build(*s);
+    s->committed = true;
+    return 0;
+}
+
+uint32_t orc_scene_lights(const orc_scene *s, float *out) {
+    if (out)
+        for (size_t i = 0; i < s->lights.size(); ++i) {
+            const Light &L = s->lights[i];
+            float v[9] = {L.le.x, L.le.y, L.le.z, L.area, L.sphere[0], L.sphere[1], L.sphere[2], L.sphere[3], (float)L.geom};
+            std::memcpy(out + 9 * i, v, sizeof v);
+        }
+    return (uint32_t)s->lights.size();
+}
+
+void orc_intersect(const orc_scene *s, const float *rays, uint32_t n, int32_t use_bvh, float *out_tuv,
+                   uint32_t *out_geom, uint32_t *out_prim, uint32_t *out_ties) {
+    unsigned int csr = _mm_getcsr();
+    _mm_setcsr(csr | 0x8040);
+    for (u32 i = 0; i < n; ++i) {
+        const float *r = rays + 8 * i;
+        RayR<float> ray;
+        ray.o = V3<float>(r[0], r[1], r[2]);
+        ray.tnear = r[3];
+        ray.d = V3<float>(r[4], r[5], r[6]);
+        ray.tfar = r[7];
+        HitR<float> h;
+        bool ok = use_bvh ? intersect_bvh<float>(*s, ray, h) : intersect_brute<float>(*s, ray, h, out_ties ? &out_ties[i] : nullptr);
+        if (use_bvh && out_ties) out_ties[i] = 0;
+        out_tuv[3 * i] = ok ? h.t : r[7];
+        out_tuv[3 * i + 1] = ok ? h.u : 0.0f;
+        out_tuv[3 * i + 2] = ok ? h.v : 0.0f;
+        out_geom[i] = ok ? h.geom : 0xFFFFFFFFu;
+        out_prim[i] = ok ? h.prim : 0xFFFFFFFFu;
+    }
+    _mm_setcsr(csr);
+}
+void orc_intersect_f64(const orc_scene *s, const double *rays, uint32_t n, double *out_tuv, uint32_t *out_geom,
+                       uint32_t *out_prim) {
+    for (u32 i = 0; i < n; ++i) {
+        const double *r = rays + 8 * i;
+        RayR<double> ray;
+        ray.o = V3<double>(r[0], r[1], r[2]);
+        ray.tnear = r[3];
+        ray.d = V3<double>(r[4], r[5], r[6]);
+        ray.tfar = r[7];
+        HitR<double> h;
+        bool ok = intersect_bvh<double>(*s, ray, h);
+        out_tuv[3 * i] = ok ? h.t : r[7];
+        out_tuv[3 * i + 1] = ok ? h.u : 0.0;
+        out_tuv[3 * i + 2] = ok ? h.v : 0.0;
+        out_geom[i] = ok ? h.geom : 0xFFFFFFFFu;
+        out_prim[i] = ok ? h.prim : 0xFFFFFFFFu;
+    }
+}
+
+double orc_render_f32(const orc_scene *s, const orc_camera_desc *cam, const orc_render_params *p, float *sum,
+                      uint32_t *bounces, uint32_t *count, uint64_t *rays) {
+    if (!s->committed) return -1.0;
+    return render_impl<float, float>(*s, *cam, *p, sum, bounces, count, rays);
+}
+double orc_render_f64(const orc_scene *s, const orc_camera_desc *cam, const orc_render_params *p, double *sum,
+                      uint32_t *bounces, uint32_t *count, uint64_t *rays) {
+    if (!s->committed) return -1.0;
+    return render_impl<double, double>(*s, *cam, *p, sum, bounces, count, rays);
+}
+
+uint32_t orc_trace_path_f32(const orc_scene *s, const orc_camera_desc *cd, uint32_t px, uint32_t py,
+                            const orc_render_params *p, float *rgb) {
+    Camera cam(*cd);
+    PcgSampler pcg(0, 0);
+    PhiloxSampler phx(p->seed, p->frame0, px, py, p->stream);
+    Sampler smp{1, &pcg, &phx};
+    Stats st;
+    V3<float> c;
+    unsigned int csr = _mm_getcsr();
+    _mm_setcsr(csr | 0x8040);
+    u32 b = render_pixel<float>(*s, cam, px, py, smp, *p, c, st);
+    _mm_setcsr(csr);
+    rgb[0] = c.x; rgb[1] = c.y; rgb[2] = c.z;
+    return b;
+}
+
+uint32_t orc_hardware_threads(void) { return std::max(1u, std::thread::hardware_concurrency()); }
+
+} // extern "C"

@@ -1,0 +1,169 @@
+/*
+ * lt_oracle.h - C interface of the CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker: a plain CPU
+ * restatement of Lantern's rendering hot path (Integrator::RenderPixel,
+ * Scene::Intersect via Embree's Moeller-Trumbore arithmetic, PinholeCamera,
+ * ReconstructionFilter, UniformSampler, AreaLight, the BSDFs and media). Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * The product (lantern_amd/, liblantern_gpu.so) never links or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the RichieSams/lantern tree). Functions with no reference counterpart (the
+ * counter-based RNG, the GGX conductor, the heterogeneous grid medium) say so.
+ */
+#ifndef LT_ORACLE_H
+#define LT_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- random numbers ---------------------------------------------------- */
+
+/* UniformSampler (source/core/math/uniform_sampler.h:17-57): PCG32 XSH-RR.
+ * Writes n outputs of NextUInt() after construction with (seed, sequence). */
+void orc_pcg32(uint64_t seed, uint64_t sequence, uint32_t n, uint32_t *out);
+/* Same stream through NextFloat() (uniform_sampler.h:45-49). */
+void orc_pcg32_float(uint64_t seed, uint64_t sequence, uint32_t n, float *out);
+/* HashMix / HashFinalize (source/core/integrator/integrator.cpp:45-73). */
+uint32_t orc_hash_mix(uint32_t hash, uint32_t k);
+uint32_t orc_hash_finalize(uint32_t hash);
+/* Per-tile seed of RenderTile (integrator.cpp:84-87). */
+uint32_t orc_tile_seed(uint32_t tile_index, uint32_t frame);
+/* Philox-4x32-10 (Salmon et al., SC'11). No reference counterpart. */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* ---- float kernels used by the FP32 estimator --------------------------- */
+void orc_sincos_turns(float u, float *s, float *c);
+float orc_logf(float x);
+float orc_expf(float x);
+
+/* ---- camera ------------------------------------------------------------- */
+typedef struct orc_camera_desc {
+    float phi, theta, radius;   /* spherical orbit, scene.cpp:229-253 */
+    uint32_t width, height;
+    float target[3];
+    float fov;
+    int32_t filter;             /* 0 tent, 1 dirac, 2 box (reconstruction_filter.h:36-41) */
+} orc_camera_desc;
+
+/* ReconstructionFilter::PreCompute cdf (reconstruction_filter.cpp:106-142). */
+void orc_filter_cdf(float out[33]);
+/* ReconstructionFilter::Sample (reconstruction_filter.cpp:34-87). */
+float orc_filter_sample(int32_t filter, float x);
+/* PinholeCamera basis: origin[3], x/y/z axis[3 each], tanX, tanY
+ * (pinhole_camera.cpp:27-40, pinhole_camera.h:101-121). out: 14 floats. */
+void orc_camera_basis(const orc_camera_desc *cam, float out[14]);
+/* CalculateRayFromPixel (pinhole_camera.cpp:88-113) with the two filter
+ * inputs given explicitly. rays: n*8 floats (org xyz, tnear, dir xyz, tfar). */
+void orc_camera_rays(const orc_camera_desc *cam, uint32_t n, const uint32_t *px,
+                     const uint32_t *py, const float *r0, const float *r1, float *rays);
+
+/* ---- geometry generators (geometry_generator.cpp) ------------------------ */
+/* Return the vertex count; *ntris receives the triangle count. Pass NULL
+ * buffers to query sizes. Positions/normals are nv*3 floats, uvs nv*2. */
+uint32_t orc_geosphere(float radius, uint32_t subdiv, float *pos, float *nrm,
+                       uint32_t *idx, uint32_t *ntris);
+uint32_t orc_grid(float width, float depth, uint32_t m, uint32_t n, float *pos,
+                  float *nrm, float *uv, uint32_t *idx, uint32_t *ntris);
+uint32_t orc_box(float width, float height, float depth, float *pos, float *nrm,
+                 float *uv, uint32_t *idx, uint32_t *ntris);
+/* CalculateBoundingSphere (scene.cpp:49-80). out: center xyz, radius. */
+void orc_bounding_sphere(const float *pos, uint32_t nv, float out[4]);
+/* AddMesh surface area (scene.cpp:464-471). */
+float orc_surface_area(const float *pos, const uint32_t *idx, uint32_t ntris);
+
+/* ---- scenes ------------------------------------------------------------- */
+typedef struct orc_scene orc_scene;
+
+enum { ORC_BSDF_LAMBERT = 0, ORC_BSDF_MIRROR = 1, ORC_BSDF_DIELECTRIC = 2, ORC_BSDF_GGX = 3 };
+enum { ORC_MEDIUM_NON_SCATTERING = 0, ORC_MEDIUM_ISOTROPIC = 1, ORC_MEDIUM_GRID = 2 };
+enum { ORC_TEX_CONSTANT = -1, ORC_TEX_UV = -2 };
+
+typedef struct orc_material_desc {
+    int32_t bsdf;
+    float albedo[3];
+    int32_t texture;      /* ORC_TEX_CONSTANT, ORC_TEX_UV, or an image id */
+    float ior;
+    float roughness;      /* GGX alpha */
+    int32_t medium;       /* -1: none */
+} orc_material_desc;
+
+typedef struct orc_medium_desc {
+    int32_t type;
+    float absorption_color[3];
+    float absorption_at_distance;
+    float scattering_distance;
+    int32_t grid;          /* grid id for ORC_MEDIUM_GRID */
+    float sigma_t_scale;
+    float albedo;          /* single-scatter albedo */
+    float hg_g;
+} orc_medium_desc;
+
+orc_scene *orc_scene_create(void);
+void orc_scene_destroy(orc_scene *s);
+/* Positions are already transformed (AddMesh applies the transform first). */
+int32_t orc_scene_add_mesh(orc_scene *s, const float *pos, uint32_t nv, const uint32_t *idx,
+                           uint32_t ntris, const float *nrm, const float *uv);
+int32_t orc_scene_add_material(orc_scene *s, const orc_material_desc *m);
+int32_t orc_scene_add_medium(orc_scene *s, const orc_medium_desc *m);
+int32_t orc_scene_add_texture(orc_scene *s, uint32_t w, uint32_t h, const float *rgb);
+int32_t orc_scene_add_grid(orc_scene *s, uint32_t nx, uint32_t ny, uint32_t nz,
+                           const float *density, const float bmin[3], const float bmax[3]);
+int32_t orc_scene_set_material(orc_scene *s, int32_t geom, int32_t material);
+/* AreaLight (area_light.h:16-21; scene.cpp:437-444). */
+int32_t orc_scene_set_emission(orc_scene *s, int32_t geom, const float color[3], float radiant_power);
+void orc_scene_set_background(orc_scene *s, const float rgb[3]);
+int32_t orc_scene_commit(orc_scene *s);
+/* Light table: per light Le xyz, area, sphere xyzw, geomID -> 9 floats. */
+uint32_t orc_scene_lights(const orc_scene *s, float *out);
+
+/* Closest hit, rtcIntersect1 semantics (rtcore.cpp:300-317,
+ * triangle_intersector_moeller.h:75-113, intersector_epilog.h:241-305).
+ * rays: n*8 floats. out_tuv: n*3 floats; out_geom/out_prim: n u32
+ * (0xFFFFFFFF on miss). use_bvh: 0 brute force over all triangles in
+ * order, 1 the oracle's binned-SAH BVH. out_ties (nullable): number of
+ * triangles whose t equals the winning t exactly (brute force only). */
+void orc_intersect(const orc_scene *s, const float *rays, uint32_t n, int32_t use_bvh,
+                   float *out_tuv, uint32_t *out_geom, uint32_t *out_prim, uint32_t *out_ties);
+void orc_intersect_f64(const orc_scene *s, const double *rays, uint32_t n,
+                       double *out_tuv, uint32_t *out_geom, uint32_t *out_prim);
+
+/* ---- rendering ---------------------------------------------------------- */
+enum { ORC_RNG_PCG_TILE = 0, ORC_RNG_PHILOX = 1 };
+
+typedef struct orc_render_params {
+    uint32_t spp;          /* frames rendered (one sample per pixel each) */
+    uint32_t max_depth;    /* reference: 1500 (integrator.cpp:114) */
+    uint32_t rr_start;     /* RR when bounces >= rr_start; reference: 4 (integrator.cpp:235) */
+    uint32_t seed;         /* philox key[0] */
+    uint32_t frame0;       /* first frame number */
+    int32_t rng;           /* ORC_RNG_* */
+    int32_t threads;       /* 0: hardware_concurrency */
+    uint32_t stream;       /* philox ctr[3]: scene index inside a batch */
+    int32_t x0, y0, x1, y1;/* pixel window; x1 <= 0 means the full frame */
+} orc_render_params;
+
+/* RenderFrame x spp (integrator.cpp:28-43, 75-96, 98-250). Accumulates into
+ * sum (w*h*3), bounces (w*h), count (w*h) like FrameBuffer (frame_buffer.h:28-30).
+ * Returns the wall time in seconds of the frame loop. */
+double orc_render_f32(const orc_scene *s, const orc_camera_desc *cam, const orc_render_params *p,
+                      float *sum, uint32_t *bounces, uint32_t *count, uint64_t *rays_traced);
+double orc_render_f64(const orc_scene *s, const orc_camera_desc *cam, const orc_render_params *p,
+                      double *sum, uint32_t *bounces, uint32_t *count, uint64_t *rays_traced);
+
+/* One path with a caller-supplied list of uniforms (FP32 estimator). Used to
+ * check path-level behaviour against closed forms. Returns the bounce count. */
+uint32_t orc_trace_path_f32(const orc_scene *s, const orc_camera_desc *cam, uint32_t px,
+                            uint32_t py, const orc_render_params *p, float *rgb);
+
+uint32_t orc_hardware_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

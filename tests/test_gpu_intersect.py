@@ -1,0 +1,135 @@
+"""GPU closest-hit / any-hit parity against the oracle.
+
+Bar: primitive IDs identical to the oracle's brute-force closest hit except
+where several triangles share the winning t exactly (ties, whose winner
+depends on traversal order); t, u, v bit-identical (both sides evaluate
+Embree's Moeller-Trumbore with the same FMA forms and a correctly rounded
+division)."""
+import numpy as np
+import pytest
+
+import lantern_amd
+from lantern_amd import scenes
+from lantern_amd.scenes import SceneDesc
+from oracle.binding import OracleScene
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_rays(n, seed, lo=-8, hi=8):
+    rng = np.random.default_rng(seed)
+    r = np.zeros((n, 8), np.float32)
+    r[:, 0:3] = rng.uniform(lo, hi, (n, 3))
+    r[:, 3] = rng.choice([0.0, 0.001], n)
+    r[:, 4:7] = rng.normal(size=(n, 3))
+    r[:, 7] = np.where(rng.uniform(size=n) < 0.15, rng.uniform(0.5, 12, n), np.inf)
+    return r
+
+
+def _camera_rays(desc, n, seed):
+    import ctypes as C
+    from oracle import binding as ob
+    rng = np.random.default_rng(seed)
+    cam = desc.camera
+    px = rng.integers(0, cam["width"], n).astype(np.uint32)
+    py = rng.integers(0, cam["height"], n).astype(np.uint32)
+    r0 = rng.uniform(0, 1, n).astype(np.float32)
+    r1 = rng.uniform(0, 1, n).astype(np.float32)
+    rays = np.zeros((n, 8), np.float32)
+    ob.lib().orc_camera_rays(C.byref(ob.camera(cam)), n, ob.up(px), ob.up(py), ob.fp(r0), ob.fp(r1), ob.fp(rays))
+    return rays
+
+
+def _compare(gs, os_, rays, brute=True):
+    tuv_g, prim_g = gs.intersect(rays)
+    geom_g, lp_g = gs.split_prims(prim_g)
+    if brute:
+        tuv_o, geom_o, lp_o, ties = os_.intersect(rays, use_bvh=False, ties=True)
+    else:
+        tuv_o, geom_o, lp_o = os_.intersect(rays, use_bvh=True)
+        ties = np.zeros(len(rays), np.uint32)
+    hit_o = geom_o != 0xFFFFFFFF
+    hit_g = prim_g != 0xFFFFFFFF
+    assert np.array_equal(hit_o, hit_g), f"hit/miss differ on {np.sum(hit_o != hit_g)} rays"
+    ok = hit_o & (ties <= 1)
+    assert np.array_equal(geom_g[ok], geom_o[ok])
+    assert np.array_equal(lp_g[ok], lp_o[ok]), f"{np.sum(lp_g[ok] != lp_o[ok])} primitive mismatches"
+    assert np.array_equal(tuv_g[hit_o], tuv_o[hit_o]) or np.array_equal(tuv_g[ok], tuv_o[ok])
+    # misses report t = tfar
+    assert np.array_equal(tuv_g[~hit_g, 0], rays[~hit_g, 7])
+    return hit_o.sum(), (ties > 1).sum()
+
+
+def test_nine_sphere_random_rays_match_brute_force(gpu_device):
+    d = scenes.nine_sphere(64, 64, 1)
+    gs = lantern_amd.Scene(gpu_device, d)
+    os_ = OracleScene(d)
+    rays = _random_rays(3000, 11)
+    nhit, nties = _compare(gs, os_, rays)
+    assert nhit > 800
+    info = gs.info()
+    assert info["num_triangles"] == 46084 and info["max_depth"] <= 64
+    gs.close()
+
+
+def test_nine_sphere_camera_rays_match_brute_force(gpu_device):
+    d = scenes.nine_sphere(512, 512, 1)
+    gs = lantern_amd.Scene(gpu_device, d)
+    os_ = OracleScene(d)
+    rays = _camera_rays(d, 3000, 12)
+    nhit, _ = _compare(gs, os_, rays)
+    assert nhit > 1500
+    gs.close()
+
+
+def test_any_hit_agrees_with_closest_hit(gpu_device):
+    d = scenes.nine_sphere(64, 64, 1)
+    gs = lantern_amd.Scene(gpu_device, d)
+    rays = _random_rays(50000, 13)
+    _, prim = gs.intersect(rays)
+    occ = gs.intersect(rays, any_hit=True)
+    assert np.array_equal(occ == 1, prim != 0xFFFFFFFF)
+    gs.close()
+
+
+def test_triangle_heavy_full_size_matches_oracle_bvh(gpu_device):
+    d = scenes.triangle_heavy(64, 36, 1)
+    assert d.num_triangles == 327684
+    gs = lantern_amd.Scene(gpu_device, d)
+    os_ = OracleScene(d)
+    rays = np.concatenate([_random_rays(20000, 14, -3, 3), _camera_rays(d, 20000, 15)])
+    nhit, _ = _compare(gs, os_, rays, brute=False)
+    assert nhit > 20000
+    info = gs.info()
+    assert info["num_nodes"] > 0 and info["max_depth"] <= 64
+    gs.close()
+
+
+@pytest.mark.parametrize("ntris", [1, 2, 3, 5])
+def test_tiny_scenes(gpu_device, ntris):
+    rng = np.random.default_rng(ntris)
+    tris = rng.uniform(-1, 1, (ntris, 3, 3)).astype(np.float32)
+    d = SceneDesc()
+    m = d.add_material(bsdf=0, albedo=(0.5, 0.5, 0.5))
+    d.add_mesh(tris.reshape(-1, 3), np.arange(3 * ntris, dtype=np.uint32).reshape(-1, 3), material=m)
+    gs = lantern_amd.Scene(gpu_device, d)
+    os_ = OracleScene(d)
+    rays = _random_rays(4000, 20 + ntris, -2, 2)
+    _compare(gs, os_, rays)
+    gs.close()
+
+
+def test_inactive_rays_and_tfar_semantics(gpu_device):
+    d = SceneDesc()
+    m = d.add_material(bsdf=0, albedo=(0.5, 0.5, 0.5))
+    d.add_mesh(np.array([[0, 0, 5], [1, 0, 5], [0, 1, 5]], np.float32), np.array([[0, 1, 2]], np.uint32), material=m)
+    gs = lantern_amd.Scene(gpu_device, d)
+    rays = np.array([[0, 0, 0, 0.0, 0.25, 0.25, 5.0, np.inf],
+                     [0, 0, 0, 1.0, 0.25, 0.25, 5.0, np.inf],    # tnear == t: strict, miss
+                     [0, 0, 0, 0.0, 0.25, 0.25, 5.0, 1.0],       # tfar == t: inclusive, hit
+                     [0, 0, 0, 0.0, 0.25, 0.25, 5.0, 0.999],
+                     [0, 0, 0, 0.0, 0.25, 0.25, 5.0, -1.0]], np.float32)   # inactive
+    tuv, prim = gs.intersect(rays)
+    assert list(prim != 0xFFFFFFFF) == [True, False, True, False, False]
+    assert tuv[0, 0] == 1.0 and tuv[2, 0] == 1.0
+    gs.close()
