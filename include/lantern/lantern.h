@@ -1,0 +1,188 @@
+/*
+ * lantern/lantern.h - C++ host surface of the gfx950 rendering path.
+ *
+ * Mirrors the classes a Lantern application drives (source/lantern/main.cpp:21-84):
+ *   Scene          source/core/scene/scene.h:33-98        (LoadSceneFromJSON, Intersect, lights)
+ *   PinholeCamera  source/core/camera/pinhole_camera.h:22-122
+ *   FrameBuffer    source/core/camera/frame_buffer.h:17-34
+ *   Integrator     source/core/integrator/integrator.h:24-51 (RenderFrame)
+ *   UniformSampler source/core/math/uniform_sampler.h:15-71
+ * Same names, argument meanings and error behaviour (loaders print and return
+ * false). Embree types in the reference signatures are replaced by the C ABI's
+ * lt_ray / lt_hit (include/lantern_gpu.h). All rendering runs through the C ABI
+ * on the GPU; these classes hold host state only.
+ */
+#pragma once
+
+#include "../lantern_gpu.h"
+
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace Lantern {
+
+typedef unsigned int uint;
+
+struct float2 {
+    float x = 0.0f, y = 0.0f;
+    float2() = default;
+    float2(float a, float b) : x(a), y(b) {}
+};
+struct float3 {
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    float3() = default;
+    explicit float3(float a) : x(a), y(a), z(a) {}
+    float3(float a, float b, float c) : x(a), y(b), z(c) {}
+    float3 &operator+=(const float3 &o) { x += o.x; y += o.y; z += o.z; return *this; }
+};
+
+/* PCG32 XSH-RR exactly as uniform_sampler.h:17-57. */
+class UniformSampler {
+public:
+    UniformSampler(uint64_t seed, uint64_t sequence = 0) : m_state(seed), m_sequence(sequence) { NextUInt(); }
+    uint32_t NextUInt() {
+        uint64_t old = m_state;
+        m_state = old * 6364136223846793005ULL + (m_sequence | 1ULL);
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((0u - rot) & 31u));
+    }
+    float NextFloat() {
+        uint32_t b = (NextUInt() >> 9u) | 0x3F800000u;
+        float f;
+        __builtin_memcpy(&f, &b, 4);
+        return f - 1.0f;
+    }
+    float2 NextFloat2() { float a = NextFloat(); return float2(a, NextFloat()); }
+    uint NextDiscrete(uint range) { return (uint)((float)range * NextFloat()); }
+
+private:
+    uint64_t m_state;
+    const uint64_t m_sequence;
+};
+
+class ReconstructionFilter {
+public:
+    enum class Type { Dirac, Box, Tent, Gaussian };
+};
+
+class FrameBuffer {
+public:
+    FrameBuffer(uint width, uint height);
+    ~FrameBuffer();
+    FrameBuffer(const FrameBuffer &) = delete;
+    FrameBuffer &operator=(const FrameBuffer &) = delete;
+    FrameBuffer(FrameBuffer &&o) noexcept;
+
+public:
+    bool Empty;
+    uint Width;
+    uint Height;
+    float3 *ColorData;
+    uint *Bounces;
+    uint *ColorSampleCount;
+
+public:
+    void Reset();
+};
+
+class PinholeCamera {
+public:
+    PinholeCamera();
+    PinholeCamera(float phi, float theta, float radius, uint clientWidth, uint clientHeight,
+                  float3 target = float3(0.0f), float fov = (float)M_PI_2,
+                  ReconstructionFilter::Type filterType = ReconstructionFilter::Type::Tent);
+
+public:
+    uint FrameBufferWidth;
+    uint FrameBufferHeight;
+
+    void Rotate(float dPhi, float dTheta);
+    void Zoom(float distance);
+    void Pan(float dx, float dy);
+    /* pinhole_camera.cpp:88-113 on the host (two sampler draws, tent filter). */
+    lt_ray CalculateRayFromPixel(uint x, uint y, UniformSampler *sampler) const;
+    /* The C-ABI description the GPU path renders with. */
+    lt_camera Desc() const;
+
+private:
+    void UpdateOrigin();
+    void UpdateCartesianCoordSystem();
+    ReconstructionFilter::Type m_filterType;
+    float m_phi, m_theta, m_radius, m_up, m_fov;
+    float3 m_target, m_origin, m_xAxis, m_yAxis, m_zAxis;
+    float m_tanFovXDiv2, m_tanFovYDiv2;
+};
+
+class Scene {
+public:
+    /* Uses GPU 0; the second form shares a caller-owned device handle. */
+    Scene();
+    explicit Scene(lt_device *device);
+    ~Scene();
+    Scene(const Scene &) = delete;
+    Scene &operator=(const Scene &) = delete;
+
+public:
+    PinholeCamera *Camera;
+    float3 BackgroundColor;
+
+    /* scene.cpp:82-103: parse, validate, build, commit. Prints and returns false on error. */
+    bool LoadSceneFromJSON(const char *filePath);
+    bool ReloadSceneFromJSON();
+
+    std::size_t NumLights() const;
+    /* Scene::Intersect (scene.cpp:116-121): closest hit of one ray; on a miss
+     * hit.prim = LT_INVALID_ID. geomID/primID give the reference's ids. */
+    void Intersect(const lt_ray &ray, lt_hit &hit, uint *geomID = nullptr, uint *primID = nullptr) const;
+    /* Many rays per call (the fast path). */
+    bool Intersect(const lt_ray *rays, lt_hit *hits, uint32_t n) const;
+
+    lt_scene *Handle() const { return m_scene; }
+    lt_device *Device() const { return m_device; }
+    const std::string &LastError() const { return m_error; }
+
+private:
+    bool ParseJSON();
+    void CleanupScene();
+    std::string m_jsonPath;
+    lt_device *m_device;
+    bool m_ownsDevice;
+    lt_scene *m_scene;
+    std::size_t m_numLights;
+    mutable std::string m_error;
+};
+
+class Integrator {
+public:
+    Integrator(Scene *scene, FrameBuffer *currentFrameBuffer, std::atomic<FrameBuffer *> *swapFrameBuffer);
+    ~Integrator();
+
+    /* integrator.cpp:28-43: swap the frame buffer, render one sample per pixel
+     * (all pixels in one GPU launch sequence), add it to the frame buffer. */
+    void RenderFrame();
+    /* Extension: `frames` consecutive RenderFrame samples in one launch sequence. */
+    void RenderFrames(uint frames);
+
+    /* The reference hard-codes these (integrator.cpp:114, 235, 45-96). */
+    uint MaxBounces = 1500;
+    uint RussianRouletteStart = 4;
+    uint Seed = 0x4C414E54;
+    uint FrameNumber() const { return m_frameNumber; }
+    const lt_render_stats &LastStats() const { return m_stats; }
+
+private:
+    Scene *m_scene;
+    FrameBuffer *m_currentFrameBuffer;
+    std::atomic<FrameBuffer *> *m_swapFrameBuffer;
+    uint m_frameNumber;
+    lt_film *m_film;
+    std::vector<float> m_sum;
+    std::vector<uint32_t> m_bounces, m_counts;
+    lt_render_stats m_stats;
+};
+
+} // namespace Lantern

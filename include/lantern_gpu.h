@@ -189,6 +189,7 @@ typedef struct lt_camera {
     float target[3];
     float fov;
     int32_t filter;
+    float up;            /* PinholeCamera::m_up: -1 flips the world up axis; 0 or 1 keeps +y */
 } lt_camera;
 
 typedef struct lt_render_params {

@@ -49,7 +49,8 @@ class lt_hit(C.Structure):
 
 class lt_camera(C.Structure):
     _fields_ = [("phi", C.c_float), ("theta", C.c_float), ("radius", C.c_float), ("width", C.c_uint32),
-                ("height", C.c_uint32), ("target", C.c_float * 3), ("fov", C.c_float), ("filter", C.c_int32)]
+                ("height", C.c_uint32), ("target", C.c_float * 3), ("fov", C.c_float), ("filter", C.c_int32),
+                ("up", C.c_float)]
 
 
 class lt_render_params(C.Structure):

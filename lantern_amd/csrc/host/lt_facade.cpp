@@ -1,0 +1,627 @@
+/*
+ * lt_facade.cpp - the C++ host surface (include/lantern/lantern.h) over the C ABI.
+ *
+ * Scene::ParseJSON restates scene.cpp:154-449 on the lt_scene C ABI:
+ * schema checks of scenes/scene.schema.json, camera defaults (:229-253), BSDF,
+ * medium and material maps (:255-366), primitives (:368-446) with the row-major
+ * transform applied to positions on the host (AddMesh, :456-461; normals are
+ * copied as given, :481-482), LMF files (lantern_model_file.cpp:33-65) and
+ * emission -> AreaLight (:437-444).
+ *
+ * Extensions outside the reference schema (each optional):
+ *   bsdf "ggx_conductor" {albedo, roughness}; primitive "box" {width, height, depth};
+ *   medium "grid" {density_file (raw little-endian float32, x fastest), nx, ny, nz,
+ *   bounds_min, bounds_max, sigma_t_scale, albedo, hg_g};
+ *   albedo texture "image" reads a PFM (FP32) file, since the PNG assets and
+ *   stb_image are not part of this build.
+ */
+#include "../../../include/lantern/lantern.h"
+#include "lt_json.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace Lantern {
+
+namespace {
+
+bool read_lmf(const std::string &path, uint8_t &vpp, std::vector<float> &pos, std::vector<uint32_t> &idx,
+              std::vector<float> &nrm, std::vector<float> &uv, std::string &err) {
+    /* ReadLMF (lantern_model_file.cpp:33-65): magic "LMF\0", byte vertsPerPrim,
+     * u32 flags, u64 numPositions + floats[3n], u64 numIndices + u32[n],
+     * optional u64 + floats for normals (flag 1) and texcoords (flag 2). */
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) {
+        err = "Unable to open \"" + path + "\" for reading";
+        return false;
+    }
+    auto rd = [&](void *dst, size_t n) { return std::fread(dst, 1, n, f) == n; };
+    uint32_t magic = 0, flags = 0;
+    uint64_t n = 0;
+    bool ok = rd(&magic, 4) && magic == (uint32_t)('L' | ('M' << 8) | ('F' << 16));
+    if (!ok) err = "bad LMF magic";
+    if (ok) ok = rd(&vpp, 1) && rd(&flags, 4);
+    if (ok && (ok = rd(&n, 8)) && n < (1ull << 31)) {
+        pos.resize(n * 3);
+        ok = n == 0 || rd(pos.data(), n * 12);
+    }
+    if (ok && (ok = rd(&n, 8)) && n < (1ull << 32)) {
+        idx.resize(n);
+        ok = n == 0 || rd(idx.data(), n * 4);
+    }
+    if (ok && (flags & 1u)) {
+        ok = rd(&n, 8) && n < (1ull << 32);
+        if (ok) {
+            nrm.resize(n);
+            ok = n == 0 || rd(nrm.data(), n * 4);
+        }
+    }
+    if (ok && (flags & 2u)) {
+        ok = rd(&n, 8) && n < (1ull << 32);
+        if (ok) {
+            uv.resize(n);
+            ok = n == 0 || rd(uv.data(), n * 4);
+        }
+    }
+    std::fclose(f);
+    if (!ok && err.empty()) err = "truncated LMF file " + path;
+    return ok;
+}
+
+/* Portable float map (PF: RGB, Pf: gray); rows bottom-to-top per the format. */
+bool read_pfm(const std::string &path, uint32_t &w, uint32_t &h, std::vector<float> &rgb, std::string &err) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f.good()) {
+        err = "could not open " + path;
+        return false;
+    }
+    std::string kind;
+    double scale;
+    f >> kind >> w >> h >> scale;
+    f.get();
+    if ((kind != "PF" && kind != "Pf") || !f.good() || w == 0 || h == 0) {
+        err = "not a PFM file: " + path;
+        return false;
+    }
+    int ch = kind == "PF" ? 3 : 1;
+    std::vector<float> raw((size_t)w * h * ch);
+    f.read((char *)raw.data(), raw.size() * 4);
+    if (!f.good()) {
+        err = "truncated PFM file " + path;
+        return false;
+    }
+    rgb.resize((size_t)w * h * 3);
+    for (uint32_t y = 0; y < h; ++y)
+        for (uint32_t x = 0; x < w; ++x)
+            for (int c = 0; c < 3; ++c)
+                rgb[((size_t)(h - 1 - y) * w + x) * 3 + c] = raw[((size_t)y * w + x) * ch + (ch == 3 ? c : 0)];
+    return true;
+}
+
+std::string parent_dir(const std::string &p) {
+    size_t k = p.find_last_of('/');
+    return k == std::string::npos ? std::string(".") : p.substr(0, k);
+}
+
+float num(const lt_json::Value &v, float dflt = 0.0f) { return v.is_number() ? (float)v.num : dflt; }
+
+/* Checks the parts of scenes/scene.schema.json that the loader relies on. */
+bool validate(const lt_json::Value &j, std::string &err) {
+    if (!j.is_object()) { err = "scene is not a JSON object"; return false; }
+    static const char *top[] = {"background_color", "camera", "bsdfs", "media", "materials", "primitives"};
+    for (const auto &kv : j.obj) {
+        bool known = false;
+        for (const char *t : top) known |= kv.first == t;
+        if (!known) { err = "additional property not allowed: " + kv.first; return false; }
+    }
+    auto float3_ok = [](const lt_json::Value &v) {
+        return v.is_array() && v.size() == 3 && v[0].is_number() && v[1].is_number() && v[2].is_number();
+    };
+    if (j.count("background_color") && !float3_ok(j["background_color"])) { err = "background_color must be float3"; return false; }
+    const lt_json::Value &cam = j["camera"];
+    if (!cam.is_object()) { err = "\"camera\" is required"; return false; }
+    if (!cam["client_width"].is_number() || !cam["client_height"].is_number()) {
+        err = "camera requires client_width and client_height"; return false;
+    }
+    for (const char *k : {"phi", "theta", "radius", "fov"})
+        if (cam.count(k) && !cam[k].is_number()) { err = std::string("camera.") + k + " must be a number"; return false; }
+    if (cam.count("target") && !float3_ok(cam["target"])) { err = "camera.target must be float3"; return false; }
+    for (const auto &b : j["bsdfs"].arr) {
+        std::string t = b["type"].is_string() ? b["type"].str : "";
+        if (!b["name"].is_string() || !b["albedo"].is_object()) { err = "bsdf requires name, type and albedo"; return false; }
+        if (t == "ideal_specular_dielectric" && !b["ior"].is_number()) { err = "dielectric bsdf requires ior"; return false; }
+        if (t == "ggx_conductor" && !b["roughness"].is_number()) { err = "ggx_conductor requires roughness"; return false; }
+        if (t != "ideal_specular_dielectric" && t != "lambert" && t != "mirror" && t != "ggx_conductor") {
+            err = "unknown bsdf type: " + t; return false;
+        }
+        const lt_json::Value &a = b["albedo"];
+        std::string at = a["type"].is_string() ? a["type"].str : "";
+        if (at != "constant" && at != "image" && at != "uv") { err = "unknown albedo type: " + at; return false; }
+        if (at == "constant" && !float3_ok(a["value"])) { err = "constant albedo requires value"; return false; }
+    }
+    for (const auto &m : j["media"].arr) {
+        std::string t = m["type"].is_string() ? m["type"].str : "";
+        if (t == "non_scattering" || t == "isotropic_scattering") {
+            if (!float3_ok(m["absorption_color"]) || !m["absorption_at_distance"].is_number()) {
+                err = "medium requires absorption_color and absorption_at_distance"; return false;
+            }
+            if (t == "isotropic_scattering" && !m["scattering_distance"].is_number()) {
+                err = "isotropic_scattering requires scattering_distance"; return false;
+            }
+        } else if (t != "grid") {
+            err = "unknown medium type: " + t; return false;
+        }
+    }
+    for (const auto &m : j["materials"].arr)
+        if (!m["bsdf"].is_string()) { err = "material requires bsdf"; return false; }
+    for (const auto &p : j["primitives"].arr) {
+        std::string t = p["type"].is_string() ? p["type"].str : "";
+        if (!p["name"].is_string() || !p["material"].is_string()) { err = "primitive requires name and material"; return false; }
+        if (p.count("transform") && !(p["transform"].is_array() && p["transform"].size() == 16)) {
+            err = "transform must be float4x4"; return false;
+        }
+    }
+    return true;
+}
+
+} // namespace
+
+/* ---- FrameBuffer (frame_buffer.cpp:12-34) ------------------------------------ */
+FrameBuffer::FrameBuffer(uint width, uint height)
+    : Empty(true), Width(width), Height(height), ColorData(new float3[(size_t)width * height]),
+      Bounces(new uint[(size_t)width * height]), ColorSampleCount(new uint[(size_t)width * height]) {
+    Reset();
+}
+FrameBuffer::FrameBuffer(FrameBuffer &&o) noexcept
+    : Empty(o.Empty), Width(o.Width), Height(o.Height), ColorData(o.ColorData), Bounces(o.Bounces),
+      ColorSampleCount(o.ColorSampleCount) {
+    o.ColorData = nullptr;
+    o.Bounces = nullptr;
+    o.ColorSampleCount = nullptr;
+}
+FrameBuffer::~FrameBuffer() {
+    delete[] ColorData;
+    delete[] ColorSampleCount;
+    delete[] Bounces;
+}
+void FrameBuffer::Reset() {
+    size_t n = (size_t)Width * Height;
+    std::memset((void *)ColorData, 0, n * sizeof(float3));
+    std::memset(Bounces, 0, n * sizeof(uint));
+    std::memset(ColorSampleCount, 0, n * sizeof(uint));
+    Empty = true;
+}
+
+/* ---- PinholeCamera (pinhole_camera.cpp) --------------------------------------- */
+namespace {
+float3 sub(float3 a, float3 b) { return float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+float3 add(float3 a, float3 b) { return float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+float3 mul(float3 a, float s) { return float3(a.x * s, a.y * s, a.z * s); }
+float dotf(float3 a, float3 b) { return std::fma(a.x, b.x, std::fma(a.y, b.y, a.z * b.z)); }
+float3 crossf(float3 a, float3 b) {
+    return float3(std::fma(a.y, b.z, -(a.z * b.y)), std::fma(a.z, b.x, -(a.x * b.z)), std::fma(a.x, b.y, -(a.y * b.x)));
+}
+float3 normalizef(float3 a) { return mul(a, 1.0f / std::sqrt(dotf(a, a))); }
+} // namespace
+
+PinholeCamera::PinholeCamera()
+    : FrameBufferWidth(0), FrameBufferHeight(0), m_filterType(ReconstructionFilter::Type::Tent), m_phi((float)M_PI_2),
+      m_theta(0.0f), m_radius(10.0f), m_up(1.0f), m_fov((float)(M_PI / 3.0)), m_target(0.0f),
+      m_tanFovXDiv2(0.5773503f), m_tanFovYDiv2(tanf(0.5235f * 720 / 1280)) {
+    UpdateOrigin();
+    UpdateCartesianCoordSystem();
+}
+
+PinholeCamera::PinholeCamera(float phi, float theta, float radius, uint clientWidth, uint clientHeight, float3 target,
+                             float fov, ReconstructionFilter::Type filterType)
+    : FrameBufferWidth(clientWidth), FrameBufferHeight(clientHeight), m_filterType(filterType), m_phi(phi),
+      m_theta(theta), m_radius(radius), m_up(1.0f), m_fov(fov), m_target(target), m_tanFovXDiv2(tanf(fov * 0.5f)),
+      m_tanFovYDiv2(tanf(fov * 0.5f * clientHeight / clientWidth)) {
+    UpdateOrigin();
+    UpdateCartesianCoordSystem();
+}
+
+void PinholeCamera::UpdateOrigin() {
+    float x = m_radius * sinf(m_phi) * sinf(m_theta);
+    float y = m_radius * cosf(m_phi);
+    float z = m_radius * sinf(m_phi) * cosf(m_theta);
+    m_origin = add(m_target, float3(x, y, z));
+}
+
+void PinholeCamera::UpdateCartesianCoordSystem() {
+    m_zAxis = normalizef(sub(m_origin, m_target));
+    if (m_zAxis.x == 0.0f && (m_zAxis.y == 1.0f || m_zAxis.y == -1.0f) && m_zAxis.z == 0.0f)
+        m_xAxis = normalizef(crossf(float3(0.0f, 0.0f, m_up), m_zAxis));
+    else
+        m_xAxis = normalizef(crossf(float3(0.0f, m_up, 0.0f), m_zAxis));
+    m_yAxis = crossf(m_zAxis, m_xAxis);
+}
+
+/* pinhole_camera.cpp:42-67 */
+void PinholeCamera::Rotate(float dPhi, float dTheta) {
+    if (m_up > 0.0f) m_theta += dTheta;
+    else m_theta -= dTheta;
+    m_phi += dPhi;
+    if (m_phi > 2.0 * (float)M_PI) m_phi -= 2.0 * (float)M_PI;
+    else if (m_phi < -2.0 * (float)M_PI) m_phi += 2.0 * (float)M_PI;
+    if ((m_phi > 0 && m_phi < M_PI) || (m_phi < -M_PI && m_phi > -2.0 * M_PI)) m_up = 1.0f;
+    else m_up = -1.0f;
+    UpdateOrigin();
+    UpdateCartesianCoordSystem();
+}
+
+/* pinhole_camera.cpp:69-80 */
+void PinholeCamera::Zoom(float distance) {
+    m_radius -= distance;
+    if (m_radius <= 0.0f) {
+        m_radius = 30.0f;
+        m_target = sub(m_target, mul(m_zAxis, 30.0f));
+    }
+    UpdateOrigin();
+}
+
+/* pinhole_camera.cpp:82-86 */
+void PinholeCamera::Pan(float dx, float dy) {
+    m_target = add(m_target, add(mul(m_xAxis, dx), mul(m_yAxis, dy)));
+    UpdateOrigin();
+}
+
+lt_ray PinholeCamera::CalculateRayFromPixel(uint x, uint y, UniformSampler *sampler) const {
+    /* tent filter of width 1 (reconstruction_filter.cpp:34-87, 106-142) */
+    static float cdf[33];
+    static float bin = 0.0f;
+    if (bin == 0.0f) {
+        float b = 1.0f / 31, pdf[32], norm = 0.0f;
+        for (uint i = 0; i < 31; ++i) { pdf[i] = 1.0f - std::fabs((float)i * b); norm += pdf[i]; }
+        for (uint i = 0; i < 31; ++i) pdf[i] /= norm;
+        cdf[0] = 0.0f;
+        for (uint i = 1; i < 32; ++i) cdf[i] = cdf[i - 1] + pdf[i - 1];
+        cdf[32] = 1.0f;
+        bin = b;
+    }
+    auto sample = [&](float xs) {
+        if (m_filterType == ReconstructionFilter::Type::Dirac) return 0.0f;
+        if (m_filterType == ReconstructionFilter::Type::Box) return xs - 0.5f;
+        bool neg = xs < 0.5f;
+        xs = neg ? xs * 2.0f : (xs - 0.5f) * 2.0f;
+        uint index = 32;
+        for (uint i = 1; i < 33; ++i)
+            if (xs < cdf[i]) { index = i - 1; break; }
+        float pdf = cdf[index + 1] - cdf[index];
+        float u = bin * ((float)index + ((xs - cdf[index]) / pdf));
+        return neg ? -u : u;
+    };
+    float u = sample(sampler->NextFloat());
+    float v = sample(sampler->NextFloat());
+    float3 view((((float)x + 0.5f + u) / (float)FrameBufferWidth * 2.0f - 1.0f) * m_tanFovXDiv2,
+                -((((float)y + 0.5f + v) / (float)FrameBufferHeight) * 2.0f - 1.0f) * m_tanFovYDiv2, -1.0f);
+    lt_ray r;
+    r.org[0] = m_origin.x; r.org[1] = m_origin.y; r.org[2] = m_origin.z;
+    r.tnear = 0.0f;
+    r.dir[0] = dotf(view, float3(m_xAxis.x, m_yAxis.x, m_zAxis.x));
+    r.dir[1] = dotf(view, float3(m_xAxis.y, m_yAxis.y, m_zAxis.y));
+    r.dir[2] = dotf(view, float3(m_xAxis.z, m_yAxis.z, m_zAxis.z));
+    r.tfar = INFINITY;
+    return r;
+}
+
+lt_camera PinholeCamera::Desc() const {
+    lt_camera c;
+    c.phi = m_phi;
+    c.theta = m_theta;
+    c.radius = m_radius;
+    c.width = FrameBufferWidth;
+    c.height = FrameBufferHeight;
+    c.target[0] = m_target.x; c.target[1] = m_target.y; c.target[2] = m_target.z;
+    c.fov = m_fov;
+    c.filter = m_filterType == ReconstructionFilter::Type::Dirac ? 1 : m_filterType == ReconstructionFilter::Type::Box ? 2 : 0;
+    c.up = m_up;
+    return c;
+}
+
+/* ---- Scene ------------------------------------------------------------------------ */
+Scene::Scene() : Camera(nullptr), BackgroundColor(0.0f), m_device(nullptr), m_ownsDevice(true), m_scene(nullptr), m_numLights(0) {
+    if (lt_device_create(0, &m_device) != LT_OK) {
+        std::printf("lt_device_create failed: %s\n", lt_last_error());
+        m_device = nullptr;
+    }
+}
+Scene::Scene(lt_device *device)
+    : Camera(nullptr), BackgroundColor(0.0f), m_device(device), m_ownsDevice(false), m_scene(nullptr), m_numLights(0) {}
+Scene::~Scene() {
+    CleanupScene();
+    if (m_ownsDevice && m_device) lt_device_destroy(m_device);
+}
+void Scene::CleanupScene() {
+    delete Camera;
+    Camera = nullptr;
+    if (m_scene) lt_scene_destroy(m_scene);
+    m_scene = nullptr;
+    m_numLights = 0;
+}
+
+bool Scene::LoadSceneFromJSON(const char *filePath) {
+    m_jsonPath = filePath ? filePath : "";
+    return ParseJSON();
+}
+bool Scene::ReloadSceneFromJSON() {
+    CleanupScene();
+    return ParseJSON();
+}
+std::size_t Scene::NumLights() const { return m_numLights; }
+
+void Scene::Intersect(const lt_ray &ray, lt_hit &hit, uint *geomID, uint *primID) const {
+    hit.prim = LT_INVALID_ID;
+    hit.t = ray.tfar;
+    if (!m_scene || lt_intersect_closest(m_scene, &ray, &hit, 1) != LT_OK) {
+        m_error = lt_last_error();
+        return;
+    }
+    if (geomID || primID) {
+        uint32_t g = LT_INVALID_ID, p = LT_INVALID_ID;
+        lt_scene_split_prims(m_scene, &hit.prim, 1, &g, &p);
+        if (geomID) *geomID = g;
+        if (primID) *primID = p;
+    }
+}
+bool Scene::Intersect(const lt_ray *rays, lt_hit *hits, uint32_t n) const {
+    if (!m_scene) return false;
+    if (lt_intersect_closest(m_scene, rays, hits, n) != LT_OK) {
+        m_error = lt_last_error();
+        return false;
+    }
+    return true;
+}
+
+#define LT_FAIL(...)                          \
+    do {                                      \
+        char _b[512];                         \
+        std::snprintf(_b, sizeof _b, __VA_ARGS__); \
+        m_error = _b;                         \
+        std::printf("%s\n", _b);              \
+        return false;                         \
+    } while (0)
+
+bool Scene::ParseJSON() {
+    lt_json::Value j;
+    std::string err;
+    if (!lt_json::parse_file(m_jsonPath, j, err)) LT_FAIL("Could not parse %s: %s", m_jsonPath.c_str(), err.c_str());
+    if (!validate(j, err)) LT_FAIL("Schema validation failed: %s", err.c_str());
+    if (!m_device) LT_FAIL("no GPU device");
+    if (lt_scene_create(m_device, &m_scene) != LT_OK) LT_FAIL("lt_scene_create: %s", lt_last_error());
+
+    if (j.count("background_color") == 1) {
+        BackgroundColor = float3(num(j["background_color"][0]), num(j["background_color"][1]), num(j["background_color"][2]));
+    }
+    const lt_json::Value &cam = j["camera"];
+    float phi = (float)M_PI_4, theta = 0.0f, radius = 10.0f, fov = (float)M_PI_4;
+    float3 target(0.0f);
+    uint w = (uint)cam["client_width"].num, h = (uint)cam["client_height"].num;
+    if (cam.count("phi")) phi = num(cam["phi"]);
+    if (cam.count("theta")) theta = num(cam["theta"]);
+    if (cam.count("radius")) radius = num(cam["radius"]);
+    if (cam.count("fov")) fov = num(cam["fov"]);
+    if (cam.count("target")) target = float3(num(cam["target"][0]), num(cam["target"][1]), num(cam["target"][2]));
+    Camera = new PinholeCamera(phi, theta, radius, w, h, target, fov);
+
+    /* bsdfs (scene.cpp:261-316); a BSDF is stored as a material template */
+    std::map<std::string, lt_material> bsdfs;
+    std::string base = parent_dir(m_jsonPath);
+    for (const auto &b : j["bsdfs"].arr) {
+        lt_material m;
+        std::memset(&m, 0, sizeof m);
+        std::string t = b["type"].str;
+        m.bsdf = t == "lambert" ? LT_BSDF_LAMBERT : t == "mirror" ? LT_BSDF_MIRROR
+               : t == "ideal_specular_dielectric" ? LT_BSDF_DIELECTRIC : LT_BSDF_GGX;
+        m.ior = num(b["ior"], 1.0f);
+        m.roughness = num(b["roughness"], 0.0f);
+        m.medium = -1;
+        const lt_json::Value &a = b["albedo"];
+        if (a["type"].str == "constant") {
+            m.texture = LT_TEX_CONSTANT;
+            for (int k = 0; k < 3; ++k) m.albedo[k] = num(a["value"][k]);
+        } else if (a["type"].str == "uv") {
+            m.texture = LT_TEX_UV;
+        } else {
+            std::string path = a["file_path"].str;
+            if (!path.empty() && path[0] != '/') path = base + "/" + path;
+            uint32_t tw, th, id;
+            std::vector<float> rgb;
+            if (!read_pfm(path, tw, th, rgb, err)) LT_FAIL("image albedo: %s", err.c_str());
+            if (lt_scene_add_texture(m_scene, tw, th, rgb.data(), &id) != LT_OK) LT_FAIL("%s", lt_last_error());
+            m.texture = (int32_t)id;
+        }
+        bsdfs[b["name"].str] = m;
+    }
+    /* media (scene.cpp:318-339) */
+    std::map<std::string, uint32_t> media;
+    for (const auto &md : j["media"].arr) {
+        lt_medium m;
+        std::memset(&m, 0, sizeof m);
+        std::string t = md["type"].str;
+        m.grid = -1;
+        if (t == "grid") {
+            uint32_t nx = (uint32_t)num(md["nx"]), ny = (uint32_t)num(md["ny"]), nz = (uint32_t)num(md["nz"]);
+            std::string path = md["density_file"].str;
+            if (!path.empty() && path[0] != '/') path = base + "/" + path;
+            std::vector<float> dens((size_t)nx * ny * nz);
+            std::ifstream f(path, std::ios::binary);
+            if (!f.good() || dens.empty()) LT_FAIL("grid medium: could not read %s", path.c_str());
+            f.read((char *)dens.data(), dens.size() * 4);
+            if (!f.good()) LT_FAIL("grid medium: %s is shorter than %ux%ux%u floats", path.c_str(), nx, ny, nz);
+            float bmin[3], bmax[3];
+            for (int k = 0; k < 3; ++k) { bmin[k] = num(md["bounds_min"][k]); bmax[k] = num(md["bounds_max"][k]); }
+            uint32_t gid;
+            if (lt_scene_add_grid(m_scene, nx, ny, nz, dens.data(), bmin, bmax, &gid) != LT_OK) LT_FAIL("%s", lt_last_error());
+            m.type = LT_MEDIUM_GRID;
+            m.grid = (int32_t)gid;
+            m.sigma_t_scale = num(md["sigma_t_scale"], 1.0f);
+            m.albedo = num(md["albedo"], 1.0f);
+            m.hg_g = num(md["hg_g"], 0.0f);
+            m.absorption_color[0] = m.absorption_color[1] = m.absorption_color[2] = 1.0f;
+            m.absorption_at_distance = 1.0f;
+        } else {
+            m.type = t == "non_scattering" ? LT_MEDIUM_NON_SCATTERING : LT_MEDIUM_ISOTROPIC;
+            for (int k = 0; k < 3; ++k) m.absorption_color[k] = num(md["absorption_color"][k]);
+            m.absorption_at_distance = num(md["absorption_at_distance"]);
+            m.scattering_distance = num(md["scattering_distance"], 1.0f);
+        }
+        uint32_t id;
+        if (lt_scene_add_medium(m_scene, &m, &id) != LT_OK) LT_FAIL("%s", lt_last_error());
+        media[md["name"].str] = id;
+    }
+    /* materials (scene.cpp:341-366): unknown bsdf/medium names skip the material */
+    std::map<std::string, uint32_t> materials;
+    for (const auto &m : j["materials"].arr) {
+        std::string name = m["name"].str, bname = m["bsdf"].str;
+        auto it = bsdfs.find(bname);
+        if (it == bsdfs.end()) {
+            std::printf("BSDF [%s] could not be found for Material [%s]\n", bname.c_str(), name.c_str());
+            continue;
+        }
+        lt_material mat = it->second;
+        if (m.count("medium")) {
+            auto mi = media.find(m["medium"].str);
+            if (mi == media.end()) {
+                std::printf("Medium [%s] could not be found for Material [%s]\n", m["medium"].str.c_str(), name.c_str());
+                continue;
+            }
+            mat.medium = (int32_t)mi->second;
+        }
+        uint32_t id;
+        if (lt_scene_add_material(m_scene, &mat, &id) != LT_OK) LT_FAIL("%s", lt_last_error());
+        materials[name] = id;
+    }
+    /* primitives (scene.cpp:368-446) */
+    for (const auto &p : j["primitives"].arr) {
+        std::string name = p["name"].str, type = p["type"].str;
+        float T[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+        if (p.count("transform"))
+            for (int k = 0; k < 16; ++k) T[k] = num(p["transform"][k]);
+        std::vector<float> pos, nrm, uv;
+        std::vector<uint32_t> idx;
+        uint32_t vpp = 3;
+        if (type == "lmf") {
+            std::string path = p["file_path"].str;
+            if (!path.empty() && path[0] != '/') path = base + "/" + path;
+            uint8_t v8 = 0;
+            if (!read_lmf(path, v8, pos, idx, nrm, uv, err)) {
+                std::printf("%s\n", err.c_str());
+                continue;
+            }
+            if (v8 != 3 && v8 != 4) {
+                std::printf("Lantern only supports 3 or 4 vertices per primitive. Given [%hhu]\n", v8);
+                continue;
+            }
+            vpp = v8;
+        } else if (type == "grid" || type == "geosphere" || type == "box") {
+            uint32_t nv, nt;
+            if (type == "grid") {
+                uint32_t gn = (uint32_t)num(p["n"]), gm = (uint32_t)num(p["m"]);
+                nv = lt_geometry_grid(num(p["width"]), num(p["depth"]), gm, gn, nullptr, nullptr, nullptr, nullptr, &nt);
+                pos.resize(nv * 3); nrm.resize(nv * 3); uv.resize(nv * 2); idx.resize(nt * 3);
+                lt_geometry_grid(num(p["width"]), num(p["depth"]), gm, gn, pos.data(), nrm.data(), uv.data(), idx.data(), &nt);
+            } else if (type == "geosphere") {
+                uint32_t n = (uint32_t)num(p["n"]);
+                nv = lt_geometry_geosphere(num(p["radius"]), n, nullptr, nullptr, nullptr, &nt);
+                pos.resize(nv * 3); nrm.resize(nv * 3); idx.resize(nt * 3);
+                lt_geometry_geosphere(num(p["radius"]), n, pos.data(), nrm.data(), idx.data(), &nt);
+            } else {
+                nv = lt_geometry_box(num(p["width"]), num(p["height"]), num(p["depth"]), nullptr, nullptr, nullptr, nullptr, &nt);
+                pos.resize(nv * 3); nrm.resize(nv * 3); uv.resize(nv * 2); idx.resize(nt * 3);
+                lt_geometry_box(num(p["width"]), num(p["height"]), num(p["depth"]), pos.data(), nrm.data(), uv.data(), idx.data(), &nt);
+            }
+            if (nt == 0) {
+                std::printf("Primitive [%s] is empty\n", name.c_str());
+                continue;
+            }
+        } else {
+            std::printf("Unknown primitive type: [%s]\n", type.c_str());
+            continue;
+        }
+        /* AddMesh: vertices[j] = transform * float3a(p, 1) with LinearSpace4's
+         * column form b.x*vx + b.y*vy + b.z*vz + b.w*vw (linearspace4.h:165) */
+        for (size_t v = 0; v < pos.size() / 3; ++v) {
+            float x = pos[3 * v], y = pos[3 * v + 1], z = pos[3 * v + 2];
+            float o[3];
+            for (int r = 0; r < 3; ++r) o[r] = ((x * T[4 * r] + y * T[4 * r + 1]) + z * T[4 * r + 2]) + 1.0f * T[4 * r + 3];
+            pos[3 * v] = o[0]; pos[3 * v + 1] = o[1]; pos[3 * v + 2] = o[2];
+        }
+        auto mi = materials.find(p["material"].str);
+        if (mi == materials.end()) LT_FAIL("Material [%s] of primitive [%s] is not defined", p["material"].str.c_str(), name.c_str());
+        uint32_t nprims = (uint32_t)(idx.size() / vpp), geom;
+        if (lt_scene_add_mesh(m_scene, pos.data(), 12, (uint32_t)(pos.size() / 3), idx.data(), nprims, vpp,
+                              nrm.empty() ? nullptr : nrm.data(), uv.empty() ? nullptr : uv.data(), &geom) != LT_OK)
+            LT_FAIL("Primitive [%s]: %s", name.c_str(), lt_last_error());
+        lt_scene_set_material(m_scene, geom, mi->second);
+        if (p.count("emission") == 1) {
+            const lt_json::Value &e = p["emission"];
+            float col[3] = {num(e["color"][0]), num(e["color"][1]), num(e["color"][2])};
+            if (lt_scene_set_emission(m_scene, geom, col, num(e["radiant_power"])) != LT_OK)
+                LT_FAIL("Primitive [%s]: %s", name.c_str(), lt_last_error());
+            ++m_numLights;
+        }
+    }
+    float bg[3] = {BackgroundColor.x, BackgroundColor.y, BackgroundColor.z};
+    lt_scene_set_background(m_scene, bg);
+    if (lt_scene_commit(m_scene, nullptr) != LT_OK) LT_FAIL("rtcCommitScene equivalent failed: %s", lt_last_error());
+    return true;
+}
+
+/* ---- Integrator (integrator.cpp:28-43) --------------------------------------------- */
+Integrator::Integrator(Scene *scene, FrameBuffer *currentFrameBuffer, std::atomic<FrameBuffer *> *swapFrameBuffer)
+    : m_scene(scene), m_currentFrameBuffer(currentFrameBuffer), m_swapFrameBuffer(swapFrameBuffer), m_frameNumber(0u),
+      m_film(nullptr) {
+    std::memset(&m_stats, 0, sizeof m_stats);
+}
+Integrator::~Integrator() {
+    if (m_film) lt_film_destroy(m_film);
+}
+
+void Integrator::RenderFrame() { RenderFrames(1); }
+
+void Integrator::RenderFrames(uint frames) {
+    if (!m_scene || !m_scene->Handle() || !m_scene->Camera || frames == 0) return;
+    const uint w = m_scene->Camera->FrameBufferWidth, h = m_scene->Camera->FrameBufferHeight;
+    if (!m_film) {
+        if (lt_film_create(m_scene->Device(), w, h, &m_film) != LT_OK) {
+            std::printf("lt_film_create: %s\n", lt_last_error());
+            return;
+        }
+    } else {
+        lt_film_reset(m_film);
+    }
+    if (m_swapFrameBuffer) m_currentFrameBuffer = std::atomic_exchange(m_swapFrameBuffer, m_currentFrameBuffer);
+    m_currentFrameBuffer->Empty = false;
+    lt_camera cam = m_scene->Camera->Desc();
+    lt_render_params p;
+    std::memset(&p, 0, sizeof p);
+    p.spp = frames;
+    p.max_depth = MaxBounces;
+    p.rr_start = RussianRouletteStart;
+    p.seed = Seed;
+    p.frame0 = m_frameNumber;
+    if (lt_render(m_scene->Handle(), &cam, &p, m_film, &m_stats) != LT_OK) {
+        std::printf("lt_render: %s\n", lt_last_error());
+        return;
+    }
+    size_t n = (size_t)w * h;
+    m_sum.resize(n * 3);
+    m_bounces.resize(n);
+    m_counts.resize(n);
+    lt_film_download(m_film, m_sum.data(), m_bounces.data(), m_counts.data());
+    FrameBuffer *fb = m_currentFrameBuffer;
+    for (size_t i = 0; i < n; ++i) {
+        fb->ColorData[i] += float3(m_sum[3 * i], m_sum[3 * i + 1], m_sum[3 * i + 2]);
+        fb->Bounces[i] += m_bounces[i];
+        fb->ColorSampleCount[i] += m_counts[i];
+    }
+    m_frameNumber += frames;
+}
+
+} // namespace Lantern

@@ -538,8 +538,9 @@ void camera_setup(const lt_camera &c, CameraSetup &o) {
     V3 origin = target + v3(x, y, z);
     V3 za = normalize(origin - target);
     V3 xa;
-    if (za.x == 0.0f && (za.y == 1.0f || za.y == -1.0f) && za.z == 0.0f) xa = normalize(cross(v3(0.0f, 0.0f, 1.0f), za));
-    else xa = normalize(cross(v3(0.0f, 1.0f, 0.0f), za));
+    const float up = c.up < 0.0f ? -1.0f : 1.0f;
+    if (za.x == 0.0f && (za.y == 1.0f || za.y == -1.0f) && za.z == 0.0f) xa = normalize(cross(v3(0.0f, 0.0f, up), za));
+    else xa = normalize(cross(v3(0.0f, up, 0.0f), za));
     V3 ya = cross(za, xa);
     o.origin[0] = origin.x; o.origin[1] = origin.y; o.origin[2] = origin.z;
     o.mx[0] = xa.x; o.mx[1] = ya.x; o.mx[2] = za.x;
