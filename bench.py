@@ -38,7 +38,12 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build renders on one GPU; --gpus must be 1 (multi-GPU sharding is a design "
                                    "note: split samples or scenes across GPUs and sum the films)"}))
@@ -127,6 +132,19 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def copy_probe(nbytes):
+    """A plain copy kernel of known size for scaling FETCH_SIZE / WRITE_SIZE:
+    each dispatch reads and writes exactly nbytes."""
+    import torch
+    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes}))
 
 
 def cpu_baseline(desc, params, seconds):
