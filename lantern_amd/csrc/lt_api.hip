@@ -449,6 +449,30 @@ int lt_scene_commit(lt_scene *s, const lt_build_params *params) {
     for (const DMedium &m : s->media)
         if (m.type == MEDIUM_GRID && (m.grid < 0 || (size_t)m.grid >= s->grids.size()))
             return set_error(LT_E_INVALID, "lt_scene_commit: grid medium references a missing grid");
+    /* Triangles of each light, in scene order, for the light-first next-event test. */
+    s->light_tris.clear();
+    bool light_first = !s->lights.empty();
+    for (DLight &L : s->lights) {
+        uint32_t g = L.geom;
+        uint32_t t0 = s->geoms[g].tbase;
+        uint32_t t1 = g + 1 < s->geoms.size() ? s->geoms[g + 1].tbase : (uint32_t)s->tri_geom.size();
+        L.tri_offset = (uint32_t)(s->light_tris.size() / 12);
+        L.tri_count = t1 - t0;
+        light_first = light_first && L.tri_count <= kLightFirstMaxTris;
+        const float *P = s->pos.data();
+        for (uint32_t t = t0; t < t1; ++t) {
+            const uint32_t *ix = &s->vidx[3 * (size_t)t];
+            V3 v0 = v3(P[3 * ix[0]], P[3 * ix[0] + 1], P[3 * ix[0] + 2]);
+            V3 v1 = v3(P[3 * ix[1]], P[3 * ix[1] + 1], P[3 * ix[1] + 2]);
+            V3 v2 = v3(P[3 * ix[2]], P[3 * ix[2] + 1], P[3 * ix[2] + 2]);
+            V3 e1 = v0 - v1, e2 = v2 - v0;
+            uint32_t tb = t;
+            float tbf;
+            std::memcpy(&tbf, &tb, 4);
+            float rec[12] = {v0.x, v0.y, v0.z, tbf, e1.x, e1.y, e1.z, 0.0f, e2.x, e2.y, e2.z, 0.0f};
+            s->light_tris.insert(s->light_tris.end(), rec, rec + 12);
+        }
+    }
     uint32_t max_leaf = params && params->max_leaf_size ? params->max_leaf_size : 4;
     LT_CHECK(max_leaf >= 1 && max_leaf <= 8, "lt_scene_commit: max_leaf_size must be 1..8");
     hipStream_t st = s->dev->stream;
@@ -464,7 +488,8 @@ int lt_scene_commit(lt_scene *s, const lt_build_params *params) {
         (rc = s->d_texels.upload(s->texels.data(), s->texels.size() * 4, st)) ||
         (rc = s->d_grids.upload(s->grids.data(), s->grids.size() * sizeof(DGrid), st)) ||
         (rc = s->d_voxels.upload(s->voxels.data(), s->voxels.size() * 4, st)) ||
-        (rc = s->d_lights.upload(s->lights.data(), s->lights.size() * sizeof(DLight), st)))
+        (rc = s->d_lights.upload(s->lights.data(), s->lights.size() * sizeof(DLight), st)) ||
+        (rc = s->d_light_tris.upload(s->light_tris.data(), s->light_tris.size() * 4, st)))
         return rc;
     auto t0 = std::chrono::steady_clock::now();
     uint32_t ntris = (uint32_t)s->tri_geom.size();
@@ -488,6 +513,8 @@ int lt_scene_commit(lt_scene *s, const lt_build_params *params) {
     d.texels = s->d_texels.as<float>();
     d.grids = s->d_grids.as<DGrid>();
     d.voxels = s->d_voxels.as<float>();
+    d.light_tris = s->d_light_tris.as<float4>();
+    d.light_first = light_first ? 1u : 0u;
     d.ntris = ntris;
     d.nlights = (uint32_t)s->lights.size();
     for (int k = 0; k < 3; ++k) d.bg[k] = s->bg[k];

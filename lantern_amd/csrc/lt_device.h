@@ -121,6 +121,33 @@ LT_D float exp_f(float x) {
     return ldexpf(y, n);
 }
 
+/* Embree's Moeller-Trumbore with Ng = cross(e2, e1)
+ * (triangle_intersector_moeller.h:75-113, :115-124) and the finalize of
+ * MoellerTrumboreHitM (:43-48) as a correctly rounded division. */
+LT_D bool moeller(const V3 &O, const V3 &D, float tnear, float tfar, const V3 &v0, const V3 &e1, const V3 &e2, float &t,
+                  float &u, float &v) {
+    V3 Ng = cross(e2, e1);
+    V3 C = v0 - O;
+    V3 R = cross(C, D);
+    float den = dot(Ng, D);
+    float absDen = fabsf(den);
+    bool neg = signbit(den);
+    float U = dot(R, e2);
+    float V = dot(R, e1);
+    if (neg) {
+        U = -U;
+        V = -V;
+    }
+    if (!(den != 0.0f && U >= 0.0f && V >= 0.0f && U + V <= absDen)) return false;
+    float T = dot(Ng, C);
+    if (neg) T = -T;
+    if (!((absDen * tnear < T) && (T <= absDen * tfar))) return false;
+    t = T / absDen;
+    u = U / absDen;
+    v = V / absDen;
+    return true;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Random numbers                                                            */
 /* ------------------------------------------------------------------------ */
@@ -207,7 +234,9 @@ struct DLight {
     float area;
     float sphere[4];
     uint32_t geom;
-    uint32_t pad[3];
+    uint32_t tri_offset;   /* first triangle in DScene::light_tris */
+    uint32_t tri_count;
+    uint32_t pad;
 };
 struct DMedium {
     int32_t type;
@@ -246,11 +275,16 @@ struct DScene {
     const float *texels;
     const DGrid *grids;
     const float *voxels;
+    const float4 *light_tris;  /* per light, its triangles in scene order: 3 x float4 like tris */
     uint32_t ntris;
     uint32_t nlights;
     float bg[3];
-    uint32_t root_is_empty;
+    uint32_t light_first;      /* 1: every light has <= kLightFirstMaxTris triangles */
 };
+
+/* Lights with at most this many triangles are intersected directly in the
+ * shade kernel, which turns their next-event rays into occlusion queries. */
+constexpr uint32_t kLightFirstMaxTris = 32;
 
 /* Rays and hits as the public ABI lays them out (32 B + 16 B). */
 struct Ray {

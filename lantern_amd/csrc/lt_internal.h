@@ -113,8 +113,9 @@ struct lt_scene {
     float bg[3] = {0.0f, 0.0f, 0.0f};
     bool committed = false;
     /* device copies */
+    std::vector<float> light_tris;    /* 12 floats per light triangle (v0|prim, e1, e2) */
     lt::DevBuf d_pos, d_nrm, d_uv, d_vidx, d_tri_geom, d_geoms, d_mats, d_media, d_texs, d_texels, d_grids,
-        d_voxels, d_lights, d_nodes, d_tris;
+        d_voxels, d_lights, d_nodes, d_tris, d_light_tris;
     lt::BvhStats bvh;
     float build_ms = 0.0f;
     lt::DScene ds;

@@ -157,9 +157,33 @@ __device__ __forceinline__ void store_ray(Ray *dst, V3 o, V3 d, float tnear, flo
     rp[1] = make_float4(d.x, d.y, d.z, tfar);
 }
 
+/* Closest hit of (o, d, tnear 0.001) among one light's own triangles. For a
+ * small light this replaces the first half of the reference's shadow query
+ * (area_light.cpp:58, 100): the scene's closest hit is the light exactly when
+ * the light is hit at t_L and nothing else is hit before t_L, so the scene
+ * query shrinks to an occlusion test over (0.001, t_L). */
+__device__ __forceinline__ bool light_hit(const DScene &sc, const DLight &L, const V3 &o, const V3 &d, Hit &h) {
+    float tfar = __builtin_huge_valf();
+    bool found = false;
+    for (uint32_t k = 0; k < L.tri_count; ++k) {
+        const float4 *tp = sc.light_tris + 3 * (size_t)(L.tri_offset + k);
+        float4 ta = tp[0], tb = tp[1], tc = tp[2];
+        float t, u, v;
+        if (moeller(o, d, 0.001f, tfar, v3(ta.x, ta.y, ta.z), v3(tb.x, tb.y, tb.z), v3(tc.x, tc.y, tc.z), t, u, v)) {
+            tfar = t;
+            h.t = t;
+            h.u = u;
+            h.v = v;
+            h.prim = __float_as_uint(ta.w);
+            found = true;
+        }
+    }
+    return found;
+}
+
 /* One bounce of RenderPixel for every live path (integrator.cpp:119-242). */
 __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next,
-                                               NeeRec nee, Ray *nee_rays, Ctl *ctl) {
+                                               NeeRec nee, Ray *nee_rays, Hit *nee_hits, Ctl *ctl) {
     const uint32_t count = ctl->count[b];
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->nee_count[b] = 2 * count;
     const int lane = threadIdx.x & 63;
@@ -171,6 +195,7 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
         const uint32_t q = base + lane;
         bool alive = false;
         bool nee1 = false, nee2 = false;
+        float tfar1 = 0.0f, tfar2 = 0.0f;
         uint32_t p = 0;
         V3 new_o = v3s(0.0f), new_d = v3s(0.0f);
         if (q < count) {
@@ -284,6 +309,15 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
                                 p1 = bsdf_pdf(m, ic);
                                 nee1 = true;
                                 wi1 = ldir;
+                                tfar1 = inf;
+                                if (sc.light_first) {
+                                    Hit lh;
+                                    nee1 = light_hit(sc, L, ic.P, ldir, lh);
+                                    if (nee1) {
+                                        nee_hits[2 * (size_t)q] = lh;
+                                        tfar1 = __uint_as_float(__float_as_uint(lh.t) - 1u);
+                                    }
+                                }
                             }
                         }
                         bsdf_sample(m, ic, rng);
@@ -292,6 +326,15 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
                         if (p2 != 0.0f && !is_black(f2) && dot(ic.wi, ic.N) > 0.0f) {  /* AreaLight::PdfLi horizon test */
                             nee2 = true;
                             wi2 = ic.wi;
+                            tfar2 = inf;
+                            if (sc.light_first) {
+                                Hit lh;
+                                nee2 = light_hit(sc, L, ic.P, wi2, lh);
+                                if (nee2) {
+                                    nee_hits[2 * (size_t)q + 1] = lh;
+                                    tfar2 = __uint_as_float(__float_as_uint(lh.t) - 1u);
+                                }
+                            }
                         }
                         if (nee1 || nee2) {
                             nee.tx[q] = T.x; nee.ty[q] = T.y; nee.tz[q] = T.z;
@@ -323,8 +366,8 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
                 }
             }
             nee.info[q] = nee_info;
-            store_ray(nee_rays + 2 * (size_t)q, P, wi1, 0.001f, nee1 ? inf : -1.0f);
-            store_ray(nee_rays + 2 * (size_t)q + 1, P, wi2, 0.001f, nee2 ? inf : -1.0f);
+            store_ray(nee_rays + 2 * (size_t)q, P, wi1, 0.001f, nee1 ? tfar1 : -1.0f);
+            store_ray(nee_rays + 2 * (size_t)q + 1, P, wi2, 0.001f, nee2 ? tfar2 : -1.0f);
             alive = !dead && (b + 1 < a.max_depth);
             if (!alive) a.ps.bounces[p] = dead ? b : b + 1;
             a.ps.tx[p] = T.x; a.ps.ty[p] = T.y; a.ps.tz[p] = T.z;
@@ -357,7 +400,7 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
 /* The casts of AreaLight::SampleLi / PdfLi resolved (area_light.cpp:58-71,
  * 100-109) and the MIS sums of EstimateDirect (integrator.cpp:287-314). */
 __global__ __launch_bounds__(256) void k_resolve(WaveArgs a, uint32_t b, Queue cur, NeeRec nee, const Ray *nee_rays,
-                                                 const Hit *nee_hits, Ctl *ctl) {
+                                                 const Hit *nee_hits, const uint32_t *nee_occl, Ctl *ctl) {
     const uint32_t count = ctl->count[b];
     const DScene &sc = a.sc;
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
@@ -369,7 +412,9 @@ __global__ __launch_bounds__(256) void k_resolve(WaveArgs a, uint32_t b, Queue c
         V3 direct = v3s(0.0f);
         if (info & 0x40000000u) {
             const Hit h = nee_hits[2 * (size_t)q];
-            if (h.prim != 0xFFFFFFFFu && sc.tri_geom[h.prim] == L.geom) {
+            bool reached = sc.light_first ? nee_occl[2 * (size_t)q] == 0u
+                                          : (h.prim != 0xFFFFFFFFu && sc.tri_geom[h.prim] == L.geom);
+            if (reached) {
                 const Ray r = nee_rays[2 * (size_t)q];
                 V3 o = v3(r.ox, r.oy, r.oz), d = v3(r.dx, r.dy, r.dz);
                 V3 ip = o + normalize(d) * h.t;
@@ -386,7 +431,9 @@ __global__ __launch_bounds__(256) void k_resolve(WaveArgs a, uint32_t b, Queue c
         }
         if (info & 0x80000000u) {
             const Hit h = nee_hits[2 * (size_t)q + 1];
-            if (h.prim != 0xFFFFFFFFu && sc.tri_geom[h.prim] == L.geom) {
+            bool reached = sc.light_first ? nee_occl[2 * (size_t)q + 1] == 0u
+                                          : (h.prim != 0xFFFFFFFFu && sc.tri_geom[h.prim] == L.geom);
+            if (reached) {
                 const Ray r = nee_rays[2 * (size_t)q + 1];
                 V3 o = v3(r.ox, r.oy, r.oz), d = v3(r.dx, r.dy, r.dz);
                 V3 ip = o + normalize(d) * h.t;
@@ -444,7 +491,7 @@ __global__ void k_reset_ctl(Ctl *ctl) {
 /* Bytes of workspace per path in flight. */
 constexpr size_t kPathBytes = 4 * 11;                 /* path state */
 constexpr size_t kQueueBytes = sizeof(Ray) + 4;       /* per queue slot */
-constexpr size_t kNeeBytes = 4 * 14 + 2 * sizeof(Ray) + 2 * sizeof(Hit);
+constexpr size_t kNeeBytes = 4 * 14 + 2 * sizeof(Ray) + 2 * sizeof(Hit) + 2 * 4;
 constexpr size_t kBytesPerPath = kPathBytes + 2 * kQueueBytes + sizeof(Hit) + kNeeBytes;
 
 struct Workspace {
@@ -454,6 +501,7 @@ struct Workspace {
     NeeRec nee;
     Ray *nee_rays;
     Hit *nee_hits;
+    uint32_t *nee_occl;
 };
 
 int carve(lt_device *dev, size_t n, Workspace &w) {
@@ -486,6 +534,7 @@ int carve(lt_device *dev, size_t n, Workspace &w) {
     w.nee.nl = (float *)take(4);
     w.nee_rays = (Ray *)take(2 * sizeof(Ray));
     w.nee_hits = (Hit *)take(2 * sizeof(Hit));
+    w.nee_occl = (uint32_t *)take(2 * 4);
     return LT_OK;
 }
 
@@ -640,19 +689,25 @@ int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &
             ++ext_launches;
             tm.begin(T_SHADE);
             hipLaunchKernelGGL(k_shade, dim3(sgrid), dim3(256), 0, st, a, b, cur, (const Hit *)w.hits, nxt, w.nee,
-                               w.nee_rays, ctl);
+                               w.nee_rays, w.nee_hits, ctl);
             tm.end(T_SHADE);
             LT_HIP(hipGetLastError());
             /* next-event rays: two per shaded slot; inactive ones have tfar < 0 */
             tm.begin(T_SHADOW);
-            if ((rc = launch_trace_closest(st, a.sc, w.nee_rays, w.nee_hits, &ctl->nee_count[b], 0,
-                                           &ctl->work[wslot++], tblock, tgrid, spill)))
-                return rc;
+            /* next-event rays: occlusion up to the light hit (small lights) or
+             * the reference's closest-hit query */
+            if (a.sc.light_first)
+                rc = launch_trace_any(st, a.sc, w.nee_rays, w.nee_occl, &ctl->nee_count[b], 0, &ctl->work[wslot++],
+                                      tblock, tgrid, spill);
+            else
+                rc = launch_trace_closest(st, a.sc, w.nee_rays, w.nee_hits, &ctl->nee_count[b], 0,
+                                          &ctl->work[wslot++], tblock, tgrid, spill);
+            if (rc) return rc;
             tm.end(T_SHADOW);
             ++sh_launches;
             tm.begin(T_RESOLVE);
             hipLaunchKernelGGL(k_resolve, dim3(sgrid), dim3(256), 0, st, a, b, cur, w.nee, (const Ray *)w.nee_rays,
-                               (const Hit *)w.nee_hits, ctl);
+                               (const Hit *)w.nee_hits, (const uint32_t *)w.nee_occl, ctl);
             tm.end(T_RESOLVE);
             LT_HIP(hipGetLastError());
             std::swap(cur, nxt);

@@ -61,33 +61,6 @@ LT_D bool slab(float lox, float hix, float loy, float hiy, float loz, float hiz,
     return tn <= tf;
 }
 
-/* Embree's Moeller-Trumbore with Ng = cross(e2, e1)
- * (triangle_intersector_moeller.h:75-113, :115-124) and the finalize of
- * MoellerTrumboreHitM (:43-48) as a correctly rounded division. */
-LT_D bool moeller(const V3 &O, const V3 &D, float tnear, float tfar, const V3 &v0, const V3 &e1, const V3 &e2, float &t,
-                  float &u, float &v) {
-    V3 Ng = cross(e2, e1);
-    V3 C = v0 - O;
-    V3 R = cross(C, D);
-    float den = dot(Ng, D);
-    float absDen = fabsf(den);
-    bool neg = signbit(den);
-    float U = dot(R, e2);
-    float V = dot(R, e1);
-    if (neg) {
-        U = -U;
-        V = -V;
-    }
-    if (!(den != 0.0f && U >= 0.0f && V >= 0.0f && U + V <= absDen)) return false;
-    float T = dot(Ng, C);
-    if (neg) T = -T;
-    if (!((absDen * tnear < T) && (T <= absDen * tfar))) return false;
-    t = T / absDen;
-    u = U / absDen;
-    v = V / absDen;
-    return true;
-}
-
 template <bool ANY, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, const Ray *__restrict__ rays, Hit *__restrict__ hits,
                                                  uint32_t *__restrict__ occluded, const uint32_t *count_ptr,
