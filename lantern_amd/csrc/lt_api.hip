@@ -493,7 +493,8 @@ int lt_scene_commit(lt_scene *s, const lt_build_params *params) {
         return rc;
     auto t0 = std::chrono::steady_clock::now();
     uint32_t ntris = (uint32_t)s->tri_geom.size();
-    if ((rc = build_bvh(st, s->d_pos.as<float>(), s->d_vidx.as<uint32_t>(), ntris, max_leaf, s->d_nodes, s->d_tris, s->bvh)))
+    if ((rc = build_bvh(st, s->d_pos.as<float>(), s->d_vidx.as<uint32_t>(), s->d_tri_geom.as<uint32_t>(), ntris, max_leaf,
+                        s->d_nodes, s->d_tris, s->bvh)))
         return rc;
     LT_HIP(hipStreamSynchronize(st));
     s->build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -577,8 +578,8 @@ static int intersect_device(lt_scene *s, const lt_ray *rays, void *out, uint32_t
     if (rc) return rc;
     LT_HIP(hipMemsetAsync(d->trace_work.p, 0, sizeof(uint32_t), st));
     if (any)
-        return launch_trace_any(st, s->ds, (const Ray *)rays, (uint32_t *)out, nullptr, n, d->trace_work.as<uint32_t>(),
-                                block, grid, (int *)d->spill.p);
+        return launch_trace_any(st, s->ds, (const Ray *)rays, (uint32_t *)out, nullptr, nullptr, n,
+                                d->trace_work.as<uint32_t>(), block, grid, (int *)d->spill.p);
     return launch_trace_closest(st, s->ds, (const Ray *)rays, (Hit *)out, nullptr, n, d->trace_work.as<uint32_t>(),
                                 block, grid, (int *)d->spill.p);
 }

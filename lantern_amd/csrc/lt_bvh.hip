@@ -23,7 +23,7 @@
  *   n1 = (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y)
  *   n2 = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)
  *   n3 = (child0, child1, -, -) as int: >= 0 internal node, < 0 leaf ~first_tri
- * Triangles (48 B, BVH order): (v0.xyz, prim), (e1.xyz, last_in_leaf), (e2.xyz, 0)
+ * Triangles (48 B, BVH order): (v0.xyz, prim), (e1.xyz, last_in_leaf), (e2.xyz, geomID)
  * with e1 = v0 - v1 and e2 = v2 - v0 precomputed as Embree's TriangleM does
  * (kernels/geometry/triangle.h:53-54).
  */
@@ -231,7 +231,8 @@ __global__ void k_reach(BuildArrays a, int *emit, unsigned int *stats) {
     }
 }
 
-__global__ void k_write_tris(const float *pos, const uint32_t *vidx, const uint64_t *keys, uint32_t n, float4 *tris) {
+__global__ void k_write_tris(const float *pos, const uint32_t *vidx, const uint32_t *tri_geom, const uint64_t *keys,
+                             uint32_t n, float4 *tris) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     uint32_t t = (uint32_t)(keys[k] & 0xFFFFFFFFull);
@@ -242,7 +243,7 @@ __global__ void k_write_tris(const float *pos, const uint32_t *vidx, const uint6
     V3 e1 = v0 - v1, e2 = v2 - v0;
     tris[3 * k + 0] = make_float4(v0.x, v0.y, v0.z, __uint_as_float(t));
     tris[3 * k + 1] = make_float4(e1.x, e1.y, e1.z, __uint_as_float(0u));
-    tris[3 * k + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+    tris[3 * k + 2] = make_float4(e2.x, e2.y, e2.z, __uint_as_float(tri_geom[t]));
 }
 
 LT_D void emit_child(const BuildArrays &a, int c, float lo[3], float hi[3], int &ptr, int &last, bool &leaf) {
@@ -307,8 +308,8 @@ inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)((n + bl
 
 } // namespace
 
-int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, uint32_t n, uint32_t max_leaf,
-              DevBuf &nodes, DevBuf &tris, BvhStats &stats) {
+int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const uint32_t *d_tri_geom, uint32_t n,
+              uint32_t max_leaf, DevBuf &nodes, DevBuf &tris, BvhStats &stats) {
     stats = BvhStats();
     if (n == 0) return LT_OK;
     const unsigned B = 256;
@@ -342,7 +343,8 @@ int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, uint32
     uint32_t nnodes = n > 1 ? n - 1 : 1;
     if ((rc = nodes.alloc(sizeof(float4) * 4 * (size_t)nnodes))) return rc;
     LT_HIP(hipMemsetAsync(nodes.p, 0, sizeof(float4) * 4 * (size_t)nnodes, st));
-    hipLaunchKernelGGL(k_write_tris, dim3(grid_for(n, B)), dim3(B), 0, st, d_pos, d_vidx, sk, n, (float4 *)tris.p);
+    hipLaunchKernelGGL(k_write_tris, dim3(grid_for(n, B)), dim3(B), 0, st, d_pos, d_vidx, d_tri_geom, sk, n,
+                       (float4 *)tris.p);
     LT_HIP(hipGetLastError());
 
     if (n == 1) {

@@ -53,15 +53,17 @@ struct BvhStats {
     float sah_cost = 0.0f;
 };
 
-int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, uint32_t n, uint32_t max_leaf,
-              DevBuf &nodes, DevBuf &tris, BvhStats &stats);
+int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const uint32_t *d_tri_geom, uint32_t n,
+              uint32_t max_leaf, DevBuf &nodes, DevBuf &tris, BvhStats &stats);
 
 /* Traversal launches (lt_trace.hip). count_ptr (device) overrides count when
  * non-null; work is a device counter that must be zero at launch. */
 int launch_trace_closest(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, const uint32_t *count_ptr,
                          uint32_t count, uint32_t *work, int block, int grid, int *spill);
+/* ignore_geom (nullable): per ray, a geomID whose triangles do not occlude. */
 int launch_trace_any(hipStream_t st, const DScene &sc, const Ray *rays, uint32_t *occluded,
-                     const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block, int grid, int *spill);
+                     const uint32_t *ignore_geom, const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block,
+                     int grid, int *spill);
 int trace_grid(int num_cus, int block);
 size_t trace_spill_bytes(int grid, int block);
 int trace_default_block();

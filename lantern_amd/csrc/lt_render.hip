@@ -160,8 +160,10 @@ __device__ __forceinline__ void store_ray(Ray *dst, V3 o, V3 d, float tnear, flo
 /* Closest hit of (o, d, tnear 0.001) among one light's own triangles. For a
  * small light this replaces the first half of the reference's shadow query
  * (area_light.cpp:58, 100): the scene's closest hit is the light exactly when
- * the light is hit at t_L and nothing else is hit before t_L, so the scene
- * query shrinks to an occlusion test over (0.001, t_L). */
+ * the light is hit at t_L and no other geometry is hit in (0.001, t_L], so
+ * the scene query shrinks to an occlusion test that skips the light's own
+ * triangles (which cannot be closer than t_L; testing them again could only
+ * report the light itself through a rounding tie). */
 __device__ __forceinline__ bool light_hit(const DScene &sc, const DLight &L, const V3 &o, const V3 &d, Hit &h) {
     float tfar = __builtin_huge_valf();
     bool found = false;
@@ -183,7 +185,8 @@ __device__ __forceinline__ bool light_hit(const DScene &sc, const DLight &L, con
 
 /* One bounce of RenderPixel for every live path (integrator.cpp:119-242). */
 __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next,
-                                               NeeRec nee, Ray *nee_rays, Hit *nee_hits, Ctl *ctl) {
+                                               NeeRec nee, Ray *nee_rays, Hit *nee_hits, uint32_t *nee_ignore,
+                                               Ctl *ctl) {
     const uint32_t count = ctl->count[b];
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->nee_count[b] = 2 * count;
     const int lane = threadIdx.x & 63;
@@ -315,7 +318,7 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
                                     nee1 = light_hit(sc, L, ic.P, ldir, lh);
                                     if (nee1) {
                                         nee_hits[2 * (size_t)q] = lh;
-                                        tfar1 = __uint_as_float(__float_as_uint(lh.t) - 1u);
+                                        tfar1 = lh.t;
                                     }
                                 }
                             }
@@ -332,7 +335,7 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
                                 nee2 = light_hit(sc, L, ic.P, wi2, lh);
                                 if (nee2) {
                                     nee_hits[2 * (size_t)q + 1] = lh;
-                                    tfar2 = __uint_as_float(__float_as_uint(lh.t) - 1u);
+                                    tfar2 = lh.t;
                                 }
                             }
                         }
@@ -342,6 +345,8 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
                             nee.f2x[q] = f2.x; nee.f2y[q] = f2.y; nee.f2z[q] = f2.z; nee.p2[q] = p2;
                             nee.nl[q] = (float)nl;
                             nee_info = li | (nee1 ? 0x40000000u : 0u) | (nee2 ? 0x80000000u : 0u);
+                            nee_ignore[2 * (size_t)q] = L.geom;
+                            nee_ignore[2 * (size_t)q + 1] = L.geom;
                         }
                     }
                     /* continuation (integrator.cpp:199-231) */
@@ -491,7 +496,7 @@ __global__ void k_reset_ctl(Ctl *ctl) {
 /* Bytes of workspace per path in flight. */
 constexpr size_t kPathBytes = 4 * 11;                 /* path state */
 constexpr size_t kQueueBytes = sizeof(Ray) + 4;       /* per queue slot */
-constexpr size_t kNeeBytes = 4 * 14 + 2 * sizeof(Ray) + 2 * sizeof(Hit) + 2 * 4;
+constexpr size_t kNeeBytes = 4 * 14 + 2 * sizeof(Ray) + 2 * sizeof(Hit) + 2 * 4 + 2 * 4;
 constexpr size_t kBytesPerPath = kPathBytes + 2 * kQueueBytes + sizeof(Hit) + kNeeBytes;
 
 struct Workspace {
@@ -502,6 +507,7 @@ struct Workspace {
     Ray *nee_rays;
     Hit *nee_hits;
     uint32_t *nee_occl;
+    uint32_t *nee_ignore;    /* per next-event ray: the light's geomID */
 };
 
 int carve(lt_device *dev, size_t n, Workspace &w) {
@@ -535,6 +541,7 @@ int carve(lt_device *dev, size_t n, Workspace &w) {
     w.nee_rays = (Ray *)take(2 * sizeof(Ray));
     w.nee_hits = (Hit *)take(2 * sizeof(Hit));
     w.nee_occl = (uint32_t *)take(2 * 4);
+    w.nee_ignore = (uint32_t *)take(2 * 4);
     return LT_OK;
 }
 
@@ -689,7 +696,7 @@ int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &
             ++ext_launches;
             tm.begin(T_SHADE);
             hipLaunchKernelGGL(k_shade, dim3(sgrid), dim3(256), 0, st, a, b, cur, (const Hit *)w.hits, nxt, w.nee,
-                               w.nee_rays, w.nee_hits, ctl);
+                               w.nee_rays, w.nee_hits, w.nee_ignore, ctl);
             tm.end(T_SHADE);
             LT_HIP(hipGetLastError());
             /* next-event rays: two per shaded slot; inactive ones have tfar < 0 */
@@ -697,8 +704,8 @@ int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &
             /* next-event rays: occlusion up to the light hit (small lights) or
              * the reference's closest-hit query */
             if (a.sc.light_first)
-                rc = launch_trace_any(st, a.sc, w.nee_rays, w.nee_occl, &ctl->nee_count[b], 0, &ctl->work[wslot++],
-                                      tblock, tgrid, spill);
+                rc = launch_trace_any(st, a.sc, w.nee_rays, w.nee_occl, w.nee_ignore, &ctl->nee_count[b], 0,
+                                      &ctl->work[wslot++], tblock, tgrid, spill);
             else
                 rc = launch_trace_closest(st, a.sc, w.nee_rays, w.nee_hits, &ctl->nee_count[b], 0,
                                           &ctl->work[wslot++], tblock, tgrid, spill);

@@ -63,8 +63,9 @@ LT_D bool slab(float lox, float hix, float loy, float hiy, float loz, float hiz,
 
 template <bool ANY, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, const Ray *__restrict__ rays, Hit *__restrict__ hits,
-                                                 uint32_t *__restrict__ occluded, const uint32_t *count_ptr,
-                                                 uint32_t count_imm, uint32_t *work, int *spill) {
+                                                 uint32_t *__restrict__ occluded, const uint32_t *ignore_geom,
+                                                 const uint32_t *count_ptr, uint32_t count_imm, uint32_t *work,
+                                                 int *spill) {
     __shared__ int lds[LDS_STACK * BLOCK];
     const uint32_t n = count_ptr ? *count_ptr : count_imm;
     const int tid = threadIdx.x;
@@ -92,6 +93,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, const Ray *__restric
         r.idir = v3(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
         float best_t = r.tfar, best_u = 0.0f, best_v = 0.0f;
         uint32_t best_prim = 0xFFFFFFFFu;
+        const uint32_t ignore = (ANY && ignore_geom) ? ignore_geom[i] : 0xFFFFFFFFu;
         bool done = !(r.tfar >= 0.0f) || sc.ntris == 0;
 
         int sp = 0;
@@ -140,7 +142,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, const Ray *__restric
                     const float4 *tp = tris + 3 * k;
                     float4 a = tp[0], b = tp[1], c = tp[2];
                     float t, u, v;
-                    if (moeller(r.o, r.d, r.tnear, r.tfar, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t,
+                    if ((!ANY || __float_as_uint(c.w) != ignore) &&
+                        moeller(r.o, r.d, r.tnear, r.tfar, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t,
                                 u, v)) {
                         r.tfar = t;
                         best_t = t;
@@ -171,12 +174,12 @@ __global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, const Ray *__restric
 }
 
 template <bool ANY>
-int launch(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, uint32_t *occl, const uint32_t *count_ptr,
-           uint32_t count, uint32_t *work, int block, int grid, int *spill) {
+int launch(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, uint32_t *occl, const uint32_t *ign,
+           const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block, int grid, int *spill) {
     switch (block) {
-    case 64: hipLaunchKernelGGL((k_trace<ANY, 64>), dim3(grid), dim3(64), 0, st, sc, rays, hits, occl, count_ptr, count, work, spill); break;
-    case 128: hipLaunchKernelGGL((k_trace<ANY, 128>), dim3(grid), dim3(128), 0, st, sc, rays, hits, occl, count_ptr, count, work, spill); break;
-    case 256: hipLaunchKernelGGL((k_trace<ANY, 256>), dim3(grid), dim3(256), 0, st, sc, rays, hits, occl, count_ptr, count, work, spill); break;
+    case 64: hipLaunchKernelGGL((k_trace<ANY, 64>), dim3(grid), dim3(64), 0, st, sc, rays, hits, occl, ign, count_ptr, count, work, spill); break;
+    case 128: hipLaunchKernelGGL((k_trace<ANY, 128>), dim3(grid), dim3(128), 0, st, sc, rays, hits, occl, ign, count_ptr, count, work, spill); break;
+    case 256: hipLaunchKernelGGL((k_trace<ANY, 256>), dim3(grid), dim3(256), 0, st, sc, rays, hits, occl, ign, count_ptr, count, work, spill); break;
     default: return set_error(LT_E_INVALID, "traversal block size must be 64, 128 or 256");
     }
     LT_HIP(hipGetLastError());
@@ -200,11 +203,12 @@ size_t trace_spill_bytes(int grid, int block) { return (size_t)grid * block * (S
 
 int launch_trace_closest(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, const uint32_t *count_ptr,
                          uint32_t count, uint32_t *work, int block, int grid, int *spill) {
-    return launch<false>(st, sc, rays, hits, nullptr, count_ptr, count, work, block, grid, spill);
+    return launch<false>(st, sc, rays, hits, nullptr, nullptr, count_ptr, count, work, block, grid, spill);
 }
 int launch_trace_any(hipStream_t st, const DScene &sc, const Ray *rays, uint32_t *occluded,
-                     const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block, int grid, int *spill) {
-    return launch<true>(st, sc, rays, nullptr, occluded, count_ptr, count, work, block, grid, spill);
+                     const uint32_t *ignore_geom, const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block,
+                     int grid, int *spill) {
+    return launch<true>(st, sc, rays, nullptr, occluded, ignore_geom, count_ptr, count, work, block, grid, spill);
 }
 
 } // namespace lt
