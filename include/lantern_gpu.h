@@ -88,8 +88,9 @@ typedef struct lt_medium {
 } lt_medium;
 
 typedef struct lt_build_params {
-    uint32_t max_leaf_size;   /* 1..8, default 4 (Embree sahBlockSize/minLeaf 4) */
-    uint32_t reserved[7];
+    uint32_t max_leaf_size;   /* 1..4, default 4 (Embree's sahBlockSize/minLeaf is 4) */
+    uint32_t layout;          /* 0: 8-wide compressed nodes (default), 1: binary nodes */
+    uint32_t reserved[6];
 } lt_build_params;
 
 typedef struct lt_scene_info {
@@ -102,6 +103,10 @@ typedef struct lt_scene_info {
     float sah_cost;
     float build_ms;
     uint64_t device_bytes;
+    uint32_t wide_nodes;      /* 8-wide compressed nodes */
+    uint32_t wide_depth;
+    uint32_t layout;          /* layout the traversal uses */
+    uint32_t reserved;
 } lt_scene_info;
 
 /* rtcNewScene + rtcSetSceneFlags/BuildQuality (scene.cpp:196-198). */

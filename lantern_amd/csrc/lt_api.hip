@@ -195,6 +195,18 @@ static void bounding_sphere(const float *p, size_t n, float out[4]) {
 }
 
 } // namespace
+
+/* Reads and clears the count of rays that hit a traversal bound. */
+int check_traversal_bounds(lt_device *d) {
+    uint32_t bad = 0;
+    LT_HIP(hipMemcpy(&bad, d->diag.p, sizeof bad, hipMemcpyDeviceToHost));
+    if (bad) {
+        LT_HIP(hipMemset(d->diag.p, 0, sizeof bad));
+        return set_error(LT_E_STATE, std::to_string(bad) + " rays exceeded the traversal stack or step bound");
+    }
+    return LT_OK;
+}
+
 } // namespace lt
 
 using namespace lt;
@@ -228,6 +240,8 @@ int lt_device_create(int ordinal, lt_device **out) {
         return hip_error(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
     }
     int rc = d->trace_work.alloc(64);
+    if (!rc) rc = d->diag.alloc(64);
+    if (!rc && hipMemset(d->diag.p, 0, 64) != hipSuccess) rc = set_error(LT_E_HIP, "hipMemset of the diagnostics word failed");
     if (rc) {
         (void)hipStreamDestroy(d->stream);
         delete d;
@@ -245,6 +259,7 @@ int lt_device_destroy(lt_device *d) {
     d->ctl.release();
     d->trace_work.release();
     d->spill.release();
+    d->diag.release();
     (void)hipStreamDestroy(d->stream);
     delete d;
     return LT_OK;
@@ -259,7 +274,7 @@ int lt_device_stream(lt_device *d, void **out) {
 int lt_device_synchronize(lt_device *d) {
     LT_CHECK(d, "lt_device_synchronize: NULL device");
     LT_HIP(hipStreamSynchronize(d->stream));
-    return LT_OK;
+    return lt::check_traversal_bounds(d);
 }
 
 /* ---- scene ----------------------------------------------------------------- */
@@ -474,7 +489,10 @@ int lt_scene_commit(lt_scene *s, const lt_build_params *params) {
         }
     }
     uint32_t max_leaf = params && params->max_leaf_size ? params->max_leaf_size : 4;
-    LT_CHECK(max_leaf >= 1 && max_leaf <= 8, "lt_scene_commit: max_leaf_size must be 1..8");
+    LT_CHECK(max_leaf >= 1 && max_leaf <= 4, "lt_scene_commit: max_leaf_size must be 1..4");
+    uint32_t layout = params ? params->layout : 0;
+    LT_CHECK(layout <= 1, "lt_scene_commit: layout must be 0 (wide) or 1 (binary)");
+    s->layout = layout;
     hipStream_t st = s->dev->stream;
     LT_HIP(hipSetDevice(s->dev->ordinal));
     int rc;
@@ -494,7 +512,7 @@ int lt_scene_commit(lt_scene *s, const lt_build_params *params) {
     auto t0 = std::chrono::steady_clock::now();
     uint32_t ntris = (uint32_t)s->tri_geom.size();
     if ((rc = build_bvh(st, s->d_pos.as<float>(), s->d_vidx.as<uint32_t>(), s->d_tri_geom.as<uint32_t>(), ntris, max_leaf,
-                        s->d_nodes, s->d_tris, s->bvh)))
+                        s->d_nodes, s->d_tris, s->d_wnodes, s->d_wtris, s->bvh)))
         return rc;
     LT_HIP(hipStreamSynchronize(st));
     s->build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -515,6 +533,9 @@ int lt_scene_commit(lt_scene *s, const lt_build_params *params) {
     d.grids = s->d_grids.as<DGrid>();
     d.voxels = s->d_voxels.as<float>();
     d.light_tris = s->d_light_tris.as<float4>();
+    d.wnodes = s->d_wnodes.as<float4>();
+    d.wtris = s->d_wtris.as<float4>();
+    d.layout = s->layout;
     d.light_first = light_first ? 1u : 0u;
     d.ntris = ntris;
     d.nlights = (uint32_t)s->lights.size();
@@ -533,9 +554,13 @@ int lt_scene_get_info(lt_scene *s, lt_scene_info *o) {
     o->max_depth = s->bvh.max_depth;
     o->num_lights = (uint32_t)s->lights.size();
     o->sah_cost = s->bvh.sah_cost;
+    o->wide_nodes = s->bvh.wide_nodes;
+    o->wide_depth = s->bvh.wide_depth;
+    o->layout = s->layout;
     o->build_ms = s->build_ms;
     o->device_bytes = s->d_pos.bytes + s->d_nrm.bytes + s->d_uv.bytes + s->d_vidx.bytes + s->d_tri_geom.bytes +
-                      s->d_nodes.bytes + s->d_tris.bytes + s->d_texels.bytes + s->d_voxels.bytes;
+                      s->d_nodes.bytes + s->d_tris.bytes + s->d_texels.bytes + s->d_voxels.bytes + s->d_wnodes.bytes +
+                      s->d_wtris.bytes;
     return LT_OK;
 }
 
@@ -579,9 +604,9 @@ static int intersect_device(lt_scene *s, const lt_ray *rays, void *out, uint32_t
     LT_HIP(hipMemsetAsync(d->trace_work.p, 0, sizeof(uint32_t), st));
     if (any)
         return launch_trace_any(st, s->ds, (const Ray *)rays, (uint32_t *)out, nullptr, nullptr, n,
-                                d->trace_work.as<uint32_t>(), block, grid, (int *)d->spill.p);
+                                d->trace_work.as<uint32_t>(), block, grid, (int *)d->spill.p, d->diag.as<uint32_t>());
     return launch_trace_closest(st, s->ds, (const Ray *)rays, (Hit *)out, nullptr, n, d->trace_work.as<uint32_t>(),
-                                block, grid, (int *)d->spill.p);
+                                block, grid, (int *)d->spill.p, d->diag.as<uint32_t>());
 }
 
 int lt_intersect_closest_device(lt_scene *s, const lt_ray *rays, lt_hit *hits, uint32_t n, void *stream) {
@@ -601,7 +626,7 @@ static int intersect_host(lt_scene *s, const lt_ray *rays, void *out, size_t out
     if ((rc = intersect_device(s, dr.as<lt_ray>(), dh.p, n, st, any))) return rc;
     LT_HIP(hipMemcpyAsync(out, dh.p, out_each * (size_t)n, hipMemcpyDeviceToHost, st));
     LT_HIP(hipStreamSynchronize(st));
-    return LT_OK;
+    return check_traversal_bounds(s->dev);
 }
 int lt_intersect_closest(lt_scene *s, const lt_ray *rays, lt_hit *hits, uint32_t n) {
     return intersect_host(s, rays, hits, sizeof(lt_hit), n, false);

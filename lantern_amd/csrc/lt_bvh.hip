@@ -304,14 +304,214 @@ __global__ void k_single_leaf_root(const float *pos, const uint32_t *vidx, const
     tris[3 * (n - 1) + 1].w = __uint_as_float(1u);
 }
 
+
+/* ---- 8-wide compressed layout ---------------------------------------------
+ * Node (80 B = 5 x float4), after Ylitie, Karras & Laine, "Efficient
+ * Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs", HPG 2017:
+ *   n0 = (p.x, p.y, p.z, e.x | e.y << 8 | e.z << 16 | imask << 24)
+ *   n1 = (child_base, tri_base, meta[0..3], meta[4..7])
+ *   n2 = (qlo.x[0..3], qlo.x[4..7], qlo.y[0..3], qlo.y[4..7])
+ *   n3 = (qlo.z[0..3], qlo.z[4..7], qhi.x[0..3], qhi.x[4..7])
+ *   n4 = (qhi.y[0..3], qhi.y[4..7], qhi.z[0..3], qhi.z[4..7])
+ * Child slot j: box = p + q * 2^(e-127) per axis; imask bit j marks an
+ * internal child whose node index is child_base + meta[j]; otherwise a
+ * non-zero meta[j] is a leaf of (meta >> 5) triangles starting at
+ * tri_base + (meta & 31); meta 0 with the imask bit clear is an empty slot.
+ * p lies on the 2^e grid and |p| / 2^e <= 2^23, so p + q * 2^e is an exact
+ * float: the decoded planes are exactly the quantized (outward-rounded)
+ * boxes and the traversal's robust slab test stays conservative. */
+struct WideArgs {
+    BuildArrays a;
+    int *wmap;            /* wide node -> binary internal node */
+    float4 *wnodes;
+    float4 *wtris;
+    const uint32_t *tri_geom;
+    unsigned *counters;   /* [0] wide nodes allocated, [1] triangles placed */
+    int begin, end;
+};
+
+LT_D bool is_leafy(const BuildArrays &a, int c) { return c < 0 || a.collapse[c]; }
+LT_D int2 leaf_range(const BuildArrays &a, int c) { return c < 0 ? make_int2(~c, ~c) : a.range[c]; }
+
+/* Grid exponent for one axis of a node box [lo, hi]: 2^e >= (hi - lo) / 254
+ * and 2^e >= 2^-23 * 2^ceil(log2(max|lo|,|hi|)). */
+LT_D int grid_exponent(float lo, float hi) {
+    int k1 = -126, k2 = -126, ex;
+    double ext = (double)hi - (double)lo;
+    if (ext > 0.0) {
+        (void)frexp(ext / 254.0, &ex);
+        k1 = ex;
+    }
+    double mx = fmax(fabs((double)lo), fabs((double)hi));
+    if (mx > 0.0) {
+        (void)frexp(mx, &ex);
+        k2 = ex - 23;
+    }
+    int e = max(k1, k2);
+    return min(max(e, -126), 127);
+}
+
+__global__ void k_wide_pass(WideArgs w) {
+    int wi = w.begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (wi >= w.end) return;
+    const BuildArrays &a = w.a;
+    int b = w.wmap[wi];
+    int ids[8];
+    float area[8];
+    bool leafy[8];
+    int n = 2;
+    int2 c0 = a.child[b];
+    ids[0] = c0.x;
+    ids[1] = c0.y;
+    for (int j = 0; j < 2; ++j) {
+        leafy[j] = is_leafy(a, ids[j]);
+        area[j] = ids[j] >= 0 ? a.bhi[ids[j]].w : 0.0f;
+    }
+    /* greedy expansion: open the internal child with the largest surface area */
+    while (n < 8) {
+        int best = -1;
+        float ba = -1.0f;
+        for (int j = 0; j < n; ++j)
+            if (!leafy[j] && area[j] > ba) {
+                ba = area[j];
+                best = j;
+            }
+        if (best < 0) break;
+        int2 cc = a.child[ids[best]];
+        ids[best] = cc.x;
+        ids[n] = cc.y;
+        leafy[best] = is_leafy(a, cc.x);
+        area[best] = cc.x >= 0 ? a.bhi[cc.x].w : 0.0f;
+        leafy[n] = is_leafy(a, cc.y);
+        area[n] = cc.y >= 0 ? a.bhi[cc.y].w : 0.0f;
+        ++n;
+    }
+    int nint = 0, ntri = 0;
+    for (int j = 0; j < n; ++j) {
+        if (leafy[j]) {
+            int2 r = leaf_range(a, ids[j]);
+            ntri += r.y - r.x + 1;
+        } else {
+            ++nint;
+        }
+    }
+    unsigned child_base = nint ? atomicAdd(&w.counters[0], (unsigned)nint) : 0u;
+    unsigned tri_base = ntri ? atomicAdd(&w.counters[1], (unsigned)ntri) : 0u;
+    float4 nlo = a.blo[b], nhi = a.bhi[b];
+    float blo[3] = {nlo.x, nlo.y, nlo.z}, bhi[3] = {nhi.x, nhi.y, nhi.z};
+    int ee[3];
+    double sc[3], pd[3];
+    for (int k = 0; k < 3; ++k) {
+        ee[k] = grid_exponent(blo[k], bhi[k]);
+        sc[k] = ldexp(1.0, ee[k]);
+        pd[k] = floor((double)blo[k] / sc[k]) * sc[k];
+    }
+    unsigned qlo[3][2] = {{0u, 0u}, {0u, 0u}, {0u, 0u}}, qhi[3][2] = {{0u, 0u}, {0u, 0u}, {0u, 0u}};
+    unsigned meta[2] = {0u, 0u};
+    unsigned imask = 0u;
+    int rank = 0, off = 0;
+    for (int j = 0; j < n; ++j) {
+        float lo[3], hi[3], cost, ar;
+        child_info(a, ids[j], lo, hi, cost, ar);
+        for (int k = 0; k < 3; ++k) {
+            double ql = floor(((double)lo[k] - pd[k]) / sc[k]);
+            double qh = ceil(((double)hi[k] - pd[k]) / sc[k]);
+            unsigned uql = (unsigned)fmin(fmax(ql, 0.0), 255.0);
+            unsigned uqh = (unsigned)fmin(fmax(qh, 0.0), 255.0);
+            qlo[k][j >> 2] |= uql << (8 * (j & 3));
+            qhi[k][j >> 2] |= uqh << (8 * (j & 3));
+        }
+        unsigned m;
+        if (leafy[j]) {
+            int2 r = leaf_range(a, ids[j]);
+            int cnt = r.y - r.x + 1;
+            m = ((unsigned)cnt << 5) | (unsigned)off;
+            for (int t = 0; t < cnt; ++t) {
+                uint32_t tid = (uint32_t)(a.keys[r.x + t] & 0xFFFFFFFFull);
+                uint32_t ia = a.vidx[3 * tid], ib = a.vidx[3 * tid + 1], ic = a.vidx[3 * tid + 2];
+                V3 v0 = v3(a.pos[3 * ia], a.pos[3 * ia + 1], a.pos[3 * ia + 2]);
+                V3 v1 = v3(a.pos[3 * ib], a.pos[3 * ib + 1], a.pos[3 * ib + 2]);
+                V3 v2 = v3(a.pos[3 * ic], a.pos[3 * ic + 1], a.pos[3 * ic + 2]);
+                V3 e1 = v0 - v1, e2 = v2 - v0;
+                size_t o = 3 * (size_t)(tri_base + off + t);
+                w.wtris[o + 0] = make_float4(v0.x, v0.y, v0.z, __uint_as_float(tid));
+                w.wtris[o + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+                w.wtris[o + 2] = make_float4(e2.x, e2.y, e2.z, __uint_as_float(w.tri_geom[tid]));
+            }
+            off += cnt;
+        } else {
+            m = (unsigned)rank;
+            imask |= 1u << j;
+            w.wmap[child_base + rank] = ids[j];
+            ++rank;
+        }
+        meta[j >> 2] |= (m & 0xFFu) << (8 * (j & 3));
+    }
+    unsigned ebits = (unsigned)(ee[0] + 127) | ((unsigned)(ee[1] + 127) << 8) | ((unsigned)(ee[2] + 127) << 16) |
+                     (imask << 24);
+    float4 *nd = w.wnodes + 5 * (size_t)wi;
+    nd[0] = make_float4((float)pd[0], (float)pd[1], (float)pd[2], __uint_as_float(ebits));
+    nd[1] = make_float4(__uint_as_float(child_base), __uint_as_float(tri_base), __uint_as_float(meta[0]),
+                        __uint_as_float(meta[1]));
+    nd[2] = make_float4(__uint_as_float(qlo[0][0]), __uint_as_float(qlo[0][1]), __uint_as_float(qlo[1][0]),
+                        __uint_as_float(qlo[1][1]));
+    nd[3] = make_float4(__uint_as_float(qlo[2][0]), __uint_as_float(qlo[2][1]), __uint_as_float(qhi[0][0]),
+                        __uint_as_float(qhi[0][1]));
+    nd[4] = make_float4(__uint_as_float(qhi[1][0]), __uint_as_float(qhi[1][1]), __uint_as_float(qhi[2][0]),
+                        __uint_as_float(qhi[2][1]));
+}
+
+/* Wide root for a scene whose triangles form a single leaf (<= max_leaf). */
+__global__ void k_wide_single_leaf(const float *pos, const uint32_t *vidx, const uint32_t *tri_geom,
+                                   const uint64_t *keys, uint32_t n, float4 *wnodes, float4 *wtris) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float lo[3] = {__builtin_huge_valf(), __builtin_huge_valf(), __builtin_huge_valf()};
+    float hi[3] = {-__builtin_huge_valf(), -__builtin_huge_valf(), -__builtin_huge_valf()};
+    for (uint32_t t = 0; t < n; ++t) {
+        uint32_t tid = (uint32_t)(keys[t] & 0xFFFFFFFFull);
+        float bl[3], bh[3];
+        tri_box(pos, vidx, tid, bl, bh);
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fminf(lo[k], bl[k]);
+            hi[k] = fmaxf(hi[k], bh[k]);
+        }
+        uint32_t ia = vidx[3 * tid], ib = vidx[3 * tid + 1], ic = vidx[3 * tid + 2];
+        V3 v0 = v3(pos[3 * ia], pos[3 * ia + 1], pos[3 * ia + 2]);
+        V3 v1 = v3(pos[3 * ib], pos[3 * ib + 1], pos[3 * ib + 2]);
+        V3 v2 = v3(pos[3 * ic], pos[3 * ic + 1], pos[3 * ic + 2]);
+        V3 e1 = v0 - v1, e2 = v2 - v0;
+        wtris[3 * t + 0] = make_float4(v0.x, v0.y, v0.z, __uint_as_float(tid));
+        wtris[3 * t + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+        wtris[3 * t + 2] = make_float4(e2.x, e2.y, e2.z, __uint_as_float(tri_geom[tid]));
+    }
+    int ee[3];
+    double sc[3], pd[3];
+    unsigned q[3] = {0u, 0u, 0u};
+    for (int k = 0; k < 3; ++k) {
+        ee[k] = grid_exponent(lo[k], hi[k]);
+        sc[k] = ldexp(1.0, ee[k]);
+        pd[k] = floor((double)lo[k] / sc[k]) * sc[k];
+        double qh = ceil(((double)hi[k] - pd[k]) / sc[k]);
+        q[k] = (unsigned)fmin(fmax(qh, 0.0), 255.0);
+    }
+    unsigned ebits = (unsigned)(ee[0] + 127) | ((unsigned)(ee[1] + 127) << 8) | ((unsigned)(ee[2] + 127) << 16);
+    unsigned meta0 = (n << 5);   /* slot 0: leaf of n triangles at offset 0 */
+    wnodes[0] = make_float4((float)pd[0], (float)pd[1], (float)pd[2], __uint_as_float(ebits));
+    wnodes[1] = make_float4(__uint_as_float(0u), __uint_as_float(0u), __uint_as_float(meta0), __uint_as_float(0u));
+    wnodes[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);     /* qlo = 0 */
+    wnodes[3] = make_float4(0.0f, 0.0f, __uint_as_float(q[0]), 0.0f);
+    wnodes[4] = make_float4(__uint_as_float(q[1]), 0.0f, __uint_as_float(q[2]), 0.0f);
+}
+
 inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
 
 } // namespace
 
 int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const uint32_t *d_tri_geom, uint32_t n,
-              uint32_t max_leaf, DevBuf &nodes, DevBuf &tris, BvhStats &stats) {
+              uint32_t max_leaf, DevBuf &nodes, DevBuf &tris, DevBuf &wnodes, DevBuf &wtris, BvhStats &stats) {
     stats = BvhStats();
     if (n == 0) return LT_OK;
+    if (max_leaf > 4) return set_error(LT_E_INVALID, "BVH build: max_leaf_size must be <= 4 for the wide layout");
     const unsigned B = 256;
     int rc;
     DevBuf keys, keys_sorted, cb;
@@ -347,10 +547,16 @@ int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const 
                        (float4 *)tris.p);
     LT_HIP(hipGetLastError());
 
+    if ((rc = wtris.alloc(sizeof(float4) * 3 * (size_t)n))) return rc;
+    if ((rc = wnodes.alloc(sizeof(float4) * 5 * (size_t)std::max<uint32_t>(1u, n - 1)))) return rc;
     if (n == 1) {
         hipLaunchKernelGGL(k_single_leaf_root, dim3(1), dim3(64), 0, st, d_pos, d_vidx, sk, n, (float4 *)nodes.p,
                            (float4 *)tris.p);
+        hipLaunchKernelGGL(k_wide_single_leaf, dim3(1), dim3(64), 0, st, d_pos, d_vidx, d_tri_geom, sk, n,
+                           (float4 *)wnodes.p, (float4 *)wtris.p);
         LT_HIP(hipGetLastError());
+        stats.wide_nodes = 1;
+        stats.wide_depth = 1;
         stats.num_nodes = 1;
         stats.num_leaves = 1;
         stats.max_depth = 1;
@@ -401,7 +607,11 @@ int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const 
     if (root_collapse) {
         hipLaunchKernelGGL(k_single_leaf_root, dim3(1), dim3(64), 0, st, d_pos, d_vidx, sk, n, (float4 *)nodes.p,
                            (float4 *)tris.p);
+        hipLaunchKernelGGL(k_wide_single_leaf, dim3(1), dim3(64), 0, st, d_pos, d_vidx, d_tri_geom, sk, n,
+                           (float4 *)wnodes.p, (float4 *)wtris.p);
         LT_HIP(hipGetLastError());
+        stats.wide_nodes = 1;
+        stats.wide_depth = 1;
         stats.num_nodes = 1;
         stats.num_leaves = 1;
         stats.max_depth = 1;
@@ -419,6 +629,38 @@ int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const 
     stats.num_leaves = hs[1];
     stats.max_depth = hs[2];
     stats.sah_cost = root_lo.w;
+
+    /* 8-wide collapse, one level per pass */
+    DevBuf wmap, wcnt;
+    if ((rc = wmap.alloc(sizeof(int) * ni)) || (rc = wcnt.alloc(sizeof(unsigned) * 2))) return rc;
+    unsigned init_cnt[2] = {1u, 0u};
+    int zero = 0;
+    LT_HIP(hipMemcpyAsync(wcnt.p, init_cnt, sizeof init_cnt, hipMemcpyHostToDevice, st));
+    LT_HIP(hipMemcpyAsync(wmap.p, &zero, sizeof zero, hipMemcpyHostToDevice, st));
+    WideArgs wa;
+    wa.a = a;
+    wa.wmap = (int *)wmap.p;
+    wa.wnodes = (float4 *)wnodes.p;
+    wa.wtris = (float4 *)wtris.p;
+    wa.tri_geom = d_tri_geom;
+    wa.counters = (unsigned *)wcnt.p;
+    int begin = 0, end = 1, levels = 0;
+    while (begin < end) {
+        wa.begin = begin;
+        wa.end = end;
+        hipLaunchKernelGGL(k_wide_pass, dim3(grid_for(end - begin, 128)), dim3(128), 0, st, wa);
+        LT_HIP(hipGetLastError());
+        unsigned cnt[2];
+        LT_HIP(hipMemcpyAsync(cnt, wcnt.p, sizeof cnt, hipMemcpyDeviceToHost, st));
+        LT_HIP(hipStreamSynchronize(st));
+        ++levels;
+        begin = end;
+        end = (int)cnt[0];
+        if (levels > 64 || (size_t)end > ni) return set_error(LT_E_STATE, "BVH build: wide collapse did not converge");
+        if (begin == end && cnt[1] != n) return set_error(LT_E_STATE, "BVH build: wide collapse lost triangles");
+    }
+    stats.wide_nodes = (uint32_t)end;
+    stats.wide_depth = (uint32_t)levels;
     return LT_OK;
 }
 

@@ -276,6 +276,9 @@ struct DScene {
     const DGrid *grids;
     const float *voxels;
     const float4 *light_tris;  /* per light, its triangles in scene order: 3 x float4 like tris */
+    const float4 *wnodes;      /* 8-wide compressed nodes, 5 x float4 each */
+    const float4 *wtris;       /* triangles in wide-leaf order, 3 x float4 each */
+    uint32_t layout;           /* 0: wide, 1: binary */
     uint32_t ntris;
     uint32_t nlights;
     float bg[3];

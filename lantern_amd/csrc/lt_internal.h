@@ -51,19 +51,22 @@ struct DevBuf {
 struct BvhStats {
     uint32_t num_nodes = 0, num_leaves = 0, max_depth = 0;
     float sah_cost = 0.0f;
+    uint32_t wide_nodes = 0, wide_depth = 0;
 };
 
+/* Builds the binary hierarchy (nodes, tris) and its 8-wide compressed
+ * collapse (wnodes, wtris). */
 int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const uint32_t *d_tri_geom, uint32_t n,
-              uint32_t max_leaf, DevBuf &nodes, DevBuf &tris, BvhStats &stats);
+              uint32_t max_leaf, DevBuf &nodes, DevBuf &tris, DevBuf &wnodes, DevBuf &wtris, BvhStats &stats);
 
 /* Traversal launches (lt_trace.hip). count_ptr (device) overrides count when
  * non-null; work is a device counter that must be zero at launch. */
 int launch_trace_closest(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, const uint32_t *count_ptr,
-                         uint32_t count, uint32_t *work, int block, int grid, int *spill);
+                         uint32_t count, uint32_t *work, int block, int grid, int *spill, uint32_t *overflow);
 /* ignore_geom (nullable): per ray, a geomID whose triangles do not occlude. */
 int launch_trace_any(hipStream_t st, const DScene &sc, const Ray *rays, uint32_t *occluded,
                      const uint32_t *ignore_geom, const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block,
-                     int grid, int *spill);
+                     int grid, int *spill, uint32_t *overflow);
 int trace_grid(int num_cus, int block);
 size_t trace_spill_bytes(int grid, int block);
 int trace_default_block();
@@ -84,6 +87,11 @@ void camera_setup(const lt_camera &c, CameraSetup &out);
 
 } // namespace lt
 
+struct lt_device;
+namespace lt {
+int check_traversal_bounds(lt_device *d);
+}
+
 struct lt_device {
     int ordinal = 0;
     hipStream_t stream = nullptr;
@@ -94,6 +102,7 @@ struct lt_device {
     lt::DevBuf ctl;          /* queue counts and work counters */
     lt::DevBuf trace_work;   /* work counter for the intersect entry points */
     lt::DevBuf spill;        /* traversal stack overflow area */
+    lt::DevBuf diag;         /* [0]: rays that hit a traversal bound (stack/steps) */
     uint32_t *pinned = nullptr;
 };
 
@@ -117,7 +126,8 @@ struct lt_scene {
     /* device copies */
     std::vector<float> light_tris;    /* 12 floats per light triangle (v0|prim, e1, e2) */
     lt::DevBuf d_pos, d_nrm, d_uv, d_vidx, d_tri_geom, d_geoms, d_mats, d_media, d_texs, d_texels, d_grids,
-        d_voxels, d_lights, d_nodes, d_tris, d_light_tris;
+        d_voxels, d_lights, d_nodes, d_tris, d_light_tris, d_wnodes, d_wtris;
+    uint32_t layout = 0;   /* 0: 8-wide compressed nodes, 1: binary nodes */
     lt::BvhStats bvh;
     float build_ms = 0.0f;
     lt::DScene ds;

@@ -690,7 +690,7 @@ int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &
             if (wslot + 2 > 4096) return set_error(LT_E_STATE, "too many traversal launches in one wave");
             tm.begin(T_EXT);
             if ((rc = launch_trace_closest(st, a.sc, cur.rays, w.hits, &ctl->count[b], 0, &ctl->work[wslot++], tblock,
-                                           tgrid, spill)))
+                                           tgrid, spill, dev->diag.as<uint32_t>())))
                 return rc;
             tm.end(T_EXT);
             ++ext_launches;
@@ -705,10 +705,10 @@ int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &
              * the reference's closest-hit query */
             if (a.sc.light_first)
                 rc = launch_trace_any(st, a.sc, w.nee_rays, w.nee_occl, w.nee_ignore, &ctl->nee_count[b], 0,
-                                      &ctl->work[wslot++], tblock, tgrid, spill);
+                                      &ctl->work[wslot++], tblock, tgrid, spill, dev->diag.as<uint32_t>());
             else
                 rc = launch_trace_closest(st, a.sc, w.nee_rays, w.nee_hits, &ctl->nee_count[b], 0,
-                                          &ctl->work[wslot++], tblock, tgrid, spill);
+                                          &ctl->work[wslot++], tblock, tgrid, spill, dev->diag.as<uint32_t>());
             if (rc) return rc;
             tm.end(T_SHADOW);
             ++sh_launches;
@@ -744,6 +744,7 @@ int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &
     LT_HIP(hipEventElapsedTime(&total, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if ((rc = check_traversal_bounds(dev))) return rc;
     if (stats) {
         std::memset(stats, 0, sizeof *stats);
         stats->samples = (uint64_t)npix * prm.spp;

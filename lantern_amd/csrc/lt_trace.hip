@@ -65,7 +65,7 @@ template <bool ANY, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, const Ray *__restrict__ rays, Hit *__restrict__ hits,
                                                  uint32_t *__restrict__ occluded, const uint32_t *ignore_geom,
                                                  const uint32_t *count_ptr, uint32_t count_imm, uint32_t *work,
-                                                 int *spill) {
+                                                 int *spill, uint32_t *overflow) {
     __shared__ int lds[LDS_STACK * BLOCK];
     const uint32_t n = count_ptr ? *count_ptr : count_imm;
     const int tid = threadIdx.x;
@@ -104,6 +104,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, const Ray *__restric
             if (++steps > MAX_STEPS) {
                 best_prim = 0xFFFFFFFFu;
                 best_t = -1.0f;
+                atomicAdd(overflow, 1u);
                 break;
             }
             if (node >= 0) {
@@ -122,6 +123,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, const Ray *__restric
                     if (sp >= STACK_CAP) {
                         best_prim = 0xFFFFFFFFu;
                         best_t = -1.0f;
+                        atomicAdd(overflow, 1u);
                         break;
                     }
                     if (sp < LDS_STACK) lds[sp * BLOCK + tid] = farc;
@@ -173,13 +175,203 @@ __global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, const Ray *__restric
     }
 }
 
+
+/* ---- 8-wide compressed traversal -------------------------------------------
+ * One node fetch (80 B) tests eight quantized child boxes. Leaf children are
+ * intersected as soon as their box is hit (shrinking tfar for the remaining
+ * slots); the nearest internal child is descended into and the others are
+ * pushed with their entry distance, so a pop can discard children that lie
+ * beyond the closest hit found since. Stack entries (node, t) live in LDS,
+ * [depth][lane]; deeper entries spill to a per-thread slice of global memory.
+ * A push beyond STACK8_CAP is counted in *overflow and the ray ends as a
+ * miss, so an overflow is reported to the caller instead of going unnoticed. */
+constexpr int LDS8 = 16;
+constexpr int STACK8_CAP = 256;
+
+LT_D float qbyte(uint32_t w, int j) { return (float)((w >> (8 * (j & 3))) & 0xFFu); }
+
+template <bool ANY, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace8(DScene sc, const Ray *__restrict__ rays, Hit *__restrict__ hits,
+                                                  uint32_t *__restrict__ occluded, const uint32_t *ignore_geom,
+                                                  const uint32_t *count_ptr, uint32_t count_imm, uint32_t *work,
+                                                  int2 *spill, uint32_t *overflow) {
+    __shared__ int lds_node[LDS8 * BLOCK];
+    __shared__ float lds_t[LDS8 * BLOCK];
+    const uint32_t n = count_ptr ? *count_ptr : count_imm;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    int2 *gstack = spill + (size_t)(blockIdx.x * BLOCK + tid) * (STACK8_CAP - LDS8);
+    const float4 *__restrict__ wn = sc.wnodes;
+    const float4 *__restrict__ wt = sc.wtris;
+
+    while (true) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(work, 64u);
+        base = __shfl(base, 0);
+        if (base >= n) break;
+        const uint32_t i = base + lane;
+        if (i >= n) continue;
+
+        const float4 *rp = (const float4 *)(rays + i);
+        float4 ra = rp[0], rb = rp[1];
+        TravRay r;
+        r.o = v3(ra.x, ra.y, ra.z);
+        r.tnear = ra.w;
+        r.d = v3(rb.x, rb.y, rb.z);
+        r.tfar = rb.w;
+        r.tmin = fmaxf(r.tnear, 0.0f);
+        r.idir = v3(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
+        float best_t = r.tfar, best_u = 0.0f, best_v = 0.0f;
+        uint32_t best_prim = 0xFFFFFFFFu;
+        const uint32_t ignore = (ANY && ignore_geom) ? ignore_geom[i] : 0xFFFFFFFFu;
+        bool done = !(r.tfar >= 0.0f) || sc.ntris == 0;
+
+        int sp = 0;
+        int node = 0;
+        uint32_t steps = 0;
+        while (!done) {
+            if (++steps > MAX_STEPS) {
+                best_prim = 0xFFFFFFFFu;
+                best_t = -1.0f;
+                atomicAdd(overflow, 1u);
+                break;
+            }
+            const float4 *np = wn + 5 * (size_t)node;
+            const float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3], n4 = np[4];
+            const uint32_t eb = __float_as_uint(n0.w);
+            const float sx = __uint_as_float((eb & 0xFFu) << 23);
+            const float sy = __uint_as_float(((eb >> 8) & 0xFFu) << 23);
+            const float sz = __uint_as_float(((eb >> 16) & 0xFFu) << 23);
+            const uint32_t imask = eb >> 24;
+            const uint32_t cbase = __float_as_uint(n1.x), tbase = __float_as_uint(n1.y);
+            const uint32_t mw[2] = {__float_as_uint(n1.z), __float_as_uint(n1.w)};
+            const uint32_t qlx[2] = {__float_as_uint(n2.x), __float_as_uint(n2.y)};
+            const uint32_t qly[2] = {__float_as_uint(n2.z), __float_as_uint(n2.w)};
+            const uint32_t qlz[2] = {__float_as_uint(n3.x), __float_as_uint(n3.y)};
+            const uint32_t qhx[2] = {__float_as_uint(n3.z), __float_as_uint(n3.w)};
+            const uint32_t qhy[2] = {__float_as_uint(n4.x), __float_as_uint(n4.y)};
+            const uint32_t qhz[2] = {__float_as_uint(n4.z), __float_as_uint(n4.w)};
+            int near_node = -1;
+            float near_t = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t meta = (mw[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+                const bool isint = (imask >> j) & 1u;
+                if (!isint && meta == 0u) continue;
+                float tn;
+                const bool hit = slab(__builtin_fmaf(qbyte(qlx[j >> 2], j), sx, n0.x),
+                                      __builtin_fmaf(qbyte(qhx[j >> 2], j), sx, n0.x),
+                                      __builtin_fmaf(qbyte(qly[j >> 2], j), sy, n0.y),
+                                      __builtin_fmaf(qbyte(qhy[j >> 2], j), sy, n0.y),
+                                      __builtin_fmaf(qbyte(qlz[j >> 2], j), sz, n0.z),
+                                      __builtin_fmaf(qbyte(qhz[j >> 2], j), sz, n0.z), r, tn);
+                if (!hit) continue;
+                if (isint) {
+                    const int child = (int)(cbase + meta);
+                    if (near_node < 0) {
+                        near_node = child;
+                        near_t = tn;
+                    } else {
+                        int pn = child;
+                        float pt = tn;
+                        if (tn < near_t) {
+                            pn = near_node;
+                            pt = near_t;
+                            near_node = child;
+                            near_t = tn;
+                        }
+                        if (sp >= STACK8_CAP) {
+                            atomicAdd(overflow, 1u);
+                            best_prim = 0xFFFFFFFFu;
+                            best_t = -1.0f;
+                            done = true;
+                            break;
+                        }
+                        if (sp < LDS8) {
+                            lds_node[sp * BLOCK + tid] = pn;
+                            lds_t[sp * BLOCK + tid] = pt;
+                        } else {
+                            gstack[sp - LDS8] = make_int2(pn, __float_as_int(pt));
+                        }
+                        ++sp;
+                    }
+                } else {
+                    const uint32_t first = tbase + (meta & 31u), cnt = meta >> 5;
+                    for (uint32_t k = 0; k < cnt; ++k) {
+                        const float4 *tp = wt + 3 * (size_t)(first + k);
+                        const float4 a = tp[0], b = tp[1], c = tp[2];
+                        float t, u, v;
+                        if ((!ANY || __float_as_uint(c.w) != ignore) &&
+                            moeller(r.o, r.d, r.tnear, r.tfar, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z),
+                                    t, u, v)) {
+                            r.tfar = t;
+                            best_t = t;
+                            best_u = u;
+                            best_v = v;
+                            best_prim = __float_as_uint(a.w);
+                            if (ANY) {
+                                done = true;
+                                break;
+                            }
+                        }
+                    }
+                    if (done) break;
+                }
+            }
+            if (done) break;
+            if (near_node >= 0) {
+                node = near_node;
+                continue;
+            }
+            bool found = false;
+            const float limit = r.tfar * 1.0000005f;
+            while (sp > 0) {
+                --sp;
+                int nd;
+                float t;
+                if (sp < LDS8) {
+                    nd = lds_node[sp * BLOCK + tid];
+                    t = lds_t[sp * BLOCK + tid];
+                } else {
+                    int2 e = gstack[sp - LDS8];
+                    nd = e.x;
+                    t = __int_as_float(e.y);
+                }
+                if (t <= limit) {
+                    node = nd;
+                    found = true;
+                    break;
+                }
+            }
+            if (!found) break;
+        }
+        if (ANY) {
+            occluded[i] = best_prim != 0xFFFFFFFFu ? 1u : 0u;
+        } else {
+            *(float4 *)(hits + i) = make_float4(best_t, best_u, best_v, __uint_as_float(best_prim));
+        }
+    }
+}
+
 template <bool ANY>
 int launch(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, uint32_t *occl, const uint32_t *ign,
-           const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block, int grid, int *spill) {
+           const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block, int grid, int *spill,
+           uint32_t *overflow) {
+    if (sc.layout == 0) {
+        int2 *sp2 = (int2 *)spill;
+        switch (block) {
+        case 64: hipLaunchKernelGGL((k_trace8<ANY, 64>), dim3(grid), dim3(64), 0, st, sc, rays, hits, occl, ign, count_ptr, count, work, sp2, overflow); break;
+        case 128: hipLaunchKernelGGL((k_trace8<ANY, 128>), dim3(grid), dim3(128), 0, st, sc, rays, hits, occl, ign, count_ptr, count, work, sp2, overflow); break;
+        case 256: hipLaunchKernelGGL((k_trace8<ANY, 256>), dim3(grid), dim3(256), 0, st, sc, rays, hits, occl, ign, count_ptr, count, work, sp2, overflow); break;
+        default: return set_error(LT_E_INVALID, "traversal block size must be 64, 128 or 256");
+        }
+        LT_HIP(hipGetLastError());
+        return LT_OK;
+    }
     switch (block) {
-    case 64: hipLaunchKernelGGL((k_trace<ANY, 64>), dim3(grid), dim3(64), 0, st, sc, rays, hits, occl, ign, count_ptr, count, work, spill); break;
-    case 128: hipLaunchKernelGGL((k_trace<ANY, 128>), dim3(grid), dim3(128), 0, st, sc, rays, hits, occl, ign, count_ptr, count, work, spill); break;
-    case 256: hipLaunchKernelGGL((k_trace<ANY, 256>), dim3(grid), dim3(256), 0, st, sc, rays, hits, occl, ign, count_ptr, count, work, spill); break;
+    case 64: hipLaunchKernelGGL((k_trace<ANY, 64>), dim3(grid), dim3(64), 0, st, sc, rays, hits, occl, ign, count_ptr, count, work, spill, overflow); break;
+    case 128: hipLaunchKernelGGL((k_trace<ANY, 128>), dim3(grid), dim3(128), 0, st, sc, rays, hits, occl, ign, count_ptr, count, work, spill, overflow); break;
+    case 256: hipLaunchKernelGGL((k_trace<ANY, 256>), dim3(grid), dim3(256), 0, st, sc, rays, hits, occl, ign, count_ptr, count, work, spill, overflow); break;
     default: return set_error(LT_E_INVALID, "traversal block size must be 64, 128 or 256");
     }
     LT_HIP(hipGetLastError());
@@ -190,25 +382,31 @@ int launch(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, uint32_
 
 int trace_default_block() { return 128; }
 
-/* Resident blocks: LDS per block is LDS_STACK * block * 4 B of the CU's
- * 160 KB, and at most 32 waves per CU are worth scheduling. */
+/* Resident blocks: LDS per block is the larger of the two kernels' stacks
+ * (binary: LDS_STACK * 4 B, wide: LDS8 * 8 B per thread) of the CU's 160 KB,
+ * and at most 32 waves per CU are worth scheduling. */
 int trace_grid(int num_cus, int block) {
     int waves_per_block = block / 64;
-    int by_lds = (160 * 1024) / (LDS_STACK * block * 4);
+    int lds_per_thread = std::max(LDS_STACK * 4, LDS8 * 8);
+    int by_lds = (160 * 1024) / (lds_per_thread * block);
     int by_waves = std::max(1, 32 / waves_per_block);
     return std::max(1, num_cus) * std::max(1, std::min(by_lds, by_waves));
 }
 
-size_t trace_spill_bytes(int grid, int block) { return (size_t)grid * block * (STACK_CAP - LDS_STACK) * sizeof(int); }
+size_t trace_spill_bytes(int grid, int block) {
+    size_t per = std::max((size_t)(STACK_CAP - LDS_STACK) * sizeof(int), (size_t)(STACK8_CAP - LDS8) * sizeof(int2));
+    return (size_t)grid * block * per;
+}
 
 int launch_trace_closest(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, const uint32_t *count_ptr,
-                         uint32_t count, uint32_t *work, int block, int grid, int *spill) {
-    return launch<false>(st, sc, rays, hits, nullptr, nullptr, count_ptr, count, work, block, grid, spill);
+                         uint32_t count, uint32_t *work, int block, int grid, int *spill, uint32_t *overflow) {
+    return launch<false>(st, sc, rays, hits, nullptr, nullptr, count_ptr, count, work, block, grid, spill, overflow);
 }
 int launch_trace_any(hipStream_t st, const DScene &sc, const Ray *rays, uint32_t *occluded,
                      const uint32_t *ignore_geom, const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block,
-                     int grid, int *spill) {
-    return launch<true>(st, sc, rays, nullptr, occluded, ignore_geom, count_ptr, count, work, block, grid, spill);
+                     int grid, int *spill, uint32_t *overflow) {
+    return launch<true>(st, sc, rays, nullptr, occluded, ignore_geom, count_ptr, count, work, block, grid, spill,
+                        overflow);
 }
 
 } // namespace lt

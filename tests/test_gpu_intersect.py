@@ -60,9 +60,13 @@ def _compare(gs, os_, rays, brute=True):
     return hit_o.sum(), (ties > 1).sum()
 
 
-def test_nine_sphere_random_rays_match_brute_force(gpu_device):
+LAYOUTS = [pytest.param(0, id="wide8"), pytest.param(1, id="binary")]
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_nine_sphere_random_rays_match_brute_force(gpu_device, layout):
     d = scenes.nine_sphere(64, 64, 1)
-    gs = lantern_amd.Scene(gpu_device, d)
+    gs = lantern_amd.Scene(gpu_device, d, layout=layout)
     os_ = OracleScene(d)
     rays = _random_rays(3000, 11)
     nhit, nties = _compare(gs, os_, rays)
@@ -72,9 +76,10 @@ def test_nine_sphere_random_rays_match_brute_force(gpu_device):
     gs.close()
 
 
-def test_nine_sphere_camera_rays_match_brute_force(gpu_device):
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_nine_sphere_camera_rays_match_brute_force(gpu_device, layout):
     d = scenes.nine_sphere(512, 512, 1)
-    gs = lantern_amd.Scene(gpu_device, d)
+    gs = lantern_amd.Scene(gpu_device, d, layout=layout)
     os_ = OracleScene(d)
     rays = _camera_rays(d, 3000, 12)
     nhit, _ = _compare(gs, os_, rays)
@@ -82,9 +87,10 @@ def test_nine_sphere_camera_rays_match_brute_force(gpu_device):
     gs.close()
 
 
-def test_any_hit_agrees_with_closest_hit(gpu_device):
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_any_hit_agrees_with_closest_hit(gpu_device, layout):
     d = scenes.nine_sphere(64, 64, 1)
-    gs = lantern_amd.Scene(gpu_device, d)
+    gs = lantern_amd.Scene(gpu_device, d, layout=layout)
     rays = _random_rays(50000, 13)
     _, prim = gs.intersect(rays)
     occ = gs.intersect(rays, any_hit=True)
@@ -92,38 +98,42 @@ def test_any_hit_agrees_with_closest_hit(gpu_device):
     gs.close()
 
 
-def test_triangle_heavy_full_size_matches_oracle_bvh(gpu_device):
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_triangle_heavy_full_size_matches_oracle_bvh(gpu_device, layout):
     d = scenes.triangle_heavy(64, 36, 1)
     assert d.num_triangles == 327684
-    gs = lantern_amd.Scene(gpu_device, d)
+    gs = lantern_amd.Scene(gpu_device, d, layout=layout)
     os_ = OracleScene(d)
     rays = np.concatenate([_random_rays(20000, 14, -3, 3), _camera_rays(d, 20000, 15)])
     nhit, _ = _compare(gs, os_, rays, brute=False)
     assert nhit > 20000
     info = gs.info()
     assert info["num_nodes"] > 0 and info["max_depth"] <= 64
+    assert info["layout"] == layout and 0 < info["wide_nodes"] < info["num_nodes"]
     gs.close()
 
 
-@pytest.mark.parametrize("ntris", [1, 2, 3, 5])
-def test_tiny_scenes(gpu_device, ntris):
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("ntris", [1, 2, 3, 5, 9, 40])
+def test_tiny_scenes(gpu_device, ntris, layout):
     rng = np.random.default_rng(ntris)
     tris = rng.uniform(-1, 1, (ntris, 3, 3)).astype(np.float32)
     d = SceneDesc()
     m = d.add_material(bsdf=0, albedo=(0.5, 0.5, 0.5))
     d.add_mesh(tris.reshape(-1, 3), np.arange(3 * ntris, dtype=np.uint32).reshape(-1, 3), material=m)
-    gs = lantern_amd.Scene(gpu_device, d)
+    gs = lantern_amd.Scene(gpu_device, d, layout=layout)
     os_ = OracleScene(d)
     rays = _random_rays(4000, 20 + ntris, -2, 2)
     _compare(gs, os_, rays)
     gs.close()
 
 
-def test_inactive_rays_and_tfar_semantics(gpu_device):
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_inactive_rays_and_tfar_semantics(gpu_device, layout):
     d = SceneDesc()
     m = d.add_material(bsdf=0, albedo=(0.5, 0.5, 0.5))
     d.add_mesh(np.array([[0, 0, 5], [1, 0, 5], [0, 1, 5]], np.float32), np.array([[0, 1, 2]], np.uint32), material=m)
-    gs = lantern_amd.Scene(gpu_device, d)
+    gs = lantern_amd.Scene(gpu_device, d, layout=layout)
     rays = np.array([[0, 0, 0, 0.0, 0.25, 0.25, 5.0, np.inf],
                      [0, 0, 0, 1.0, 0.25, 0.25, 5.0, np.inf],    # tnear == t: strict, miss
                      [0, 0, 0, 0.0, 0.25, 0.25, 5.0, 1.0],       # tfar == t: inclusive, hit
@@ -132,4 +142,35 @@ def test_inactive_rays_and_tfar_semantics(gpu_device):
     tuv, prim = gs.intersect(rays)
     assert list(prim != 0xFFFFFFFF) == [True, False, True, False, False]
     assert tuv[0, 0] == 1.0 and tuv[2, 0] == 1.0
+    gs.close()
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_axis_aligned_and_far_geometry(gpu_device, layout):
+    """Flat (zero-thickness) boxes, axis-parallel rays and geometry far from the
+    origin exercise the quantized-box grid choice."""
+    rng = np.random.default_rng(7)
+    d = SceneDesc()
+    m = d.add_material(bsdf=0, albedo=(0.5, 0.5, 0.5))
+    from lantern_amd import scenes as sc
+    gp, gn, guv, gi = sc.grid(40.0, 40.0, 9, 9)            # 128 tris in the y = 0 plane
+    d.add_mesh(gp, gi, gn, guv, m)
+    sp, sn, si = sc.geosphere(0.01, 2)                     # tiny sphere far away
+    d.add_mesh(sp + np.float32([1000.0, 3.0, -2000.0]), si, sn, None, m)
+    gs = lantern_amd.Scene(gpu_device, d, layout=layout)
+    os_ = OracleScene(d)
+    n = 3000
+    r = np.zeros((n, 8), np.float32)
+    r[:, 0:3] = rng.uniform(-25, 25, (n, 3))
+    r[:, 1] = np.abs(r[:, 1]) + 0.5
+    axis = rng.integers(0, 3, n)
+    r[np.arange(n), 4 + axis] = np.where(axis == 1, -1.0, rng.choice([-1.0, 1.0], n))
+    r[:, 7] = np.inf
+    far = np.zeros((500, 8), np.float32)
+    tgt = sp[rng.integers(0, len(sp), 500)] + np.float32([1000.0, 3.0, -2000.0])
+    far[:, 0:3] = tgt + rng.normal(0, 0.05, (500, 3)).astype(np.float32) + np.float32([0, 0, 1.0])
+    far[:, 4:7] = tgt - far[:, 0:3]
+    far[:, 7] = np.inf
+    nhit, _ = _compare(gs, os_, np.concatenate([r, far]))
+    assert nhit > 1000
     gs.close()
