@@ -36,6 +36,7 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--wave-paths", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--layout", type=int, default=0, help="BVH layout: 0 8-wide compressed, 1 binary")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -61,7 +62,7 @@ def main():
     params["wave_paths"] = args.wave_paths
     params["block_size"] = args.block
     with lantern_amd.Device(0) as dev:
-        scene = lantern_amd.Scene(dev, desc)
+        scene = lantern_amd.Scene(dev, desc, layout=args.layout)
         info = scene.info()
         film = lantern_amd.Film(dev, args.width, args.height)
         for _ in range(args.warmup):
@@ -125,7 +126,9 @@ def main():
                      "kernel": "k_trace (closest hit: extension + next-event rays)",
                      "bytes_per_ray": RAY_BYTES, "launches_per_step": launches // args.steps,
                      "avg_launch_ms": round(trav_ms / max(launches, 1), 4)},
-        "bvh": {"nodes": info["num_nodes"], "leaves": info["num_leaves"], "depth": info["max_depth"],
+        "bvh": {"layout": "wide8" if info["layout"] == 0 else "binary", "wide_nodes": info["wide_nodes"],
+                "wide_depth": info["wide_depth"], "nodes": info["num_nodes"], "leaves": info["num_leaves"],
+                "depth": info["max_depth"],
                 "sah_cost": round(info["sah_cost"], 3), "build_ms": round(info["build_ms"], 2)},
         "mean_radiance": round(mean_radiance, 6),
         "scene_gen_s": round(gen_s, 2),
