@@ -42,7 +42,7 @@ namespace {
 /* Control block in device memory. */
 struct Ctl {
     uint32_t count[2048];      /* live paths entering bounce b */
-    uint32_t nee_count[2048];  /* next-event rays written at bounce b (2 per shaded slot) */
+    uint32_t nee_count[2048];  /* next-event rays queued (compacted) at bounce b */
     uint32_t work[4096];       /* traversal work counters, one per launch */
     unsigned long long shadow_rays;
     unsigned long long pad[7];
@@ -186,9 +186,8 @@ __device__ __forceinline__ bool light_hit(const DScene &sc, const DLight &L, con
 /* One bounce of RenderPixel for every live path (integrator.cpp:119-242). */
 __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next,
                                                NeeRec nee, Ray *nee_rays, Hit *nee_hits, uint32_t *nee_ignore,
-                                               Ctl *ctl) {
+                                               uint32_t *nee_cidx, Ctl *ctl) {
     const uint32_t count = ctl->count[b];
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->nee_count[b] = 2 * count;
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
@@ -201,6 +200,8 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
         float tfar1 = 0.0f, tfar2 = 0.0f;
         uint32_t p = 0;
         V3 new_o = v3s(0.0f), new_d = v3s(0.0f);
+        V3 P = v3s(0.0f), wi1 = v3s(0.0f), wi2 = v3s(0.0f);
+        uint32_t ignore_geom = 0xFFFFFFFFu;
         if (q < count) {
             p = cur.path[q];
             uint32_t x, y, frame;
@@ -218,7 +219,6 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
             int medium = a.ps.medium[p];
             bool dead = false;
             uint32_t nee_info = 0;
-            V3 P = v3s(0.0f), wi1 = v3s(0.0f), wi2 = v3s(0.0f);
             if (h.prim == 0xFFFFFFFFu) {
                 C = C + T * v3(sc.bg[0], sc.bg[1], sc.bg[2]);
                 dead = true;
@@ -345,8 +345,7 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
                             nee.f2x[q] = f2.x; nee.f2y[q] = f2.y; nee.f2z[q] = f2.z; nee.p2[q] = p2;
                             nee.nl[q] = (float)nl;
                             nee_info = li | (nee1 ? 0x40000000u : 0u) | (nee2 ? 0x80000000u : 0u);
-                            nee_ignore[2 * (size_t)q] = L.geom;
-                            nee_ignore[2 * (size_t)q + 1] = L.geom;
+                            ignore_geom = L.geom;
                         }
                     }
                     /* continuation (integrator.cpp:199-231) */
@@ -371,8 +370,6 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
                 }
             }
             nee.info[q] = nee_info;
-            store_ray(nee_rays + 2 * (size_t)q, P, wi1, 0.001f, nee1 ? tfar1 : -1.0f);
-            store_ray(nee_rays + 2 * (size_t)q + 1, P, wi2, 0.001f, nee2 ? tfar2 : -1.0f);
             alive = !dead && (b + 1 < a.max_depth);
             if (!alive) a.ps.bounces[p] = dead ? b : b + 1;
             a.ps.tx[p] = T.x; a.ps.ty[p] = T.y; a.ps.tz[p] = T.z;
@@ -382,22 +379,38 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
             a.ps.lobe[p] = (uint32_t)it.lobe;
             a.ps.rngi[p] = rng.i;
         }
-        /* wave-level compaction of the survivors */
+        /* wave-level compaction of the survivors and of the next-event rays */
+        const unsigned long long below = (1ull << lane) - 1ull;
         unsigned long long mask = __ballot(alive);
-        unsigned long long nmask = __ballot(nee1);
+        unsigned long long nmask1 = __ballot(nee1);
         unsigned long long nmask2 = __ballot(nee2);
-        uint32_t slot0 = 0;
+        uint32_t slot0 = 0, nslot0 = 0;
         if (lane == 0) {
             if (mask) slot0 = atomicAdd(&ctl->count[b + 1], (uint32_t)__popcll(mask));
-            uint32_t ns = (uint32_t)(__popcll(nmask) + __popcll(nmask2));
-            if (ns) atomicAdd(&ctl->shadow_rays, (unsigned long long)ns);
+            uint32_t ns = (uint32_t)(__popcll(nmask1) + __popcll(nmask2));
+            if (ns) nslot0 = atomicAdd(&ctl->nee_count[b], ns);
         }
         slot0 = __shfl(slot0, 0);
+        nslot0 = __shfl(nslot0, 0);
         if (alive) {
-            uint32_t off = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+            uint32_t off = (uint32_t)__popcll(mask & below);
             uint32_t s = slot0 + off;
             next.path[s] = p;
             store_ray(next.rays + s, new_o, new_d, 0.001f, inf);
+        }
+        if (nee1 || nee2) {
+            uint32_t c = nslot0 + (uint32_t)(__popcll(nmask1 & below) + __popcll(nmask2 & below));
+            if (nee1) {
+                store_ray(nee_rays + c, P, wi1, 0.001f, tfar1);
+                nee_ignore[c] = ignore_geom;
+                nee_cidx[2 * (size_t)q] = c;
+                ++c;
+            }
+            if (nee2) {
+                store_ray(nee_rays + c, P, wi2, 0.001f, tfar2);
+                nee_ignore[c] = ignore_geom;
+                nee_cidx[2 * (size_t)q + 1] = c;
+            }
         }
     }
 }
@@ -405,7 +418,8 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
 /* The casts of AreaLight::SampleLi / PdfLi resolved (area_light.cpp:58-71,
  * 100-109) and the MIS sums of EstimateDirect (integrator.cpp:287-314). */
 __global__ __launch_bounds__(256) void k_resolve(WaveArgs a, uint32_t b, Queue cur, NeeRec nee, const Ray *nee_rays,
-                                                 const Hit *nee_hits, const uint32_t *nee_occl, Ctl *ctl) {
+                                                 const Hit *nee_hits, const Hit *nee_trace_hits,
+                                                 const uint32_t *nee_occl, const uint32_t *nee_cidx, Ctl *ctl) {
     const uint32_t count = ctl->count[b];
     const DScene &sc = a.sc;
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
@@ -416,11 +430,12 @@ __global__ __launch_bounds__(256) void k_resolve(WaveArgs a, uint32_t b, Queue c
         V3 le = v3(L.le[0], L.le[1], L.le[2]);
         V3 direct = v3s(0.0f);
         if (info & 0x40000000u) {
-            const Hit h = nee_hits[2 * (size_t)q];
-            bool reached = sc.light_first ? nee_occl[2 * (size_t)q] == 0u
+            const uint32_t c = nee_cidx[2 * (size_t)q];
+            const Hit h = sc.light_first ? nee_hits[2 * (size_t)q] : nee_trace_hits[c];
+            bool reached = sc.light_first ? nee_occl[c] == 0u
                                           : (h.prim != 0xFFFFFFFFu && sc.tri_geom[h.prim] == L.geom);
             if (reached) {
-                const Ray r = nee_rays[2 * (size_t)q];
+                const Ray r = nee_rays[c];
                 V3 o = v3(r.ox, r.oy, r.oz), d = v3(r.dx, r.dy, r.dz);
                 V3 ip = o + normalize(d) * h.t;
                 V3 dd = ip - o;
@@ -435,11 +450,12 @@ __global__ __launch_bounds__(256) void k_resolve(WaveArgs a, uint32_t b, Queue c
             }
         }
         if (info & 0x80000000u) {
-            const Hit h = nee_hits[2 * (size_t)q + 1];
-            bool reached = sc.light_first ? nee_occl[2 * (size_t)q + 1] == 0u
+            const uint32_t c = nee_cidx[2 * (size_t)q + 1];
+            const Hit h = sc.light_first ? nee_hits[2 * (size_t)q + 1] : nee_trace_hits[c];
+            bool reached = sc.light_first ? nee_occl[c] == 0u
                                           : (h.prim != 0xFFFFFFFFu && sc.tri_geom[h.prim] == L.geom);
             if (reached) {
-                const Ray r = nee_rays[2 * (size_t)q + 1];
+                const Ray r = nee_rays[c];
                 V3 o = v3(r.ox, r.oy, r.oz), d = v3(r.dx, r.dy, r.dz);
                 V3 ip = o + normalize(d) * h.t;
                 V3 dd = ip - o;
@@ -496,7 +512,7 @@ __global__ void k_reset_ctl(Ctl *ctl) {
 /* Bytes of workspace per path in flight. */
 constexpr size_t kPathBytes = 4 * 11;                 /* path state */
 constexpr size_t kQueueBytes = sizeof(Ray) + 4;       /* per queue slot */
-constexpr size_t kNeeBytes = 4 * 14 + 2 * sizeof(Ray) + 2 * sizeof(Hit) + 2 * 4 + 2 * 4;
+constexpr size_t kNeeBytes = 4 * 14 + 2 * sizeof(Ray) + 4 * sizeof(Hit) + 2 * 4 + 2 * 4 + 2 * 4;
 constexpr size_t kBytesPerPath = kPathBytes + 2 * kQueueBytes + sizeof(Hit) + kNeeBytes;
 
 struct Workspace {
@@ -506,8 +522,10 @@ struct Workspace {
     NeeRec nee;
     Ray *nee_rays;
     Hit *nee_hits;
-    uint32_t *nee_occl;
-    uint32_t *nee_ignore;    /* per next-event ray: the light's geomID */
+    uint32_t *nee_occl;      /* per compacted next-event ray */
+    uint32_t *nee_ignore;    /* per compacted next-event ray: the light's geomID */
+    Hit *nee_trace_hits;     /* per compacted next-event ray (closest-hit mode) */
+    uint32_t *nee_cidx;      /* per (slot, ray): index of its compacted ray */
 };
 
 int carve(lt_device *dev, size_t n, Workspace &w) {
@@ -542,6 +560,8 @@ int carve(lt_device *dev, size_t n, Workspace &w) {
     w.nee_hits = (Hit *)take(2 * sizeof(Hit));
     w.nee_occl = (uint32_t *)take(2 * 4);
     w.nee_ignore = (uint32_t *)take(2 * 4);
+    w.nee_trace_hits = (Hit *)take(2 * sizeof(Hit));
+    w.nee_cidx = (uint32_t *)take(2 * 4);
     return LT_OK;
 }
 
@@ -696,7 +716,7 @@ int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &
             ++ext_launches;
             tm.begin(T_SHADE);
             hipLaunchKernelGGL(k_shade, dim3(sgrid), dim3(256), 0, st, a, b, cur, (const Hit *)w.hits, nxt, w.nee,
-                               w.nee_rays, w.nee_hits, w.nee_ignore, ctl);
+                               w.nee_rays, w.nee_hits, w.nee_ignore, w.nee_cidx, ctl);
             tm.end(T_SHADE);
             LT_HIP(hipGetLastError());
             /* next-event rays: two per shaded slot; inactive ones have tfar < 0 */
@@ -707,14 +727,15 @@ int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &
                 rc = launch_trace_any(st, a.sc, w.nee_rays, w.nee_occl, w.nee_ignore, &ctl->nee_count[b], 0,
                                       &ctl->work[wslot++], tblock, tgrid, spill, dev->diag.as<uint32_t>());
             else
-                rc = launch_trace_closest(st, a.sc, w.nee_rays, w.nee_hits, &ctl->nee_count[b], 0,
+                rc = launch_trace_closest(st, a.sc, w.nee_rays, w.nee_trace_hits, &ctl->nee_count[b], 0,
                                           &ctl->work[wslot++], tblock, tgrid, spill, dev->diag.as<uint32_t>());
             if (rc) return rc;
             tm.end(T_SHADOW);
             ++sh_launches;
             tm.begin(T_RESOLVE);
             hipLaunchKernelGGL(k_resolve, dim3(sgrid), dim3(256), 0, st, a, b, cur, w.nee, (const Ray *)w.nee_rays,
-                               (const Hit *)w.nee_hits, (const uint32_t *)w.nee_occl, ctl);
+                               (const Hit *)w.nee_hits, (const Hit *)w.nee_trace_hits, (const uint32_t *)w.nee_occl,
+                               (const uint32_t *)w.nee_cidx, ctl);
             tm.end(T_RESOLVE);
             LT_HIP(hipGetLastError());
             std::swap(cur, nxt);
@@ -725,17 +746,17 @@ int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &
         tm.end(T_ACC);
         LT_HIP(hipGetLastError());
         if (stats) {
-            std::vector<uint32_t> counts(prm.max_depth + 1);
-            unsigned long long sh = 0;
+            std::vector<uint32_t> counts(prm.max_depth + 1), ncounts(prm.max_depth + 1);
             LT_HIP(hipMemcpyAsync(counts.data(), ctl->count, sizeof(uint32_t) * (prm.max_depth + 1),
                                   hipMemcpyDeviceToHost, st));
-            LT_HIP(hipMemcpyAsync(&sh, &ctl->shadow_rays, sizeof sh, hipMemcpyDeviceToHost, st));
+            LT_HIP(hipMemcpyAsync(ncounts.data(), ctl->nee_count, sizeof(uint32_t) * (prm.max_depth + 1),
+                                  hipMemcpyDeviceToHost, st));
             LT_HIP(hipStreamSynchronize(st));
             for (uint32_t b = 0; b < prm.max_depth; ++b) {
                 closest += counts[b];
                 segments += counts[b];
+                shadow += ncounts[b];
             }
-            shadow += sh;
         }
     }
     LT_HIP(hipEventRecord(e1, st));

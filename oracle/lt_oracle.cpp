@@ -598,13 +598,17 @@ bool intersect_brute(const orc_scene &s, const RayR<R> &ray, HitR<R> &hit, u32 *
         }
     }
     if (ties) {
+        /* Order-dependent winners: every triangle that the acceptance test
+         * T <= |den| * tfar (triangle_intersector_moeller.h:106) would accept
+         * with tfar = the winning t. Traversal order decides among these in
+         * Embree too ("<=" lets a later candidate replace an earlier one). */
         u32 n = 0;
         if (found)
             for (size_t i = 0; i < s.tris.size(); ++i) {
                 V3<R> v0, e1, e2;
                 tri_edges<R>(s.tris[i], v0, e1, e2);
                 R t, u, v;
-                if (moeller(ray.o, ray.d, ray.tnear, ray.tfar, v0, e1, e2, t, u, v) && t == hit.t) ++n;
+                if (moeller(ray.o, ray.d, ray.tnear, hit.t, v0, e1, e2, t, u, v)) ++n;
             }
         *ties = n;
     }

@@ -126,8 +126,9 @@ uint32_t orc_scene_lights(const orc_scene *s, float *out);
  * triangle_intersector_moeller.h:75-113, intersector_epilog.h:241-305).
  * rays: n*8 floats. out_tuv: n*3 floats; out_geom/out_prim: n u32
  * (0xFFFFFFFF on miss). use_bvh: 0 brute force over all triangles in
- * order, 1 the oracle's binned-SAH BVH. out_ties (nullable): number of
- * triangles whose t equals the winning t exactly (brute force only). */
+ * order, 1 the oracle's binned-SAH BVH. out_ties (nullable, brute force
+ * only): number of triangles the acceptance test T <= |den| * tfar accepts
+ * at tfar = the winning t; above 1, the winner depends on visiting order. */
 void orc_intersect(const orc_scene *s, const float *rays, uint32_t n, int32_t use_bvh,
                    float *out_tuv, uint32_t *out_geom, uint32_t *out_prim, uint32_t *out_ties);
 void orc_intersect_f64(const orc_scene *s, const double *rays, uint32_t n,
