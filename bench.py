@@ -37,6 +37,7 @@ def main():
     ap.add_argument("--wave-paths", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--layout", type=int, default=0, help="BVH layout: 0 8-wide compressed, 1 binary")
+    ap.add_argument("--builder", type=int, default=0, help="BVH builder: 0 PLOC, 1 linear BVH")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -62,7 +63,7 @@ def main():
     params["wave_paths"] = args.wave_paths
     params["block_size"] = args.block
     with lantern_amd.Device(0) as dev:
-        scene = lantern_amd.Scene(dev, desc, layout=args.layout)
+        scene = lantern_amd.Scene(dev, desc, layout=args.layout, builder=args.builder)
         info = scene.info()
         film = lantern_amd.Film(dev, args.width, args.height)
         for _ in range(args.warmup):
@@ -126,7 +127,9 @@ def main():
                      "kernel": "k_trace (closest hit: extension + next-event rays)",
                      "bytes_per_ray": RAY_BYTES, "launches_per_step": launches // args.steps,
                      "avg_launch_ms": round(trav_ms / max(launches, 1), 4)},
-        "bvh": {"layout": "wide8" if info["layout"] == 0 else "binary", "wide_nodes": info["wide_nodes"],
+        "bvh": {"layout": "wide8" if info["layout"] == 0 else "binary",
+                "builder": "ploc" if args.builder == 0 else "lbvh", "build_iterations": info["build_iterations"],
+                "wide_nodes": info["wide_nodes"],
                 "wide_depth": info["wide_depth"], "nodes": info["num_nodes"], "leaves": info["num_leaves"],
                 "depth": info["max_depth"],
                 "sah_cost": round(info["sah_cost"], 3), "build_ms": round(info["build_ms"], 2)},

@@ -90,7 +90,8 @@ typedef struct lt_medium {
 typedef struct lt_build_params {
     uint32_t max_leaf_size;   /* 1..4, default 4 (Embree's sahBlockSize/minLeaf is 4) */
     uint32_t layout;          /* 0: 8-wide compressed nodes (default), 1: binary nodes */
-    uint32_t reserved[6];
+    uint32_t builder;         /* 0: PLOC clustering (default), 1: linear BVH (Karras) */
+    uint32_t reserved[5];
 } lt_build_params;
 
 typedef struct lt_scene_info {
@@ -106,7 +107,7 @@ typedef struct lt_scene_info {
     uint32_t wide_nodes;      /* 8-wide compressed nodes */
     uint32_t wide_depth;
     uint32_t layout;          /* layout the traversal uses */
-    uint32_t reserved;
+    uint32_t build_iterations;/* PLOC iterations (0 for the linear BVH) */
 } lt_scene_info;
 
 /* rtcNewScene + rtcSetSceneFlags/BuildQuality (scene.cpp:196-198). */

@@ -30,7 +30,8 @@ class lt_medium(C.Structure):
 
 
 class lt_build_params(C.Structure):
-    _fields_ = [("max_leaf_size", C.c_uint32), ("layout", C.c_uint32), ("reserved", C.c_uint32 * 6)]
+    _fields_ = [("max_leaf_size", C.c_uint32), ("layout", C.c_uint32), ("builder", C.c_uint32),
+                ("reserved", C.c_uint32 * 5)]
 
 
 class lt_scene_info(C.Structure):
@@ -38,7 +39,7 @@ class lt_scene_info(C.Structure):
                 ("num_leaves", C.c_uint32), ("max_depth", C.c_uint32), ("num_lights", C.c_uint32),
                 ("sah_cost", C.c_float), ("build_ms", C.c_float), ("device_bytes", C.c_uint64),
                 ("wide_nodes", C.c_uint32), ("wide_depth", C.c_uint32), ("layout", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("build_iterations", C.c_uint32)]
 
 
 class lt_ray(C.Structure):

@@ -492,6 +492,8 @@ int lt_scene_commit(lt_scene *s, const lt_build_params *params) {
     LT_CHECK(max_leaf >= 1 && max_leaf <= 4, "lt_scene_commit: max_leaf_size must be 1..4");
     uint32_t layout = params ? params->layout : 0;
     LT_CHECK(layout <= 1, "lt_scene_commit: layout must be 0 (wide) or 1 (binary)");
+    uint32_t builder = params ? params->builder : 0;
+    LT_CHECK(builder <= 1, "lt_scene_commit: builder must be 0 (PLOC) or 1 (LBVH)");
     s->layout = layout;
     hipStream_t st = s->dev->stream;
     LT_HIP(hipSetDevice(s->dev->ordinal));
@@ -512,7 +514,7 @@ int lt_scene_commit(lt_scene *s, const lt_build_params *params) {
     auto t0 = std::chrono::steady_clock::now();
     uint32_t ntris = (uint32_t)s->tri_geom.size();
     if ((rc = build_bvh(st, s->d_pos.as<float>(), s->d_vidx.as<uint32_t>(), s->d_tri_geom.as<uint32_t>(), ntris, max_leaf,
-                        s->d_nodes, s->d_tris, s->d_wnodes, s->d_wtris, s->bvh)))
+                        builder, s->d_nodes, s->d_tris, s->d_wnodes, s->d_wtris, s->bvh)))
         return rc;
     LT_HIP(hipStreamSynchronize(st));
     s->build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -557,6 +559,7 @@ int lt_scene_get_info(lt_scene *s, lt_scene_info *o) {
     o->wide_nodes = s->bvh.wide_nodes;
     o->wide_depth = s->bvh.wide_depth;
     o->layout = s->layout;
+    o->build_iterations = s->bvh.build_iterations;
     o->build_ms = s->build_ms;
     o->device_bytes = s->d_pos.bytes + s->d_nrm.bytes + s->d_uv.bytes + s->d_vidx.bytes + s->d_tri_geom.bytes +
                       s->d_nodes.bytes + s->d_tris.bytes + s->d_texels.bytes + s->d_voxels.bytes + s->d_wnodes.bytes +

@@ -503,12 +503,214 @@ __global__ void k_wide_single_leaf(const float *pos, const uint32_t *vidx, const
     wnodes[4] = make_float4(__uint_as_float(q[1]), 0.0f, __uint_as_float(q[2]), 0.0f);
 }
 
+
+/* ---- PLOC: parallel locally-ordered clustering ------------------------------
+ * D. Meister and J. Bittner, "Parallel Locally-Ordered Clustering for Bounding
+ * Volume Hierarchy Construction", IEEE TVCG 2018. Clusters start as the
+ * Morton-ordered triangles; each iteration every cluster finds the cluster
+ * within +-radius positions whose merged box has the smallest surface area,
+ * mutual nearest neighbours merge, and the survivors are compacted in order.
+ * Node ids: leaves 0..n-1 (Morton position), internal n.. in creation order;
+ * iter[] records the iteration that created each internal node, so the
+ * follow-up passes can walk the tree bottom-up or top-down one iteration at a
+ * time without any intra-launch dependence. */
+struct PlocArrays {
+    float4 *lo, *hi;      /* boxes of all 2n-1 nodes */
+    int2 *kids;           /* internal node (id - n) -> child ids */
+    int *iter;            /* internal node (id - n) -> creating iteration */
+    int n;
+};
+
+__global__ void k_ploc_leaf_boxes(const float *pos, const uint32_t *vidx, const uint64_t *keys, int n, PlocArrays P,
+                                  int *clusters) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    float lo[3], hi[3];
+    tri_box(pos, vidx, (uint32_t)(keys[k] & 0xFFFFFFFFull), lo, hi);
+    P.lo[k] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+    P.hi[k] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+    clusters[k] = k;
+}
+
+LT_D float merged_area(float4 al, float4 ah, float4 bl, float4 bh) {
+    float dx = fmaxf(ah.x, bh.x) - fminf(al.x, bl.x);
+    float dy = fmaxf(ah.y, bh.y) - fminf(al.y, bl.y);
+    float dz = fmaxf(ah.z, bh.z) - fminf(al.z, bl.z);
+    return dx * dy + dy * dz + dz * dx;
+}
+
+__global__ void k_ploc_nn(const int *C, int m, PlocArrays P, int radius, int *nn) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int ci = C[i];
+    const float4 al = P.lo[ci], ah = P.hi[ci];
+    float best = __builtin_huge_valf();
+    int bj = i == 0 ? 1 : i - 1;
+    const int j0 = max(0, i - radius), j1 = min(m - 1, i + radius);
+    for (int j = j0; j <= j1; ++j) {
+        if (j == i) continue;
+        const int cj = C[j];
+        float a = merged_area(al, ah, P.lo[cj], P.hi[cj]);
+        if (a < best) {   /* ties keep the lower index */
+            best = a;
+            bj = j;
+        }
+    }
+    nn[i] = bj;
+}
+
+__global__ void k_ploc_merge(const int *C, const int *nn, int m, PlocArrays P, int iter, unsigned *next_id, int *val,
+                             int *keep) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int j = nn[i];
+    if (nn[j] == i) {
+        if (i < j) {
+            const int a = C[i], b = C[j];
+            const int id = P.n + (int)atomicAdd(next_id, 1u);
+            const float4 al = P.lo[a], ah = P.hi[a], bl = P.lo[b], bh = P.hi[b];
+            P.lo[id] = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.0f);
+            P.hi[id] = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.0f);
+            P.kids[id - P.n] = make_int2(a, b);
+            P.iter[id - P.n] = iter;
+            val[i] = id;
+            keep[i] = 1;
+        } else {
+            keep[i] = 0;
+        }
+    } else {
+        val[i] = C[i];
+        keep[i] = 1;
+    }
+}
+
+__global__ void k_ploc_scatter(const int *val, const int *keep, const int *pos, int m, int *Cnext) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    if (keep[i]) Cnext[pos[i]] = val[i];
+}
+
+/* Bottom-up leaf counts, one creation iteration per launch. */
+__global__ void k_ploc_count(PlocArrays P, int iter, int *cnt) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= P.n - 1 || P.iter[v] != iter) return;
+    int2 k = P.kids[v];
+    int a = k.x < P.n ? 1 : cnt[k.x - P.n];
+    int b = k.y < P.n ? 1 : cnt[k.y - P.n];
+    cnt[v] = a + b;
+}
+
+/* Top-down depth-first numbering, one creation iteration per launch, from the
+ * last: off = first leaf position of the subtree, idx = pre-order index of an
+ * internal node (root 0). */
+__global__ void k_ploc_number(PlocArrays P, int iter, const int *cnt, int *off, int *idx, int *leaf_off) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= P.n - 1 || P.iter[v] != iter) return;
+    int2 k = P.kids[v];
+    int o = off[v], x = idx[v];
+    int ca = k.x < P.n ? 1 : cnt[k.x - P.n];
+    if (k.x < P.n) leaf_off[k.x] = o;
+    else { off[k.x - P.n] = o; idx[k.x - P.n] = x + 1; }
+    if (k.y < P.n) leaf_off[k.y] = o + ca;
+    else { off[k.y - P.n] = o + ca; idx[k.y - P.n] = x + ca; }
+}
+
+/* Writes the Karras-style arrays (child, range, parent_int) in pre-order and
+ * the triangle ids in depth-first leaf order. */
+__global__ void k_ploc_convert(PlocArrays P, const int *cnt, const int *off, const int *idx, const int *leaf_off,
+                               const uint64_t *keys, int2 *child, int2 *range, int *parent_int, uint64_t *keys_out) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < P.n) keys_out[leaf_off[v]] = keys[v];
+    if (v >= P.n - 1) return;
+    int2 k = P.kids[v];
+    int i = idx[v];
+    int a = k.x < P.n ? ~leaf_off[k.x] : idx[k.x - P.n];
+    int b = k.y < P.n ? ~leaf_off[k.y] : idx[k.y - P.n];
+    child[i] = make_int2(a, b);
+    range[i] = make_int2(off[v], off[v] + cnt[v] - 1);
+    if (a >= 0) parent_int[a] = i;
+    if (b >= 0) parent_int[b] = i;
+}
+
 inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
 
 } // namespace
 
+/* PLOC on the Morton-ordered keys; fills child/range/parent_int and the
+ * depth-first triangle order keys_out. Returns the number of iterations. */
+static int ploc_build(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const uint64_t *sk, uint32_t n,
+                      int radius, DevBuf &child, DevBuf &range, DevBuf &pint, DevBuf &keys_out, int &iters) {
+    const unsigned B = 256;
+    int rc;
+    const size_t ni = n - 1;
+    DevBuf lo, hi, kids, iter, ca, cb, nn, val, keep, posb, cnt, off, idx, loff, nid, temp;
+    if ((rc = lo.alloc(sizeof(float4) * (2 * (size_t)n - 1))) || (rc = hi.alloc(sizeof(float4) * (2 * (size_t)n - 1))) ||
+        (rc = kids.alloc(sizeof(int2) * ni)) || (rc = iter.alloc(sizeof(int) * ni)) || (rc = ca.alloc(sizeof(int) * n)) ||
+        (rc = cb.alloc(sizeof(int) * n)) || (rc = nn.alloc(sizeof(int) * n)) || (rc = val.alloc(sizeof(int) * n)) ||
+        (rc = keep.alloc(sizeof(int) * n)) || (rc = posb.alloc(sizeof(int) * n)) || (rc = cnt.alloc(sizeof(int) * ni)) ||
+        (rc = off.alloc(sizeof(int) * ni)) || (rc = idx.alloc(sizeof(int) * ni)) || (rc = loff.alloc(sizeof(int) * n)) ||
+        (rc = nid.alloc(sizeof(unsigned))))
+        return rc;
+    PlocArrays P;
+    P.lo = (float4 *)lo.p;
+    P.hi = (float4 *)hi.p;
+    P.kids = (int2 *)kids.p;
+    P.iter = (int *)iter.p;
+    P.n = (int)n;
+    LT_HIP(hipMemsetAsync(nid.p, 0, sizeof(unsigned), st));
+    hipLaunchKernelGGL(k_ploc_leaf_boxes, dim3(grid_for(n, B)), dim3(B), 0, st, d_pos, d_vidx, sk, (int)n, P, (int *)ca.p);
+    LT_HIP(hipGetLastError());
+    size_t temp_bytes = 0;
+    LT_HIP(rocprim::exclusive_scan(nullptr, temp_bytes, (int *)keep.p, (int *)posb.p, 0, (size_t)n, rocprim::plus<int>(),
+                                   st));
+    if ((rc = temp.alloc(std::max<size_t>(temp_bytes, 16)))) return rc;
+    int *C = (int *)ca.p, *Cn = (int *)cb.p;
+    int m = (int)n;
+    iters = 0;
+    while (m > 1) {
+        hipLaunchKernelGGL(k_ploc_nn, dim3(grid_for(m, B)), dim3(B), 0, st, C, m, P, radius, (int *)nn.p);
+        hipLaunchKernelGGL(k_ploc_merge, dim3(grid_for(m, B)), dim3(B), 0, st, C, (const int *)nn.p, m, P, iters,
+                           (unsigned *)nid.p, (int *)val.p, (int *)keep.p);
+        LT_HIP(hipGetLastError());
+        LT_HIP(rocprim::exclusive_scan(temp.p, temp_bytes, (int *)keep.p, (int *)posb.p, 0, (size_t)m,
+                                       rocprim::plus<int>(), st));
+        hipLaunchKernelGGL(k_ploc_scatter, dim3(grid_for(m, B)), dim3(B), 0, st, (const int *)val.p, (const int *)keep.p,
+                           (const int *)posb.p, m, Cn);
+        LT_HIP(hipGetLastError());
+        int last_pos = 0, last_keep = 0;
+        LT_HIP(hipMemcpyAsync(&last_pos, (int *)posb.p + (m - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+        LT_HIP(hipMemcpyAsync(&last_keep, (int *)keep.p + (m - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+        LT_HIP(hipStreamSynchronize(st));
+        int m2 = last_pos + last_keep;
+        if (m2 >= m) return set_error(LT_E_STATE, "BVH build: PLOC made no progress");
+        m = m2;
+        std::swap(C, Cn);
+        ++iters;
+    }
+    /* C[0] is the root: internal id n + (n - 2) */
+    LT_HIP(hipMemsetAsync(off.p, 0, sizeof(int) * ni, st));
+    LT_HIP(hipMemsetAsync(idx.p, 0, sizeof(int) * ni, st));
+    for (int it = 0; it < iters; ++it)
+        hipLaunchKernelGGL(k_ploc_count, dim3(grid_for(ni, B)), dim3(B), 0, st, P, it, (int *)cnt.p);
+    for (int it = iters - 1; it >= 0; --it)
+        hipLaunchKernelGGL(k_ploc_number, dim3(grid_for(ni, B)), dim3(B), 0, st, P, it, (const int *)cnt.p, (int *)off.p,
+                           (int *)idx.p, (int *)loff.p);
+    LT_HIP(hipGetLastError());
+    if ((rc = child.alloc(sizeof(int2) * ni)) || (rc = range.alloc(sizeof(int2) * ni)) ||
+        (rc = pint.alloc(sizeof(int) * ni)) || (rc = keys_out.alloc(sizeof(uint64_t) * n)))
+        return rc;
+    LT_HIP(hipMemsetAsync(pint.p, 0, sizeof(int) * ni, st));
+    hipLaunchKernelGGL(k_ploc_convert, dim3(grid_for(n, B)), dim3(B), 0, st, P, (const int *)cnt.p, (const int *)off.p,
+                       (const int *)idx.p, (const int *)loff.p, sk, (int2 *)child.p, (int2 *)range.p, (int *)pint.p,
+                       (uint64_t *)keys_out.p);
+    LT_HIP(hipGetLastError());
+    LT_HIP(hipStreamSynchronize(st));
+    return LT_OK;
+}
+
 int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const uint32_t *d_tri_geom, uint32_t n,
-              uint32_t max_leaf, DevBuf &nodes, DevBuf &tris, DevBuf &wnodes, DevBuf &wtris, BvhStats &stats) {
+              uint32_t max_leaf, uint32_t builder, DevBuf &nodes, DevBuf &tris, DevBuf &wnodes, DevBuf &wtris,
+              BvhStats &stats) {
     stats = BvhStats();
     if (n == 0) return LT_OK;
     if (max_leaf > 4) return set_error(LT_E_INVALID, "BVH build: max_leaf_size must be <= 4 for the wide layout");
@@ -543,13 +745,10 @@ int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const 
     uint32_t nnodes = n > 1 ? n - 1 : 1;
     if ((rc = nodes.alloc(sizeof(float4) * 4 * (size_t)nnodes))) return rc;
     LT_HIP(hipMemsetAsync(nodes.p, 0, sizeof(float4) * 4 * (size_t)nnodes, st));
-    hipLaunchKernelGGL(k_write_tris, dim3(grid_for(n, B)), dim3(B), 0, st, d_pos, d_vidx, d_tri_geom, sk, n,
-                       (float4 *)tris.p);
-    LT_HIP(hipGetLastError());
-
     if ((rc = wtris.alloc(sizeof(float4) * 3 * (size_t)n))) return rc;
     if ((rc = wnodes.alloc(sizeof(float4) * 5 * (size_t)std::max<uint32_t>(1u, n - 1)))) return rc;
     if (n == 1) {
+        hipLaunchKernelGGL(k_write_tris, dim3(1), dim3(B), 0, st, d_pos, d_vidx, d_tri_geom, sk, n, (float4 *)tris.p);
         hipLaunchKernelGGL(k_single_leaf_root, dim3(1), dim3(64), 0, st, d_pos, d_vidx, sk, n, (float4 *)nodes.p,
                            (float4 *)tris.p);
         hipLaunchKernelGGL(k_wide_single_leaf, dim3(1), dim3(64), 0, st, d_pos, d_vidx, d_tri_geom, sk, n,
@@ -564,20 +763,31 @@ int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const 
         return LT_OK;
     }
 
-    DevBuf child, range, pint, pleaf, blo, bhi, done, collapse, emit, dstats;
+    DevBuf child, range, pint, pleaf, blo, bhi, done, collapse, emit, dstats, dfs_keys;
     size_t ni = n - 1;
-    if ((rc = child.alloc(sizeof(int2) * ni)) || (rc = range.alloc(sizeof(int2) * ni)) ||
-        (rc = pint.alloc(sizeof(int) * ni)) || (rc = pleaf.alloc(sizeof(int) * n)) ||
-        (rc = blo.alloc(sizeof(float4) * ni)) || (rc = bhi.alloc(sizeof(float4) * ni)) ||
+    if (builder == 0) {
+        int iters = 0;
+        if ((rc = ploc_build(st, d_pos, d_vidx, sk, n, 16, child, range, pint, dfs_keys, iters))) return rc;
+        sk = (const uint64_t *)dfs_keys.p;
+        stats.build_iterations = (uint32_t)iters;
+    } else {
+        if ((rc = child.alloc(sizeof(int2) * ni)) || (rc = range.alloc(sizeof(int2) * ni)) ||
+            (rc = pint.alloc(sizeof(int) * ni)) || (rc = pleaf.alloc(sizeof(int) * n)))
+            return rc;
+        LT_HIP(hipMemsetAsync(pint.p, 0, sizeof(int) * ni, st));
+        hipLaunchKernelGGL(k_karras, dim3(grid_for(ni, B)), dim3(B), 0, st, sk, (int)n, (int2 *)child.p, (int *)pint.p,
+                           (int *)pleaf.p, (int2 *)range.p);
+        LT_HIP(hipGetLastError());
+    }
+    if ((rc = blo.alloc(sizeof(float4) * ni)) || (rc = bhi.alloc(sizeof(float4) * ni)) ||
         (rc = done.alloc(sizeof(int) * ni)) || (rc = collapse.alloc(sizeof(int) * ni)) ||
         (rc = emit.alloc(sizeof(int) * ni)) || (rc = dstats.alloc(sizeof(unsigned) * 4)))
         return rc;
     LT_HIP(hipMemsetAsync(done.p, 0, sizeof(int) * ni, st));
     LT_HIP(hipMemsetAsync(collapse.p, 0, sizeof(int) * ni, st));
-    LT_HIP(hipMemsetAsync(pint.p, 0, sizeof(int) * ni, st));
     LT_HIP(hipMemsetAsync(dstats.p, 0, sizeof(unsigned) * 4, st));
-    hipLaunchKernelGGL(k_karras, dim3(grid_for(ni, B)), dim3(B), 0, st, sk, (int)n, (int2 *)child.p, (int *)pint.p,
-                       (int *)pleaf.p, (int2 *)range.p);
+    hipLaunchKernelGGL(k_write_tris, dim3(grid_for(n, B)), dim3(B), 0, st, d_pos, d_vidx, d_tri_geom, sk, n,
+                       (float4 *)tris.p);
     LT_HIP(hipGetLastError());
     BuildArrays a;
     a.pos = d_pos;
@@ -592,18 +802,20 @@ int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const 
     a.collapse = (int *)collapse.p;
     a.n = (int)n;
     a.max_leaf = (int)max_leaf;
-    /* The key is 62 bits wide, so a node's common prefix grows by at least one
-     * bit per level and the hierarchy is at most 63 levels deep. */
-    for (int pass = 0; pass < 64; ++pass)
-        hipLaunchKernelGGL(k_refit_pass, dim3(grid_for(ni, B)), dim3(B), 0, st, a, pass);
-    LT_HIP(hipGetLastError());
+    /* Bottom-up passes until the root completes (one tree level per pass). */
     int root_done = 0, root_collapse = 0;
     float4 root_lo;
-    LT_HIP(hipMemcpyAsync(&root_done, done.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    for (int pass = 0; pass < 4096 && !root_done; pass += 16) {
+        for (int k = 0; k < 16; ++k)
+            hipLaunchKernelGGL(k_refit_pass, dim3(grid_for(ni, B)), dim3(B), 0, st, a, pass + k);
+        LT_HIP(hipGetLastError());
+        LT_HIP(hipMemcpyAsync(&root_done, done.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        LT_HIP(hipStreamSynchronize(st));
+    }
     LT_HIP(hipMemcpyAsync(&root_collapse, collapse.p, sizeof(int), hipMemcpyDeviceToHost, st));
     LT_HIP(hipMemcpyAsync(&root_lo, blo.p, sizeof(float4), hipMemcpyDeviceToHost, st));
     LT_HIP(hipStreamSynchronize(st));
-    if (!root_done) return set_error(LT_E_STATE, "BVH build: hierarchy deeper than 64 levels");
+    if (!root_done) return set_error(LT_E_STATE, "BVH build: hierarchy deeper than 4096 levels");
     if (root_collapse) {
         hipLaunchKernelGGL(k_single_leaf_root, dim3(1), dim3(64), 0, st, d_pos, d_vidx, sk, n, (float4 *)nodes.p,
                            (float4 *)tris.p);
@@ -656,7 +868,7 @@ int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const 
         ++levels;
         begin = end;
         end = (int)cnt[0];
-        if (levels > 64 || (size_t)end > ni) return set_error(LT_E_STATE, "BVH build: wide collapse did not converge");
+        if (levels > 4096 || (size_t)end > ni) return set_error(LT_E_STATE, "BVH build: wide collapse did not converge");
         if (begin == end && cnt[1] != n) return set_error(LT_E_STATE, "BVH build: wide collapse lost triangles");
     }
     stats.wide_nodes = (uint32_t)end;

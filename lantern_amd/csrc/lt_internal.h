@@ -52,12 +52,14 @@ struct BvhStats {
     uint32_t num_nodes = 0, num_leaves = 0, max_depth = 0;
     float sah_cost = 0.0f;
     uint32_t wide_nodes = 0, wide_depth = 0;
+    uint32_t build_iterations = 0;
 };
 
 /* Builds the binary hierarchy (nodes, tris) and its 8-wide compressed
  * collapse (wnodes, wtris). */
 int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const uint32_t *d_tri_geom, uint32_t n,
-              uint32_t max_leaf, DevBuf &nodes, DevBuf &tris, DevBuf &wnodes, DevBuf &wtris, BvhStats &stats);
+              uint32_t max_leaf, uint32_t builder, DevBuf &nodes, DevBuf &tris, DevBuf &wnodes, DevBuf &wtris,
+              BvhStats &stats);
 
 /* Traversal launches (lt_trace.hip). count_ptr (device) overrides count when
  * non-null; work is a device counter that must be zero at launch. */

@@ -70,12 +70,14 @@ def _compare(gs, os_, rays, brute=True):
 
 
 LAYOUTS = [pytest.param(0, id="wide8"), pytest.param(1, id="binary")]
+BUILDERS = [pytest.param(0, id="ploc"), pytest.param(1, id="lbvh")]
 
 
+@pytest.mark.parametrize("builder", BUILDERS)
 @pytest.mark.parametrize("layout", LAYOUTS)
-def test_nine_sphere_random_rays_match_brute_force(gpu_device, layout):
+def test_nine_sphere_random_rays_match_brute_force(gpu_device, layout, builder):
     d = scenes.nine_sphere(64, 64, 1)
-    gs = lantern_amd.Scene(gpu_device, d, layout=layout)
+    gs = lantern_amd.Scene(gpu_device, d, layout=layout, builder=builder)
     os_ = OracleScene(d)
     rays = _random_rays(3000, 11)
     nhit, nties = _compare(gs, os_, rays)
@@ -107,11 +109,12 @@ def test_any_hit_agrees_with_closest_hit(gpu_device, layout):
     gs.close()
 
 
+@pytest.mark.parametrize("builder", BUILDERS)
 @pytest.mark.parametrize("layout", LAYOUTS)
-def test_triangle_heavy_full_size_matches_oracle_bvh(gpu_device, layout):
+def test_triangle_heavy_full_size_matches_oracle_bvh(gpu_device, layout, builder):
     d = scenes.triangle_heavy(64, 36, 1)
     assert d.num_triangles == 327684
-    gs = lantern_amd.Scene(gpu_device, d, layout=layout)
+    gs = lantern_amd.Scene(gpu_device, d, layout=layout, builder=builder)
     os_ = OracleScene(d)
     rays = np.concatenate([_random_rays(20000, 14, -3, 3), _camera_rays(d, 20000, 15)])
     nhit, _ = _compare(gs, os_, rays, brute=False)
@@ -122,15 +125,16 @@ def test_triangle_heavy_full_size_matches_oracle_bvh(gpu_device, layout):
     gs.close()
 
 
+@pytest.mark.parametrize("builder", BUILDERS)
 @pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("ntris", [1, 2, 3, 5, 9, 40])
-def test_tiny_scenes(gpu_device, ntris, layout):
+def test_tiny_scenes(gpu_device, ntris, layout, builder):
     rng = np.random.default_rng(ntris)
     tris = rng.uniform(-1, 1, (ntris, 3, 3)).astype(np.float32)
     d = SceneDesc()
     m = d.add_material(bsdf=0, albedo=(0.5, 0.5, 0.5))
     d.add_mesh(tris.reshape(-1, 3), np.arange(3 * ntris, dtype=np.uint32).reshape(-1, 3), material=m)
-    gs = lantern_amd.Scene(gpu_device, d, layout=layout)
+    gs = lantern_amd.Scene(gpu_device, d, layout=layout, builder=builder)
     os_ = OracleScene(d)
     rays = _random_rays(4000, 20 + ntris, -2, 2)
     _compare(gs, os_, rays)
@@ -154,8 +158,9 @@ def test_inactive_rays_and_tfar_semantics(gpu_device, layout):
     gs.close()
 
 
+@pytest.mark.parametrize("builder", BUILDERS)
 @pytest.mark.parametrize("layout", LAYOUTS)
-def test_axis_aligned_and_far_geometry(gpu_device, layout):
+def test_axis_aligned_and_far_geometry(gpu_device, layout, builder):
     """Flat (zero-thickness) boxes, axis-parallel rays and geometry far from the
     origin exercise the quantized-box grid choice."""
     rng = np.random.default_rng(7)
@@ -166,7 +171,7 @@ def test_axis_aligned_and_far_geometry(gpu_device, layout):
     d.add_mesh(gp, gi, gn, guv, m)
     sp, sn, si = sc.geosphere(0.01, 2)                     # tiny sphere far away
     d.add_mesh(sp + np.float32([1000.0, 3.0, -2000.0]), si, sn, None, m)
-    gs = lantern_amd.Scene(gpu_device, d, layout=layout)
+    gs = lantern_amd.Scene(gpu_device, d, layout=layout, builder=builder)
     os_ = OracleScene(d)
     n = 3000
     r = np.zeros((n, 8), np.float32)
