@@ -38,6 +38,7 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--layout", type=int, default=0, help="BVH layout: 0 8-wide compressed, 1 binary")
     ap.add_argument("--builder", type=int, default=0, help="BVH builder: 0 PLOC, 1 linear BVH")
+    ap.add_argument("--variant", type=int, default=0, help="traversal kernel variant (0 automatic, 1..4)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -62,6 +63,7 @@ def main():
     params = dict(desc.params)
     params["wave_paths"] = args.wave_paths
     params["block_size"] = args.block
+    params["trace_variant"] = args.variant
     with lantern_amd.Device(0) as dev:
         scene = lantern_amd.Scene(dev, desc, layout=args.layout, builder=args.builder)
         info = scene.info()

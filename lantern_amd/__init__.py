@@ -199,6 +199,7 @@ def params_struct(p):
     r.wave_paths = p.get("wave_paths", 0)
     r.block_size = p.get("block_size", 0)
     r.use_graph = p.get("use_graph", 0)
+    r.trace_variant = p.get("trace_variant", 0)
     return r
 
 

@@ -607,9 +607,10 @@ static int intersect_device(lt_scene *s, const lt_ray *rays, void *out, uint32_t
     LT_HIP(hipMemsetAsync(d->trace_work.p, 0, sizeof(uint32_t), st));
     if (any)
         return launch_trace_any(st, s->ds, (const Ray *)rays, (uint32_t *)out, nullptr, nullptr, n,
-                                d->trace_work.as<uint32_t>(), block, grid, (int *)d->spill.p, d->diag.as<uint32_t>());
+                                d->trace_work.as<uint32_t>(), block, grid, (int *)d->spill.p, d->diag.as<uint32_t>(),
+                                kDefaultTraceVariant);
     return launch_trace_closest(st, s->ds, (const Ray *)rays, (Hit *)out, nullptr, n, d->trace_work.as<uint32_t>(),
-                                block, grid, (int *)d->spill.p, d->diag.as<uint32_t>());
+                                block, grid, (int *)d->spill.p, d->diag.as<uint32_t>(), kDefaultTraceVariant);
 }
 
 int lt_intersect_closest_device(lt_scene *s, const lt_ray *rays, lt_hit *hits, uint32_t n, void *stream) {

@@ -63,12 +63,18 @@ int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const 
 
 /* Traversal launches (lt_trace.hip). count_ptr (device) overrides count when
  * non-null; work is a device counter that must be zero at launch. */
+/* Traversal kernel variants of the 8-wide layout (LDS stack depth,
+ * occupancy target): 0 = (16, compiler), 1 = (8, compiler), 2 = (8, 6 waves),
+ * 3 = (8, 8 waves). The binary layout ignores the variant. */
+constexpr int kTraceVariants = 4;
+constexpr int kDefaultTraceVariant = 0;
 int launch_trace_closest(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, const uint32_t *count_ptr,
-                         uint32_t count, uint32_t *work, int block, int grid, int *spill, uint32_t *overflow);
+                         uint32_t count, uint32_t *work, int block, int grid, int *spill, uint32_t *overflow,
+                         int variant);
 /* ignore_geom (nullable): per ray, a geomID whose triangles do not occlude. */
 int launch_trace_any(hipStream_t st, const DScene &sc, const Ray *rays, uint32_t *occluded,
                      const uint32_t *ignore_geom, const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block,
-                     int grid, int *spill, uint32_t *overflow);
+                     int grid, int *spill, uint32_t *overflow, int variant);
 int trace_grid(int num_cus, int block);
 size_t trace_spill_bytes(int grid, int block);
 int trace_default_block();

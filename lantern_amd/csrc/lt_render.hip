@@ -661,6 +661,7 @@ int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &
     if ((rc = dev->ctl.reserve(sizeof(Ctl)))) return rc;
     Ctl *ctl = (Ctl *)dev->ctl.p;
     const int tblock = prm.block_size ? (int)prm.block_size : trace_default_block();
+    const int tvar = prm.trace_variant ? (int)prm.trace_variant - 1 : kDefaultTraceVariant;
     const int tgrid = trace_grid(dev->num_cus, tblock);
     if ((rc = dev->spill.reserve(trace_spill_bytes(tgrid, tblock)))) return rc;
     int *spill = (int *)dev->spill.p;
@@ -710,7 +711,7 @@ int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &
             if (wslot + 2 > 4096) return set_error(LT_E_STATE, "too many traversal launches in one wave");
             tm.begin(T_EXT);
             if ((rc = launch_trace_closest(st, a.sc, cur.rays, w.hits, &ctl->count[b], 0, &ctl->work[wslot++], tblock,
-                                           tgrid, spill, dev->diag.as<uint32_t>())))
+                                           tgrid, spill, dev->diag.as<uint32_t>(), tvar)))
                 return rc;
             tm.end(T_EXT);
             ++ext_launches;
@@ -725,10 +726,10 @@ int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &
              * the reference's closest-hit query */
             if (a.sc.light_first)
                 rc = launch_trace_any(st, a.sc, w.nee_rays, w.nee_occl, w.nee_ignore, &ctl->nee_count[b], 0,
-                                      &ctl->work[wslot++], tblock, tgrid, spill, dev->diag.as<uint32_t>());
+                                      &ctl->work[wslot++], tblock, tgrid, spill, dev->diag.as<uint32_t>(), tvar);
             else
                 rc = launch_trace_closest(st, a.sc, w.nee_rays, w.nee_trace_hits, &ctl->nee_count[b], 0,
-                                          &ctl->work[wslot++], tblock, tgrid, spill, dev->diag.as<uint32_t>());
+                                          &ctl->work[wslot++], tblock, tgrid, spill, dev->diag.as<uint32_t>(), tvar);
             if (rc) return rc;
             tm.end(T_SHADOW);
             ++sh_launches;

@@ -185,22 +185,23 @@ __global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, const Ray *__restric
  * [depth][lane]; deeper entries spill to a per-thread slice of global memory.
  * A push beyond STACK8_CAP is counted in *overflow and the ray ends as a
  * miss, so an overflow is reported to the caller instead of going unnoticed. */
-constexpr int LDS8 = 16;
 constexpr int STACK8_CAP = 256;
 
 LT_D float qbyte(uint32_t w, int j) { return (float)((w >> (8 * (j & 3))) & 0xFFu); }
 
-template <bool ANY, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_trace8(DScene sc, const Ray *__restrict__ rays, Hit *__restrict__ hits,
-                                                  uint32_t *__restrict__ occluded, const uint32_t *ignore_geom,
-                                                  const uint32_t *count_ptr, uint32_t count_imm, uint32_t *work,
-                                                  int2 *spill, uint32_t *overflow) {
+/* LDS8: stack entries kept in LDS per lane; WAVES: occupancy the register
+ * allocator targets (0: the compiler's choice). */
+template <bool ANY, int BLOCK, int LDS8, int WAVES>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVES : 1)))
+void k_trace8(DScene sc, const Ray *__restrict__ rays, Hit *__restrict__ hits, uint32_t *__restrict__ occluded,
+              const uint32_t *ignore_geom, const uint32_t *count_ptr, uint32_t count_imm, uint32_t *work, int2 *spill,
+              uint32_t *overflow) {
     __shared__ int lds_node[LDS8 * BLOCK];
     __shared__ float lds_t[LDS8 * BLOCK];
     const uint32_t n = count_ptr ? *count_ptr : count_imm;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    int2 *gstack = spill + (size_t)(blockIdx.x * BLOCK + tid) * (STACK8_CAP - LDS8);
+    int2 *gstack = spill + (size_t)(blockIdx.x * BLOCK + tid) * STACK8_CAP;
     const float4 *__restrict__ wn = sc.wnodes;
     const float4 *__restrict__ wt = sc.wtris;
 
@@ -356,15 +357,25 @@ __global__ __launch_bounds__(BLOCK) void k_trace8(DScene sc, const Ray *__restri
 template <bool ANY>
 int launch(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, uint32_t *occl, const uint32_t *ign,
            const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block, int grid, int *spill,
-           uint32_t *overflow) {
+           uint32_t *overflow, int variant) {
+    if (variant < 0 || variant >= kTraceVariants) return set_error(LT_E_INVALID, "unknown traversal variant");
     if (sc.layout == 0) {
         int2 *sp2 = (int2 *)spill;
+        const int v = variant;
+#define LT_T8(B, L, W) hipLaunchKernelGGL((k_trace8<ANY, B, L, W>), dim3(grid), dim3(B), 0, st, sc, rays, hits, occl, ign, count_ptr, count, work, sp2, overflow)
+#define LT_T8B(B)                          \
+        if (v == 0) LT_T8(B, 16, 0);       \
+        else if (v == 1) LT_T8(B, 8, 0);   \
+        else if (v == 2) LT_T8(B, 8, 6);   \
+        else LT_T8(B, 8, 8);
         switch (block) {
-        case 64: hipLaunchKernelGGL((k_trace8<ANY, 64>), dim3(grid), dim3(64), 0, st, sc, rays, hits, occl, ign, count_ptr, count, work, sp2, overflow); break;
-        case 128: hipLaunchKernelGGL((k_trace8<ANY, 128>), dim3(grid), dim3(128), 0, st, sc, rays, hits, occl, ign, count_ptr, count, work, sp2, overflow); break;
-        case 256: hipLaunchKernelGGL((k_trace8<ANY, 256>), dim3(grid), dim3(256), 0, st, sc, rays, hits, occl, ign, count_ptr, count, work, sp2, overflow); break;
+        case 64: LT_T8B(64); break;
+        case 128: LT_T8B(128); break;
+        case 256: LT_T8B(256); break;
         default: return set_error(LT_E_INVALID, "traversal block size must be 64, 128 or 256");
         }
+#undef LT_T8B
+#undef LT_T8
         LT_HIP(hipGetLastError());
         return LT_OK;
     }
@@ -382,31 +393,27 @@ int launch(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, uint32_
 
 int trace_default_block() { return 128; }
 
-/* Resident blocks: LDS per block is the larger of the two kernels' stacks
- * (binary: LDS_STACK * 4 B, wide: LDS8 * 8 B per thread) of the CU's 160 KB,
- * and at most 32 waves per CU are worth scheduling. */
-int trace_grid(int num_cus, int block) {
-    int waves_per_block = block / 64;
-    int lds_per_thread = std::max(LDS_STACK * 4, LDS8 * 8);
-    int by_lds = (160 * 1024) / (lds_per_thread * block);
-    int by_waves = std::max(1, 32 / waves_per_block);
-    return std::max(1, num_cus) * std::max(1, std::min(by_lds, by_waves));
-}
+/* Enough blocks for 32 waves per CU (the hardware maximum); blocks that
+ * cannot be resident start after others finish and then find the work
+ * counter exhausted. */
+int trace_grid(int num_cus, int block) { return std::max(1, num_cus) * std::max(1, 32 / (block / 64)); }
 
 size_t trace_spill_bytes(int grid, int block) {
-    size_t per = std::max((size_t)(STACK_CAP - LDS_STACK) * sizeof(int), (size_t)(STACK8_CAP - LDS8) * sizeof(int2));
+    size_t per = std::max((size_t)(STACK_CAP - LDS_STACK) * sizeof(int), (size_t)STACK8_CAP * sizeof(int2));
     return (size_t)grid * block * per;
 }
 
 int launch_trace_closest(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, const uint32_t *count_ptr,
-                         uint32_t count, uint32_t *work, int block, int grid, int *spill, uint32_t *overflow) {
-    return launch<false>(st, sc, rays, hits, nullptr, nullptr, count_ptr, count, work, block, grid, spill, overflow);
+                         uint32_t count, uint32_t *work, int block, int grid, int *spill, uint32_t *overflow,
+                         int variant) {
+    return launch<false>(st, sc, rays, hits, nullptr, nullptr, count_ptr, count, work, block, grid, spill, overflow,
+                         variant);
 }
 int launch_trace_any(hipStream_t st, const DScene &sc, const Ray *rays, uint32_t *occluded,
                      const uint32_t *ignore_geom, const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block,
-                     int grid, int *spill, uint32_t *overflow) {
+                     int grid, int *spill, uint32_t *overflow, int variant) {
     return launch<true>(st, sc, rays, nullptr, occluded, ignore_geom, count_ptr, count, work, block, grid, spill,
-                        overflow);
+                        overflow, variant);
 }
 
 } // namespace lt
