@@ -25,15 +25,28 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 RAY_BYTES = 48
 
+WORKLOADS = {
+    "triangle_heavy": {"size": (1280, 720, 64),
+                       "text": "triangle_heavy: displaced icosphere subdiv 7 (327,680 tris) + floor + quad light, "
+                               "GGX a=0.3, env 1.0, RR from depth 3"},
+    "volume_box": {"size": (512, 512, 256),
+                   "text": "volume_box: 128^3 fBm5 density (sigma_t = 5 d, albedo 0.9, HG g=0.3) in an IOR 1.5 "
+                           "CreateBox, floor + quad light, max depth 32, delta tracking"},
+    "nine_sphere": {"size": (512, 512, 16),
+                    "text": "nine_sphere: 9 x geosphere n=4 (46,080 tris) + ground + quad light, Lambert, "
+                            "max depth 8"},
+}
+
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--width", type=int, default=1280)
-    ap.add_argument("--height", type=int, default=720)
-    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--workload", default="triangle_heavy", choices=sorted(WORKLOADS))
+    ap.add_argument("--width", type=int, default=0, help="0: the workload's own size")
+    ap.add_argument("--height", type=int, default=0)
+    ap.add_argument("--spp", type=int, default=0)
     ap.add_argument("--wave-paths", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--layout", type=int, default=0, help="BVH layout: 0 8-wide compressed, 1 binary")
@@ -57,8 +70,12 @@ def main():
     import lantern_amd
     from lantern_amd import scenes
 
+    wl = WORKLOADS[args.workload]
+    args.width = args.width or wl["size"][0]
+    args.height = args.height or wl["size"][1]
+    args.spp = args.spp or wl["size"][2]
     t0 = time.time()
-    desc = scenes.triangle_heavy(args.width, args.height, args.spp)
+    desc = getattr(scenes, args.workload)(args.width, args.height, args.spp)
     gen_s = time.time() - t0
     params = dict(desc.params)
     params["wave_paths"] = args.wave_paths
@@ -115,8 +132,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "triangle_heavy: displaced icosphere subdiv 7 (327,680 tris) + floor + quad light, "
-                               "GGX a=0.3, env 1.0, RR from depth 3",
+        "config": {"workload": wl["text"],
                    "width": args.width, "height": args.height, "spp": args.spp, "max_depth": params["max_depth"],
                    "rr_start": params["rr_start"], "seed": params["seed"], "triangles": info["num_triangles"],
                    "scene_meta": {k: v for k, v in desc.meta.items()}},

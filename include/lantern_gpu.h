@@ -172,6 +172,16 @@ int lt_intersect_any_device(lt_scene *scene, const lt_ray *rays, uint32_t *occlu
 int lt_intersect_closest(lt_scene *scene, const lt_ray *rays, lt_hit *hits, uint32_t n);
 int lt_intersect_any(lt_scene *scene, const lt_ray *rays, uint32_t *occluded, uint32_t n);
 
+/* Transmittance through a grid medium along each ray's [tnear, tfar] (t in
+ * units of the ray's direction vector). method 0: ratio tracking, 1: delta
+ * tracking (0/1 estimates). Ray i draws from Philox(ctr = {i, 0, draw/4,
+ * 0x54524E53}, key = {seed, 0}). The reference has no heterogeneous media;
+ * this is the occlusion query (rtcOccluded1) generalised to attenuation. */
+int lt_medium_transmittance_device(lt_scene *scene, uint32_t medium_id, const lt_ray *rays, float *out, uint32_t n,
+                                   uint32_t seed, int32_t method, void *stream);
+int lt_medium_transmittance(lt_scene *scene, uint32_t medium_id, const lt_ray *rays, float *out, uint32_t n,
+                            uint32_t seed, int32_t method);
+
 /* ---- film ------------------------------------------------------------------
  * FrameBuffer(width, height) (frame_buffer.h:17-34): per pixel an RGB sum
  * (ColorData), a bounce sum (Bounces) and a sample count (ColorSampleCount). */

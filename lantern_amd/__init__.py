@@ -135,6 +135,15 @@ class Scene:
         check(lib().lt_intersect_closest(self.h, rays.ctypes.data, hits.ctypes.data, n), "lt_intersect_closest")
         return hits[:, :3].copy(), hits[:, 3].view(np.uint32).copy()
 
+    def transmittance(self, medium, rays, seed=0, method=0):
+        """Per-ray transmittance through grid medium `medium` over [tnear, tfar]
+        (method 0 ratio tracking, 1 delta tracking). rays: (n, 8) float32."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        out = np.zeros(rays.shape[0], np.float32)
+        check(lib().lt_medium_transmittance(self.h, medium, rays.ctypes.data, out.ctypes.data, rays.shape[0], seed,
+                                            method), "lt_medium_transmittance")
+        return out
+
     def split_prims(self, prims):
         prims = np.ascontiguousarray(prims, dtype=np.uint32)
         g = np.zeros_like(prims)

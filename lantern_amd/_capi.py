@@ -103,6 +103,8 @@ SIGNATURES = {
     "lt_intersect_any_device": (C.c_int, [_vp, _vp, _vp, _u32, _vp]),
     "lt_intersect_closest": (C.c_int, [_vp, _vp, _vp, _u32]),
     "lt_intersect_any": (C.c_int, [_vp, _vp, _vp, _u32]),
+    "lt_medium_transmittance_device": (C.c_int, [_vp, _u32, _vp, _vp, _u32, _u32, C.c_int32, _vp]),
+    "lt_medium_transmittance": (C.c_int, [_vp, _u32, _vp, _vp, _u32, _u32, C.c_int32]),
     "lt_film_create": (C.c_int, [_vp, _u32, _u32, C.POINTER(_vp)]),
     "lt_film_destroy": (C.c_int, [_vp]),
     "lt_film_reset": (C.c_int, [_vp]),

@@ -639,6 +639,34 @@ int lt_intersect_any(lt_scene *s, const lt_ray *rays, uint32_t *occluded, uint32
     return intersect_host(s, rays, occluded, sizeof(uint32_t), n, true);
 }
 
+/* ---- media ------------------------------------------------------------------ */
+int lt_medium_transmittance_device(lt_scene *s, uint32_t medium_id, const lt_ray *rays, float *out, uint32_t n,
+                                   uint32_t seed, int32_t method, void *stream) {
+    LT_CHECK(s && rays && out, "lt_medium_transmittance: NULL argument");
+    if (!s->committed) return set_error(LT_E_STATE, "lt_medium_transmittance: scene is not committed");
+    LT_CHECK(medium_id < s->media.size(), "lt_medium_transmittance: unknown medium");
+    LT_CHECK(s->media[medium_id].type == LT_MEDIUM_GRID, "lt_medium_transmittance: medium is not a grid medium");
+    LT_CHECK(method == 0 || method == 1, "lt_medium_transmittance: method must be 0 (ratio) or 1 (delta)");
+    hipStream_t st = stream ? (hipStream_t)stream : s->dev->stream;
+    return launch_transmittance(st, s->ds, s->media[medium_id], (const Ray *)rays, out, n, seed, method,
+                                s->dev->num_cus);
+}
+int lt_medium_transmittance(lt_scene *s, uint32_t medium_id, const lt_ray *rays, float *out, uint32_t n,
+                            uint32_t seed, int32_t method) {
+    LT_CHECK(s && rays && out, "lt_medium_transmittance: NULL argument");
+    if (n == 0) return LT_OK;
+    hipStream_t st = s->dev->stream;
+    DevBuf dr, dout;
+    int rc;
+    if ((rc = dr.upload(rays, sizeof(lt_ray) * (size_t)n, st)) || (rc = dout.alloc(sizeof(float) * (size_t)n)))
+        return rc;
+    if ((rc = lt_medium_transmittance_device(s, medium_id, dr.as<lt_ray>(), dout.as<float>(), n, seed, method, st)))
+        return rc;
+    LT_HIP(hipMemcpyAsync(out, dout.p, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, st));
+    LT_HIP(hipStreamSynchronize(st));
+    return LT_OK;
+}
+
 /* ---- film ------------------------------------------------------------------- */
 int lt_film_create(lt_device *dev, uint32_t w, uint32_t h, lt_film **out) {
     LT_CHECK(dev && out && w > 0 && h > 0, "lt_film_create: bad argument");

@@ -76,6 +76,11 @@ int launch_trace_any(hipStream_t st, const DScene &sc, const Ray *rays, uint32_t
                      const uint32_t *ignore_geom, const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block,
                      int grid, int *spill, uint32_t *overflow, int variant);
 int trace_grid(int num_cus, int block);
+/* Transmittance of each ray's [tnear, tfar] through grid medium md: method 0
+ * ratio tracking, 1 delta tracking. */
+constexpr uint32_t kTransmittanceStream = 0x54524E53u;
+int launch_transmittance(hipStream_t st, const DScene &sc, const DMedium &md, const Ray *rays, float *out, uint32_t n,
+                         uint32_t seed, int method, int num_cus);
 size_t trace_spill_bytes(int grid, int block);
 int trace_default_block();
 

@@ -231,12 +231,12 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
                     if (md.type == MEDIUM_GRID) {
                         const DGrid g = sc.grids[md.grid];
                         float sbar = md.scale * g.maxd;
-                        float t = 0.0f;
+                        float t = 0.0f, t1 = 0.0f;
                         bool collided = false;
-                        if (sbar > 0.0f) {
-                            while (true) {
+                        if (sbar > 0.0f && grid_span(g, origin, dir, 0.0f, h.t, t, t1)) {
+                            while (true) {                                         /* delta tracking */
                                 t = t - log_f(1.0f - rng.next_float()) / sbar;
-                                if (!(t < h.t)) break;
+                                if (!(t < t1)) break;
                                 V3 qp = origin + dir * t;
                                 float dens = grid_density(sc, g, qp.x, qp.y, qp.z);
                                 if (rng.next_float() * sbar < md.scale * dens) {

@@ -203,6 +203,46 @@ LT_D float grid_density(const DScene &sc, const DGrid &g, float px, float py, fl
     return sc.voxels[g.offset + ((size_t)ix[2] * g.ny + ix[1]) * g.nx + ix[0]];
 }
 
+/* Span [t0, t1) of o + t d inside the grid's box, intersected with
+ * [tmin, tmax]. Density is zero outside the box, so free flights sampled
+ * only over the span have the same distribution as over the whole segment. */
+LT_D bool grid_span(const DGrid &g, const V3 &o, const V3 &d, float tmin, float tmax, float &t0, float &t1) {
+    const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+    t0 = tmin;
+    t1 = tmax;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float lo = g.bmin[a], hi = g.bmax[a];
+        if (dd[a] == 0.0f) {
+            if (!(oo[a] >= lo && oo[a] <= hi)) return false;
+            continue;
+        }
+        const float inv = 1.0f / dd[a];
+        const float ta = (lo - oo[a]) * inv, tb = (hi - oo[a]) * inv;
+        const float n = ta < tb ? ta : tb, f = ta < tb ? tb : ta;
+        t0 = n > t0 ? n : t0;
+        t1 = f < t1 ? f : t1;
+    }
+    return t0 < t1;
+}
+
+/* Ratio tracking (Novak et al. 2014): transmittance over [tmin, tmax] as the
+ * product of (1 - sigma(x)/sigma_bar) at majorant-rate tentative collisions. */
+LT_D float ratio_tracking(const DScene &sc, const DGrid &g, const DMedium &md, const V3 &o, const V3 &d, float tmin,
+                          float tmax, PathRng &rng) {
+    const float sbar = md.scale * g.maxd;
+    float t0, t1;
+    if (!(sbar > 0.0f) || !grid_span(g, o, d, tmin, tmax, t0, t1)) return 1.0f;
+    float tr = 1.0f, t = t0;
+    while (true) {
+        t = t - log_f(1.0f - rng.next_float()) / sbar;
+        if (!(t < t1)) break;
+        const V3 q = o + d * t;
+        tr = tr * (1.0f - (md.scale * grid_density(sc, g, q.x, q.y, q.z)) / sbar);
+    }
+    return tr;
+}
+
 /* Henyey-Greenstein about the propagation direction d (no reference counterpart). */
 LT_D V3 sample_hg(float g, const V3 &d, PathRng &s) {
     float u1 = s.next_float(), u2 = s.next_float();

@@ -279,3 +279,69 @@ def nine_balls_reference(width=128, height=72, spp=4, seed=DEFAULT_SEED, max_dep
     d.params = dict(spp=spp, max_depth=max_depth, rr_start=4, seed=seed)
     d.meta = dict(workload="nine_balls_reference", seed=seed)
     return d
+
+
+def fbm_grid(res, seed, octaves=5, freq=3.0):
+    """(res, res, res) float32 grid of clamp(fBm(p), 0, 1) sampled at voxel
+    centres of the unit cube, z-major (index [z, y, x])."""
+    noise = GradientNoise(seed)
+    c = (np.arange(res, dtype=np.float64) + 0.5) / res
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    pts = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    out = np.empty(len(pts))
+    step = 1 << 18
+    for s in range(0, len(pts), step):
+        out[s:s + step] = fbm(noise, pts[s:s + step], octaves, freq)
+    return np.clip(out, 0.0, 1.0).astype(np.float32).reshape(res, res, res)
+
+
+def volume_box(width=512, height=512, spp=256, seed=DEFAULT_SEED, res=128, max_depth=32, sigma_scale=5.0):
+    """Heterogeneous smoke in a glass box: a 128^3 grid of clamp(fBm_5, 0, 1)
+    densities (sigma_t = 5 density, single-scatter albedo 0.9, Henyey-Greenstein
+    g = 0.3) filling a 2x2x2 CreateBox of IOR 1.5 standing on a Lambertian
+    floor, one 2x2 quad light (radiance 20) at y = 5, dim environment 0.2.
+    Free flights by delta tracking."""
+    d = SceneDesc()
+    dens = fbm_grid(res, seed)
+    gid = len(d.grids)
+    d.grids.append(dict(density=dens, bmin=(-1.0, 0.0, -1.0), bmax=(1.0, 2.0, 1.0)))
+    med = len(d.media)
+    d.media.append(dict(type=_capi.LT_MEDIUM_GRID, grid=gid, sigma_t_scale=sigma_scale, albedo=0.9, hg_g=0.3))
+    glass = d.add_material(bsdf=_capi.LT_BSDF_DIELECTRIC, albedo=(1.0, 1.0, 1.0), ior=1.5, medium=med)
+    floor = d.add_material(bsdf=_capi.LT_BSDF_LAMBERT, albedo=(0.8, 0.8, 0.8))
+    light_mat = d.add_material(bsdf=_capi.LT_BSDF_LAMBERT, albedo=(0.0, 0.0, 0.0))
+    bp, bn, buv, bi = box(2.0, 2.0, 2.0)
+    bp2, bn2 = transform(bp, bn, translate(0.0, 1.0, 0.0))
+    d.add_mesh(bp2, bi, bn2, buv, glass)
+    gp, gn, guv, gi = grid(20.0, 20.0, 2, 2)
+    d.add_mesh(gp, gi, gn, guv, floor)
+    lp, ln, luv, li, power = quad_light(2.0, 5.0, 20.0)
+    d.add_mesh(lp, li, ln, luv, light_mat, emission=((1.0, 1.0, 1.0), power))
+    d.background = (0.2, 0.2, 0.2)
+    d.camera = dict(phi=1.2, theta=0.6, radius=7.0, width=width, height=height, target=(0.0, 1.0, 0.0),
+                    fov=math.pi / 4, filter=0)
+    d.params = dict(spp=spp, max_depth=max_depth, rr_start=4, seed=seed)
+    d.meta = dict(workload="volume_box", seed=seed, grid_res=res, fbm_octaves=5, fbm_freq=3.0,
+                  sigma_t_scale=sigma_scale, single_scatter_albedo=0.9, hg_g=0.3, box_ior=1.5, light_height=5.0,
+                  light_le=20.0, background=0.2, floor_albedo=0.8, camera_phi=1.2, camera_theta=0.6,
+                  camera_radius=7.0)
+    return d
+
+
+def furnace_plane(bsdf=_capi.LT_BSDF_LAMBERT, albedo=(0.5, 0.5, 0.5), background=(1.0, 1.0, 1.0), ior=1.0,
+                  roughness=0.0, width=32, height=32, spp=4, max_depth=8):
+    """A 20x20 plane of one material under a constant environment with no
+    lights, seen from above: every path that reaches the plane leaves it
+    upward (or downward through a dielectric) and escapes, so each sample is
+    exactly albedo * background for the plane's pixels (Eval/Pdf = albedo for
+    the reference's BSDFs)."""
+    d = SceneDesc()
+    m = d.add_material(bsdf=bsdf, albedo=tuple(albedo), ior=ior, roughness=roughness)
+    gp, gn, guv, gi = grid(20.0, 20.0, 2, 2)
+    d.add_mesh(gp, gi, gn, guv, m)
+    d.background = tuple(background)
+    d.camera = dict(phi=0.3, theta=0.4, radius=4.0, width=width, height=height, target=(0.0, 0.0, 0.0),
+                    fov=math.pi / 4, filter=0)
+    d.params = dict(spp=spp, max_depth=max_depth, rr_start=max_depth, seed=DEFAULT_SEED)
+    d.meta = dict(workload="furnace_plane")
+    return d

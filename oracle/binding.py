@@ -88,6 +88,11 @@ def lib():
                                             _pu, _pu, C.POINTER(C.c_uint64)]),
             "orc_trace_path_f32": (C.c_uint32, [C.c_void_p, C.POINTER(orc_camera_desc), C.c_uint32, C.c_uint32,
                                                 C.POINTER(orc_render_params), _pf]),
+            "orc_transmittance": (None, [C.c_void_p, C.c_uint32, _pf, C.c_uint32, C.c_uint32, C.c_int32, _pf]),
+            "orc_fresnel": (None, [C.c_float, C.c_float, C.c_float, _pf]),
+            "orc_sample_hg": (None, [C.c_float, _pf, C.c_float, C.c_float, _pf]),
+            "orc_bsdf_sample": (C.c_int32, [C.POINTER(orc_material_desc), _pf, _pf, C.c_float, _pf, C.c_uint32, _pf]),
+            "orc_bsdf_eval": (None, [C.POINTER(orc_material_desc), _pf, _pf, _pf, _pf]),
             "orc_hardware_threads": (C.c_uint32, []),
         }
         for name, (res, args) in sig.items():
@@ -190,6 +195,12 @@ class OracleScene:
         lib().orc_intersect(self.h, fp(rays), n, 1 if use_bvh else 0, fp(tuv), up(geom), up(prim), up(tie))
         return (tuv, geom, prim, tie) if ties else (tuv, geom, prim)
 
+    def transmittance(self, medium, rays, seed=0, method=0):
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros(rays.shape[0], np.float32)
+        lib().orc_transmittance(self.h, medium, fp(rays), rays.shape[0], seed, method, fp(out))
+        return out
+
     def render(self, cam, params, precision=32, rng=ORC_RNG_PHILOX, threads=0, window=None, frame0=0, stream=0):
         """Returns (sum (h,w,3), bounces (h,w), count (h,w), seconds, rays)."""
         L = lib()
@@ -218,3 +229,39 @@ class OracleScene:
             s = np.zeros((h, w, 3), np.float32)
             sec = L.orc_render_f32(self.h, C.byref(camera(cam)), C.byref(p), fp(s), up(b), up(c), C.byref(rays))
         return s, b, c, sec, rays.value
+
+
+def _f3(v):
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+def material(bsdf, albedo=(1.0, 1.0, 1.0), ior=1.0, roughness=0.0):
+    return orc_material_desc(bsdf, _f3(albedo), -1, ior, roughness, -1)
+
+
+def fresnel(ni, no, cos_i):
+    out = (C.c_float * 1)()
+    lib().orc_fresnel(ni, no, cos_i, out)
+    return out[0]
+
+
+def sample_hg(g, d, u1, u2):
+    out = (C.c_float * 3)()
+    lib().orc_sample_hg(g, _f3(d), u1, u2, out)
+    return np.array(out[:], np.float32)
+
+
+def bsdf_sample(mat, n, wo, u, iori=1.0):
+    """Returns (lobe, wi, eval rgb, pdf, ioro)."""
+    u = np.ascontiguousarray(u, np.float32)
+    out = (C.c_float * 8)()
+    lobe = lib().orc_bsdf_sample(C.byref(mat), _f3(n), _f3(wo), iori, fp(u), u.size, out)
+    o = np.array(out[:], np.float32)
+    return lobe, o[0:3], o[3:6], o[6], o[7]
+
+
+def bsdf_eval(mat, n, wo, wi):
+    out = (C.c_float * 4)()
+    lib().orc_bsdf_eval(C.byref(mat), _f3(n), _f3(wo), _f3(wi), out)
+    o = np.array(out[:], np.float32)
+    return o[0:3], o[3]

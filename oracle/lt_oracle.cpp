@@ -247,13 +247,21 @@ struct PhiloxSampler {
     }
 };
 
+/* Caller-supplied uniforms (unit checks of single sampling routines). */
+struct ListSampler {
+    const float *u;
+    u32 n, i;
+    float next() { return i < n ? u[i++] : 0.0f; }
+};
+
 /* Type-erased sampler interface used by the estimator. */
 struct Sampler {
     int kind;
     PcgSampler *pcg;
     PhiloxSampler *phx;
+    ListSampler *list = nullptr;
     u32 next_u32() { return kind == 0 ? pcg->next_u32() : phx->next_u32(); }
-    float next_float() { return bits_to_unit(next_u32()); }                  /* uniform_sampler.h:45-49 */
+    float next_float() { return kind == 2 ? list->next() : bits_to_unit(next_u32()); } /* uniform_sampler.h:45-49 */
     u32 next_discrete(u32 range) { return (u32)((float)range * next_float()); } /* :55-57 */
 };
 
@@ -1132,6 +1140,49 @@ inline float grid_density(const Grid &g, float px, float py, float pz) {
     return g.d[((size_t)ix[2] * g.ny + ix[1]) * g.nx + ix[0]];
 }
 
+/* Span [t0, t1) of the ray o + t d inside the grid's box, intersected with
+ * [tmin, tmax] (no reference counterpart). Density is zero outside the box,
+ * so free-flight sampling restricted to the span has the same distribution as
+ * over the whole segment. An axis with d == 0 constrains nothing when o lies
+ * within the slab and empties the span otherwise. */
+template <class R> inline bool grid_span(const Grid &g, const V3<R> &o, const V3<R> &d, R tmin, R tmax, R &t0, R &t1) {
+    R oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+    t0 = tmin;
+    t1 = tmax;
+    for (int a = 0; a < 3; ++a) {
+        R lo = (R)g.bmin[a], hi = (R)g.bmax[a];
+        if (dd[a] == R(0)) {
+            if (!(oo[a] >= lo && oo[a] <= hi)) return false;
+            continue;
+        }
+        R inv = R(1) / dd[a];
+        R ta = (lo - oo[a]) * inv, tb = (hi - oo[a]) * inv;
+        R n = ta < tb ? ta : tb, f = ta < tb ? tb : ta;
+        t0 = n > t0 ? n : t0;
+        t1 = f < t1 ? f : t1;
+    }
+    return t0 < t1;
+}
+
+/* Ratio tracking (Novak et al. 2014) of the transmittance over [tmin, tmax]
+ * of a grid medium: product of (1 - sigma(x)/sigma_bar) at majorant-rate
+ * tentative collisions. No reference counterpart. */
+template <class R> inline R ratio_tracking(const Grid &g, const Medium &md, const V3<R> &o, const V3<R> &d, R tmin,
+                                           R tmax, Sampler &smp) {
+    R sbar = (R)md.scale * (R)g.maxd;
+    R t0, t1;
+    if (!(sbar > R(0)) || !grid_span<R>(g, o, d, tmin, tmax, t0, t1)) return R(1);
+    R tr = R(1), t = t0;
+    while (true) {
+        t = t - MathK<R>::log(R(1) - uni<R>(smp)) / sbar;
+        if (!(t < t1)) break;
+        V3<R> q = o + d * t;
+        R dens = (R)grid_density(g, (float)q.x, (float)q.y, (float)q.z);
+        tr = tr * (R(1) - ((R)md.scale * dens) / sbar);
+    }
+    return tr;
+}
+
 /* Henyey-Greenstein direction about the propagation direction d (no reference
  * counterpart): cos = (1 + g^2 - ((1-g^2)/(1-g+2gU))^2)/(2g). */
 template <class R> inline V3<R> sample_hg(R g, const V3<R> &d, Sampler &s) {
@@ -1179,12 +1230,12 @@ u32 render_pixel(const orc_scene &s, const Camera &cam, u32 px, u32 py, Sampler 
                 /* delta tracking through the majorant (no reference counterpart) */
                 const Grid &g = s.grids[md.grid];
                 R sbar = (R)md.scale * (R)g.maxd;
-                R t = 0;
+                R t = 0, t1 = 0;
                 bool collided = false;
-                if (sbar > R(0)) {
+                if (sbar > R(0) && grid_span<R>(g, origin, dir, R(0), h.t, t, t1)) {
                     while (true) {
                         t = t - MathK<R>::log(R(1) - uni<R>(smp)) / sbar;
-                        if (!(t < h.t)) break;
+                        if (!(t < t1)) break;
                         V3<R> q = origin + dir * t;
                         R dens = (R)grid_density(g, (float)q.x, (float)q.y, (float)q.z);
                         if (uni<R>(smp) * sbar < (R)md.scale * dens) {
@@ -1583,6 +1634,85 @@ uint32_t orc_trace_path_f32(const orc_scene *s, const orc_camera_desc *cd, uint3
     _mm_setcsr(csr);
     rgb[0] = c.x; rgb[1] = c.y; rgb[2] = c.z;
     return b;
+}
+
+void orc_transmittance(const orc_scene *s, uint32_t medium, const float *rays, uint32_t n, uint32_t seed,
+                       int32_t method, float *out) {
+    const Medium &md = s->media[medium];
+    const Grid &g = s->grids[md.grid];
+    unsigned int csr = _mm_getcsr();
+    _mm_setcsr(csr | 0x8040);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float *r = rays + 8 * (size_t)i;
+        PcgSampler pcg(0, 0);
+        PhiloxSampler phx(seed, 0, i, 0, ORC_TRANSMITTANCE_STREAM);
+        Sampler smp{1, &pcg, &phx};
+        V3<float> o(r[0], r[1], r[2]), d(r[4], r[5], r[6]);
+        if (method == 0) {
+            out[i] = ratio_tracking<float>(g, md, o, d, r[3], r[7], smp);
+        } else {
+            /* delta tracking: 1 if no real collision happens in the span */
+            float sbar = md.scale * g.maxd, t0, t1;
+            float tr = 1.0f;
+            if (sbar > 0.0f && grid_span<float>(g, o, d, r[3], r[7], t0, t1)) {
+                float t = t0;
+                while (true) {
+                    t = t - MathK<float>::log(1.0f - uni<float>(smp)) / sbar;
+                    if (!(t < t1)) break;
+                    V3<float> q = o + d * t;
+                    if (uni<float>(smp) * sbar < md.scale * grid_density(g, q.x, q.y, q.z)) {
+                        tr = 0.0f;
+                        break;
+                    }
+                }
+            }
+            out[i] = tr;
+        }
+    }
+    _mm_setcsr(csr);
+}
+
+void orc_fresnel(float ni, float no, float cos_i, float *out) {
+    out[0] = fresnel<float>(ni, no, cos_i, sin2_t<float>(cos_i, ni / no));
+}
+
+void orc_sample_hg(float g, const float d[3], float u1, float u2, float out[3]) {
+    float u[2] = {u1, u2};
+    ListSampler ls{u, 2, 0};
+    Sampler smp{2, nullptr, nullptr, &ls};
+    V3<float> w = sample_hg<float>(g, V3<float>(d[0], d[1], d[2]), smp);
+    out[0] = w.x; out[1] = w.y; out[2] = w.z;
+}
+
+int32_t orc_bsdf_sample(const orc_material_desc *m, const float n[3], const float wo[3], float iori, const float *u,
+                        uint32_t nu, float out[8]) {
+    static const orc_scene empty;
+    ListSampler ls{u, nu, 0};
+    Sampler smp{2, nullptr, nullptr, &ls};
+    Interaction<float> it;
+    it.N = V3<float>(n[0], n[1], n[2]);
+    it.wo = V3<float>(wo[0], wo[1], wo[2]);
+    it.iori = iori;
+    it.tu = it.tv = 0.0f;
+    bsdf_sample<float>(*m, it, smp);
+    V3<float> f = bsdf_eval<float>(empty, *m, it);
+    out[0] = it.wi.x; out[1] = it.wi.y; out[2] = it.wi.z;
+    out[3] = f.x; out[4] = f.y; out[5] = f.z;
+    out[6] = bsdf_pdf<float>(*m, it);
+    out[7] = it.ioro;
+    return it.lobe;
+}
+
+void orc_bsdf_eval(const orc_material_desc *m, const float n[3], const float wo[3], const float wi[3], float out[4]) {
+    static const orc_scene empty;
+    Interaction<float> it;
+    it.N = V3<float>(n[0], n[1], n[2]);
+    it.wo = V3<float>(wo[0], wo[1], wo[2]);
+    it.wi = V3<float>(wi[0], wi[1], wi[2]);
+    it.tu = it.tv = 0.0f;
+    V3<float> f = bsdf_eval<float>(empty, *m, it);
+    out[0] = f.x; out[1] = f.y; out[2] = f.z;
+    out[3] = bsdf_pdf<float>(*m, it);
 }
 
 uint32_t orc_hardware_threads(void) { return std::max(1u, std::thread::hardware_concurrency()); }

@@ -162,6 +162,28 @@ double orc_render_f64(const orc_scene *s, const orc_camera_desc *cam, const orc_
 uint32_t orc_trace_path_f32(const orc_scene *s, const orc_camera_desc *cam, uint32_t px,
                             uint32_t py, const orc_render_params *p, float *rgb);
 
+/* Transmittance of a grid medium along rays [tnear, tfar] (rays n*8 floats,
+ * directions need not be unit: t is in ray-parameter units). method 0: ratio
+ * tracking (Novak et al. 2014), 1: delta tracking (0/1 estimate). The
+ * uniforms of ray i come from Philox(ctr={i, 0, draw/4,
+ * ORC_TRANSMITTANCE_STREAM}, key={seed, 0}). No reference counterpart. */
+#define ORC_TRANSMITTANCE_STREAM 0x54524E53u
+void orc_transmittance(const orc_scene *s, uint32_t medium, const float *rays, uint32_t n, uint32_t seed,
+                       int32_t method, float *out);
+
+/* Single routines with caller-supplied uniforms, for closed-form checks. */
+/* Fresnel (vector_math.h:45-56) at cos_i with IORs ni (incident) and no. */
+void orc_fresnel(float ni, float no, float cos_i, float *out);
+/* Henyey-Greenstein sampling about direction d (no reference counterpart). */
+void orc_sample_hg(float g, const float d[3], float u1, float u2, float out[3]);
+/* BSDF::Sample then Eval and Pdf for a constant-albedo material
+ * (lambert_bsdf.h, mirror_bsdf.h, ideal_specular_dielectric.h; GGX new).
+ * out: wi xyz, eval rgb, pdf, ioro. Returns the sampled lobe bits. */
+int32_t orc_bsdf_sample(const orc_material_desc *m, const float n[3], const float wo[3], float iori, const float *u,
+                        uint32_t nu, float out[8]);
+/* BSDF::Eval and Pdf; out: eval rgb, pdf. */
+void orc_bsdf_eval(const orc_material_desc *m, const float n[3], const float wo[3], const float wi[3], float out[4]);
+
 uint32_t orc_hardware_threads(void);
 
 #ifdef __cplusplus

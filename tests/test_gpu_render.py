@@ -86,3 +86,26 @@ def test_frames_accumulate_like_render_frame(gpu_device):
     for f in (f1, f2, f3):
         f.close()
     gs.close()
+
+
+@pytest.mark.parametrize("bsdf,albedo", [(0, (0.3, 0.5, 0.7)), (1, (0.9, 0.8, 0.6))])
+def test_furnace_plane_is_albedo_times_background(gpu_device, bsdf, albedo):
+    # closed form: every sample of a Lambert or mirror plane under a constant
+    # environment with no lights is albedo * background after one bounce
+    bgc = (2.0, 1.0, 0.5)
+    d = scenes.furnace_plane(bsdf, albedo, bgc)
+    gs = lantern_amd.Scene(gpu_device, d)
+    film = lantern_amd.Film(gpu_device, d.camera["width"], d.camera["height"])
+    lantern_amd.render(gs, d.camera, dict(d.params, spp=4), film)
+    s, b, c = film.download()
+    np.testing.assert_allclose(s / c[..., None], np.broadcast_to(np.multiply(albedo, bgc), s.shape), rtol=2e-6)
+    assert np.all(b == 4) and np.all(c == 4)
+    film.close()
+    gs.close()
+
+
+def test_volume_box_small_matches_oracle(gpu_device):
+    # 32^3 fBm smoke in the IOR 1.5 glass box: delta tracking + HG scattering
+    d = scenes.volume_box(64, 64, 2, res=32, max_depth=32)
+    g, o, stats, rays = _render_both(gpu_device, d)
+    _check(g, o, frac=0.995)
