@@ -249,6 +249,34 @@ int lt_render(lt_scene *scene, const lt_camera *camera, const lt_render_params *
 int lt_render_batch(lt_scene *const *scenes, const lt_camera *cameras, lt_film *const *films, uint32_t n,
                     const lt_render_params *params, lt_render_stats *stats);
 
+/* ---- differentiable rendering ------------------------------------------------
+ * No counterpart in the reference (its renderer is not differentiable). The
+ * parameters are the image-texture texels and grid-medium densities. The
+ * adjoint is path replay (the forward pass records each sample's radiance;
+ * the replay walks every path again with the same random numbers). Texture
+ * gradients cover Lambert, mirror and dielectric materials (their Eval/Pdf is
+ * linear in the albedo); density gradients are the delta-tracking score
+ * terms. Russian roulette must be off (rr_start >= max_depth). */
+enum { LT_PARAM_TEXTURE = 0, LT_PARAM_GRID = 1, LT_PARAM_TEXEL_POOL = 2, LT_PARAM_VOXEL_POOL = 3 };
+/* Where texture/grid `id` lives in the texel (floats) or voxel pool, and the
+ * pool sizes (kind LT_PARAM_*_POOL). Gradient buffers use the pool layout. */
+int lt_scene_param_range(lt_scene *scene, int32_t kind, uint32_t id, uint32_t *offset, uint32_t *count);
+/* Replace a texture's texels / a grid's densities (same size). on_device: the
+ * pointer is device memory (copied on the library's stream, no host sync);
+ * densities are clamped to >= 0 and the majorant is recomputed. */
+int lt_scene_update_texture(lt_scene *scene, uint32_t id, const float *rgb, int32_t on_device);
+int lt_scene_update_grid(lt_scene *scene, uint32_t id, const float *density, int32_t on_device);
+/* lt_render that also writes each sample's radiance to sample_radiance
+ * (device, 3 floats per sample, index (frame - frame0) * W * H + y * W + x). */
+int lt_render_record(lt_scene *scene, const lt_camera *camera, const lt_render_params *params, lt_film *film,
+                     float *sample_radiance, lt_render_stats *stats);
+/* Adjoint of the recorded render: pixel_grad (device, W*H*3) is dLoss/d(pixel
+ * mean radiance); adds dLoss/dparameter into grad_texels / grad_voxels
+ * (device, pool layout). Same scene, camera and params as the recording. */
+int lt_adjoint(lt_scene *scene, const lt_camera *camera, const lt_render_params *params,
+               const float *sample_radiance, const float *pixel_grad, float *grad_texels, float *grad_voxels,
+               lt_render_stats *stats);
+
 /* Host-side generators used by the scene loader (geometry_generator.cpp).
  * Pass NULL buffers to query sizes: returns the vertex count, *num_tris gets
  * the triangle count. Positions/normals nv*3 floats, uvs nv*2. */

@@ -16,6 +16,7 @@ LT_INVALID_ID = 0xFFFFFFFF
 LT_BSDF_LAMBERT, LT_BSDF_MIRROR, LT_BSDF_DIELECTRIC, LT_BSDF_GGX = 0, 1, 2, 3
 LT_MEDIUM_NON_SCATTERING, LT_MEDIUM_ISOTROPIC, LT_MEDIUM_GRID = 0, 1, 2
 LT_TEX_CONSTANT, LT_TEX_UV = -1, -2
+LT_PARAM_TEXTURE, LT_PARAM_GRID, LT_PARAM_TEXEL_POOL, LT_PARAM_VOXEL_POOL = 0, 1, 2, 3
 
 
 class lt_material(C.Structure):
@@ -105,6 +106,13 @@ SIGNATURES = {
     "lt_intersect_any": (C.c_int, [_vp, _vp, _vp, _u32]),
     "lt_medium_transmittance_device": (C.c_int, [_vp, _u32, _vp, _vp, _u32, _u32, C.c_int32, _vp]),
     "lt_medium_transmittance": (C.c_int, [_vp, _u32, _vp, _vp, _u32, _u32, C.c_int32]),
+    "lt_scene_param_range": (C.c_int, [_vp, C.c_int32, _u32, C.POINTER(_u32), C.POINTER(_u32)]),
+    "lt_scene_update_texture": (C.c_int, [_vp, _u32, _vp, C.c_int32]),
+    "lt_scene_update_grid": (C.c_int, [_vp, _u32, _vp, C.c_int32]),
+    "lt_render_record": (C.c_int, [_vp, C.POINTER(lt_camera), C.POINTER(lt_render_params), _vp, _vp,
+                                   C.POINTER(lt_render_stats)]),
+    "lt_adjoint": (C.c_int, [_vp, C.POINTER(lt_camera), C.POINTER(lt_render_params), _vp, _vp, _vp, _vp,
+                             C.POINTER(lt_render_stats)]),
     "lt_film_create": (C.c_int, [_vp, _u32, _u32, C.POINTER(_vp)]),
     "lt_film_destroy": (C.c_int, [_vp]),
     "lt_film_reset": (C.c_int, [_vp]),

@@ -15,6 +15,10 @@ namespace lt {
 
 int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &prm, lt_film *film,
                  lt_render_stats *stats);
+int render_record(lt_scene *scene, const lt_camera &cam, const lt_render_params &prm, lt_film *film,
+                  float *sample_radiance, lt_render_stats *stats);
+int render_adjoint(lt_scene *scene, const lt_camera &cam, const lt_render_params &prm, const float *sample_radiance,
+                   const float *pixel_grad, float *grad_texels, float *grad_voxels, lt_render_stats *stats);
 
 namespace {
 thread_local std::string g_error = "no error";
@@ -498,6 +502,20 @@ int lt_scene_commit(lt_scene *s, const lt_build_params *params) {
     hipStream_t st = s->dev->stream;
     LT_HIP(hipSetDevice(s->dev->ordinal));
     int rc;
+    if (s->params_on_device) {
+        /* parameters changed on the device since the last commit: they are the current values */
+        if (s->d_texels.p && s->d_texels.bytes >= s->texels.size() * 4 && !s->texels.empty())
+            LT_HIP(hipMemcpyAsync(s->texels.data(), s->d_texels.p, s->texels.size() * 4, hipMemcpyDeviceToHost, st));
+        if (s->d_voxels.p && s->d_voxels.bytes >= s->voxels.size() * 4 && !s->voxels.empty())
+            LT_HIP(hipMemcpyAsync(s->voxels.data(), s->d_voxels.p, s->voxels.size() * 4, hipMemcpyDeviceToHost, st));
+        LT_HIP(hipStreamSynchronize(st));
+        for (DGrid &g : s->grids) {
+            float mx = 0.0f;
+            for (size_t i = 0, n = (size_t)g.nx * g.ny * g.nz; i < n; ++i) mx = std::max(mx, s->voxels[g.offset + i]);
+            g.maxd = mx;
+        }
+        s->params_on_device = false;
+    }
     if ((rc = s->d_pos.upload(s->pos.data(), s->pos.size() * 4, st)) || (rc = s->d_nrm.upload(s->nrm.data(), s->nrm.size() * 4, st)) ||
         (rc = s->d_uv.upload(s->uv.data(), s->uv.size() * 4, st)) || (rc = s->d_vidx.upload(s->vidx.data(), s->vidx.size() * 4, st)) ||
         (rc = s->d_tri_geom.upload(s->tri_geom.data(), s->tri_geom.size() * 4, st)) ||
@@ -601,7 +619,7 @@ static int intersect_device(lt_scene *s, const lt_ray *rays, void *out, uint32_t
     lt_device *d = s->dev;
     hipStream_t st = stream ? (hipStream_t)stream : d->stream;
     int block = trace_default_block();
-    int grid = trace_grid(d->num_cus, block);
+    int grid = trace_grid(d->num_cus, block, kDefaultTraceVariant);
     int rc = d->spill.reserve(trace_spill_bytes(grid, block));
     if (rc) return rc;
     LT_HIP(hipMemsetAsync(d->trace_work.p, 0, sizeof(uint32_t), st));
@@ -779,6 +797,102 @@ int lt_render_batch(lt_scene *const *scenes, const lt_camera *cams, lt_film *con
     }
     if (stats) *stats = acc;
     return LT_OK;
+}
+
+/* ---- differentiable parameters and the adjoint --------------------------------- */
+int lt_scene_param_range(lt_scene *s, int32_t kind, uint32_t id, uint32_t *offset, uint32_t *count) {
+    LT_CHECK(s && offset && count, "lt_scene_param_range: NULL argument");
+    if (kind == LT_PARAM_TEXTURE) {
+        LT_CHECK(id < s->texs.size(), "lt_scene_param_range: unknown texture");
+        *offset = s->texs[id].offset;
+        *count = s->texs[id].w * s->texs[id].h * 3;
+    } else if (kind == LT_PARAM_GRID) {
+        LT_CHECK(id < s->grids.size(), "lt_scene_param_range: unknown grid");
+        *offset = s->grids[id].offset;
+        *count = s->grids[id].nx * s->grids[id].ny * s->grids[id].nz;
+    } else if (kind == LT_PARAM_TEXEL_POOL) {
+        *offset = 0;
+        *count = (uint32_t)s->texels.size();
+    } else if (kind == LT_PARAM_VOXEL_POOL) {
+        *offset = 0;
+        *count = (uint32_t)s->voxels.size();
+    } else {
+        return set_error(LT_E_INVALID, "lt_scene_param_range: unknown kind");
+    }
+    return LT_OK;
+}
+
+int lt_scene_update_texture(lt_scene *s, uint32_t id, const float *rgb, int32_t on_device) {
+    LT_CHECK(s && rgb, "lt_scene_update_texture: NULL argument");
+    LT_CHECK(id < s->texs.size(), "lt_scene_update_texture: unknown texture");
+    const DTexture &t = s->texs[id];
+    const size_t n = (size_t)t.w * t.h * 3;
+    if (!on_device) {
+        std::memcpy(s->texels.data() + t.offset, rgb, n * 4);
+        if (!s->committed) return LT_OK;
+        LT_HIP(hipMemcpyAsync(s->d_texels.as<float>() + t.offset, rgb, n * 4, hipMemcpyHostToDevice, s->dev->stream));
+        LT_HIP(hipStreamSynchronize(s->dev->stream));
+        return LT_OK;
+    }
+    if (!s->committed) return set_error(LT_E_STATE, "lt_scene_update_texture: device update needs a committed scene");
+    LT_HIP(hipMemcpyAsync(s->d_texels.as<float>() + t.offset, rgb, n * 4, hipMemcpyDeviceToDevice, s->dev->stream));
+    s->params_on_device = true;
+    return LT_OK;
+}
+
+int lt_scene_update_grid(lt_scene *s, uint32_t id, const float *density, int32_t on_device) {
+    LT_CHECK(s && density, "lt_scene_update_grid: NULL argument");
+    LT_CHECK(id < s->grids.size(), "lt_scene_update_grid: unknown grid");
+    DGrid &g = s->grids[id];
+    const size_t n = (size_t)g.nx * g.ny * g.nz;
+    hipStream_t st = s->dev->stream;
+    if (!on_device) {
+        float mx = 0.0f;
+        for (size_t i = 0; i < n; ++i) {
+            float d = density[i] >= 0.0f ? density[i] : 0.0f;
+            s->voxels[g.offset + i] = d;
+            mx = std::max(mx, d);
+        }
+        g.maxd = mx;
+        if (!s->committed) return LT_OK;
+        LT_HIP(hipMemcpyAsync(s->d_voxels.as<float>() + g.offset, s->voxels.data() + g.offset, n * 4,
+                              hipMemcpyHostToDevice, st));
+        LT_HIP(hipMemcpyAsync(s->d_grids.as<DGrid>() + id, &g, sizeof(DGrid), hipMemcpyHostToDevice, st));
+        LT_HIP(hipStreamSynchronize(st));
+        return LT_OK;
+    }
+    if (!s->committed) return set_error(LT_E_STATE, "lt_scene_update_grid: device update needs a committed scene");
+    float *dv = s->d_voxels.as<float>() + g.offset;
+    LT_HIP(hipMemcpyAsync(dv, density, n * 4, hipMemcpyDeviceToDevice, st));
+    s->params_on_device = true;
+    return launch_grid_refresh(st, dv, n, &(s->d_grids.as<DGrid>() + id)->maxd, s->dev->num_cus);
+}
+
+static int check_render_args(lt_scene *s, const lt_camera *cam, const lt_render_params *prm, const char *who) {
+    if (!s || !cam || !prm) return set_error(LT_E_INVALID, std::string(who) + ": NULL argument");
+    if (!s->committed) return set_error(LT_E_STATE, std::string(who) + ": scene is not committed");
+    if (cam->filter < 0 || cam->filter > 2) return set_error(LT_E_INVALID, std::string(who) + ": unknown filter");
+    LT_HIP(hipSetDevice(s->dev->ordinal));
+    return LT_OK;
+}
+
+int lt_render_record(lt_scene *s, const lt_camera *cam, const lt_render_params *prm, lt_film *film,
+                     float *sample_radiance, lt_render_stats *stats) {
+    int rc = check_render_args(s, cam, prm, "lt_render_record");
+    if (rc) return rc;
+    LT_CHECK(film && sample_radiance, "lt_render_record: NULL argument");
+    LT_CHECK(film->w == cam->width && film->h == cam->height, "lt_render_record: film and camera sizes differ");
+    return render_record(s, *cam, *prm, film, sample_radiance, stats);
+}
+
+int lt_adjoint(lt_scene *s, const lt_camera *cam, const lt_render_params *prm, const float *sample_radiance,
+               const float *pixel_grad, float *grad_texels, float *grad_voxels, lt_render_stats *stats) {
+    int rc = check_render_args(s, cam, prm, "lt_adjoint");
+    if (rc) return rc;
+    LT_CHECK(sample_radiance && pixel_grad, "lt_adjoint: NULL argument");
+    LT_CHECK(grad_texels || s->texels.empty(), "lt_adjoint: the scene has textures but grad_texels is NULL");
+    LT_CHECK(grad_voxels || s->voxels.empty(), "lt_adjoint: the scene has grids but grad_voxels is NULL");
+    return render_adjoint(s, *cam, *prm, sample_radiance, pixel_grad, grad_texels, grad_voxels, stats);
 }
 
 /* ---- generators ---------------------------------------------------------------- */

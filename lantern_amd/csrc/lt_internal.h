@@ -75,12 +75,15 @@ int launch_trace_closest(hipStream_t st, const DScene &sc, const Ray *rays, Hit 
 int launch_trace_any(hipStream_t st, const DScene &sc, const Ray *rays, uint32_t *occluded,
                      const uint32_t *ignore_geom, const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block,
                      int grid, int *spill, uint32_t *overflow, int variant);
-int trace_grid(int num_cus, int block);
+int trace_grid(int num_cus, int block, int variant);
 /* Transmittance of each ray's [tnear, tfar] through grid medium md: method 0
  * ratio tracking, 1 delta tracking. */
 constexpr uint32_t kTransmittanceStream = 0x54524E53u;
 int launch_transmittance(hipStream_t st, const DScene &sc, const DMedium &md, const Ray *rays, float *out, uint32_t n,
                          uint32_t seed, int method, int num_cus);
+/* After a device-side density update: clamp to >= 0 and recompute the
+ * grid's majorant into *maxd (a field of the device grid table). */
+int launch_grid_refresh(hipStream_t st, float *voxels, size_t n, float *maxd, int num_cus);
 size_t trace_spill_bytes(int grid, int block);
 int trace_default_block();
 
@@ -136,6 +139,7 @@ struct lt_scene {
     std::vector<lt::DLight> lights;
     float bg[3] = {0.0f, 0.0f, 0.0f};
     bool committed = false;
+    bool params_on_device = false;   /* texels/voxels updated on the device after the last commit */
     /* device copies */
     std::vector<float> light_tris;    /* 12 floats per light triangle (v0|prim, e1, e2) */
     lt::DevBuf d_pos, d_nrm, d_uv, d_vidx, d_tri_geom, d_geoms, d_mats, d_media, d_texs, d_texels, d_grids,

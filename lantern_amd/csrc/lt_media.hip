@@ -62,4 +62,37 @@ int launch_transmittance(hipStream_t st, const DScene &sc, const DMedium &md, co
     return LT_OK;
 }
 
+namespace {
+
+__global__ void k_zero_u32(uint32_t *p) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p = 0u;
+}
+
+/* Clamps densities to >= 0 in place and takes their maximum into *maxd (the
+ * delta-tracking majorant). Non-negative floats order like their bits. */
+__global__ __launch_bounds__(256) void k_grid_refresh(float *v, size_t n, uint32_t *maxd_bits) {
+    uint32_t m = 0u;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float d = v[i];
+        if (!(d >= 0.0f)) {
+            d = 0.0f;
+            v[i] = d;
+        }
+        m = max(m, __float_as_uint(d));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_down((int)m, off));
+    if ((threadIdx.x & 63) == 0 && m) atomicMax(maxd_bits, m);
+}
+
+} // namespace
+
+int launch_grid_refresh(hipStream_t st, float *voxels, size_t n, float *maxd, int num_cus) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, (uint32_t *)maxd);
+    size_t blocks = std::min<size_t>((n + 255) / 256, (size_t)std::max(1, num_cus) * 8);
+    hipLaunchKernelGGL(k_grid_refresh, dim3((uint32_t)std::max<size_t>(blocks, 1)), dim3(256), 0, st, voxels, n,
+                       (uint32_t *)maxd);
+    LT_HIP(hipGetLastError());
+    return LT_OK;
+}
+
 } // namespace lt

@@ -82,8 +82,23 @@ struct NeeRec {
     float *nl;
 };
 
+/* Path-replay adjoint (Vicini, Speierer & Jakob, "Path Replay
+ * Backpropagation", SIGGRAPH 2021). The replay walks every path again with
+ * the same random numbers, starting from the radiance the forward pass
+ * recorded for it; `C` then holds the radiance still to come, and each
+ * factor the path picks up (an image-texture albedo, a delta-tracking
+ * decision) adds dLoss/dI . C / factor to that parameter's gradient. */
+struct AdjArgs {
+    float *sample_rad;        /* forward: record (if set); replay: read. 3 floats per sample, global order */
+    const float *pixel_grad;  /* replay: dLoss/d(pixel mean), 3 floats per pixel */
+    float *grad_texels;       /* replay: += over the texel pool */
+    float *grad_voxels;       /* replay: += over the voxel pool */
+    float inv_spp;
+};
+
 struct WaveArgs {
     DScene sc;
+    AdjArgs adj;
     CameraArgs cam;
     PathState ps;
     uint32_t npix;            /* pixels per sample */
@@ -121,6 +136,7 @@ __device__ __forceinline__ float filter_sample(const CameraArgs &c, float x) {
     return neg ? -u : u;
 }
 
+template <bool ADJ>
 __global__ void k_generate(WaveArgs a, Queue q, uint32_t n, Ctl *ctl) {
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
         uint32_t x, y, frame;
@@ -141,7 +157,12 @@ __global__ void k_generate(WaveArgs a, Queue q, uint32_t n, Ctl *ctl) {
         rp[1] = r1;
         q.path[p] = p;
         a.ps.tx[p] = 1.0f; a.ps.ty[p] = 1.0f; a.ps.tz[p] = 1.0f;
-        a.ps.cx[p] = 0.0f; a.ps.cy[p] = 0.0f; a.ps.cz[p] = 0.0f;
+        if (ADJ) {
+            const float *rad = a.adj.sample_rad + 3 * ((size_t)a.sample_base * a.npix + p);
+            a.ps.cx[p] = rad[0]; a.ps.cy[p] = rad[1]; a.ps.cz[p] = rad[2];
+        } else {
+            a.ps.cx[p] = 0.0f; a.ps.cy[p] = 0.0f; a.ps.cz[p] = 0.0f;
+        }
         a.ps.iori[p] = 1.0f;
         a.ps.medium[p] = -1;
         a.ps.lobe[p] = LOBE_NULL;
@@ -183,7 +204,18 @@ __device__ __forceinline__ bool light_hit(const DScene &sc, const DLight &L, con
     return found;
 }
 
-/* One bounce of RenderPixel for every live path (integrator.cpp:119-242). */
+/* Gradient weight dLoss/dI . C / spp of the radiance C still to come on a
+ * replayed path through pixel pix. */
+__device__ __forceinline__ V3 adj_weight(const WaveArgs &a, uint32_t pix, const V3 &C) {
+    const float *g = a.adj.pixel_grad + 3 * (size_t)pix;
+    return v3(g[0] * C.x, g[1] * C.y, g[2] * C.z) * a.adj.inv_spp;
+}
+
+/* One bounce of RenderPixel for every live path (integrator.cpp:119-242).
+ * ADJ: the replay of the adjoint pass; contributions are subtracted from the
+ * radiance still to come instead of added, and parameter factors scatter
+ * their gradients. */
+template <bool ADJ>
 __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next,
                                                NeeRec nee, Ray *nee_rays, Hit *nee_hits, uint32_t *nee_ignore,
                                                uint32_t *nee_cidx, Ctl *ctl) {
@@ -206,6 +238,7 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
             p = cur.path[q];
             uint32_t x, y, frame;
             path_coords(a, p, x, y, frame);
+            const uint32_t pix = y * a.cam.w + x;
             PathRng rng;
             rng.init(x, y, a.stream, a.seed, frame, a.ps.rngi[p]);
             const float4 *rp = (const float4 *)(cur.rays + q);
@@ -220,7 +253,8 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
             bool dead = false;
             uint32_t nee_info = 0;
             if (h.prim == 0xFFFFFFFFu) {
-                C = C + T * v3(sc.bg[0], sc.bg[1], sc.bg[2]);
+                if (ADJ) C = C - T * v3(sc.bg[0], sc.bg[1], sc.bg[2]);
+                else C = C + T * v3(sc.bg[0], sc.bg[1], sc.bg[2]);
                 dead = true;
             } else {
                 V3 origin = v3(ra.x, ra.y, ra.z);
@@ -238,8 +272,18 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
                                 t = t - log_f(1.0f - rng.next_float()) / sbar;
                                 if (!(t < t1)) break;
                                 V3 qp = origin + dir * t;
-                                float dens = grid_density(sc, g, qp.x, qp.y, qp.z);
-                                if (rng.next_float() * sbar < md.scale * dens) {
+                                const long vox = grid_voxel(g, qp.x, qp.y, qp.z);
+                                const float dens = vox < 0 ? 0.0f : sc.voxels[vox];
+                                const bool real = rng.next_float() * sbar < md.scale * dens;
+                                if (ADJ && vox >= 0) {
+                                    /* d log P / d density: 1/density (collision),
+                                     * -scale/(sbar - scale*density) (null) */
+                                    const V3 wgt = adj_weight(a, pix, C);
+                                    const float dl = wgt.x + wgt.y + wgt.z;
+                                    const float dlogp = real ? 1.0f / dens : -md.scale / (sbar - md.scale * dens);
+                                    unsafeAtomicAdd(a.adj.grad_voxels + vox, dl * dlogp);
+                                }
+                                if (real) {
                                     collided = true;
                                     break;
                                 }
@@ -274,7 +318,8 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
                     const int light = g.light;
                     if ((b == 0 || (it.lobe & LOBE_SPECULAR) != 0) && light >= 0) {
                         const DLight L = sc.lights[light];
-                        C = C + T * v3(L.le[0], L.le[1], L.le[2]);
+                        if (ADJ) C = C - T * v3(L.le[0], L.le[1], L.le[2]);
+                        else C = C + T * v3(L.le[0], L.le[1], L.le[2]);
                     }
                     it.P = origin + dir * h.t;
                     if (g.flags & 1u) it.N = normalize(interp_normal(sc, h.prim, h.u, h.v));
@@ -284,6 +329,18 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
                     it.wo = -dir;
                     it.ioro = 0.0f;
                     P = it.P;
+                    if (ADJ && m.texture >= 0 && m.bsdf != BSDF_GGX) {
+                        /* Lambert, mirror and dielectric: the next-event terms
+                         * here and every later term carry this albedo once,
+                         * linearly (Eval/Pdf = albedo). */
+                        const size_t to = texel_offset(sc, m, it);
+                        const V3 wgt = adj_weight(a, pix, C);
+                        const float al[3] = {sc.texels[to], sc.texels[to + 1], sc.texels[to + 2]};
+                        const float wv[3] = {wgt.x, wgt.y, wgt.z};
+#pragma unroll
+                        for (int c = 0; c < 3; ++c)
+                            if (al[c] != 0.0f && wv[c] != 0.0f) unsafeAtomicAdd(a.adj.grad_texels + to + c, wv[c] / al[c]);
+                    }
 
                     /* SampleOneLight / EstimateDirect on a copy of the interaction
                      * (integrator.cpp:252-315); the two casts are deferred. */
@@ -417,6 +474,7 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
 
 /* The casts of AreaLight::SampleLi / PdfLi resolved (area_light.cpp:58-71,
  * 100-109) and the MIS sums of EstimateDirect (integrator.cpp:287-314). */
+template <bool ADJ>
 __global__ __launch_bounds__(256) void k_resolve(WaveArgs a, uint32_t b, Queue cur, NeeRec nee, const Ray *nee_rays,
                                                  const Hit *nee_hits, const Hit *nee_trace_hits,
                                                  const uint32_t *nee_occl, const uint32_t *nee_cidx, Ctl *ctl) {
@@ -472,6 +530,7 @@ __global__ __launch_bounds__(256) void k_resolve(WaveArgs a, uint32_t b, Queue c
         const uint32_t p = cur.path[q];
         V3 T = v3(nee.tx[q], nee.ty[q], nee.tz[q]);
         V3 add = T * (nee.nl[q] * direct);
+        if (ADJ) add = -add;
         a.ps.cx[p] = a.ps.cx[p] + add.x;
         a.ps.cy[p] = a.ps.cy[p] + add.y;
         a.ps.cz[p] = a.ps.cz[p] + add.z;
@@ -485,6 +544,12 @@ __global__ void k_accumulate(WaveArgs a, uint32_t nsamples, float *sum, uint32_t
         uint32_t bb = bounces[pix], cc = count[pix];
         for (uint32_t s = 0; s < nsamples; ++s) {
             size_t p = (size_t)s * a.npix + pix;
+            if (a.adj.sample_rad) {
+                float *rad = a.adj.sample_rad + 3 * ((size_t)a.sample_base * a.npix + p);
+                rad[0] = a.ps.cx[p];
+                rad[1] = a.ps.cy[p];
+                rad[2] = a.ps.cz[p];
+            }
             sx = sx + a.ps.cx[p];
             sy = sy + a.ps.cy[p];
             sz = sz + a.ps.cz[p];
@@ -643,9 +708,10 @@ void camera_setup(const lt_camera &c, CameraSetup &o) {
     o.cdf[32] = 1.0f;
 }
 
-/* Renders one scene; see the file comment for the kernel sequence. */
-int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &prm, lt_film *film,
-                 lt_render_stats *stats) {
+/* Renders one scene; see the file comment for the kernel sequence. With
+ * replay set, runs the adjoint pass instead (no film). */
+int render_impl(lt_scene *scene, const lt_camera &cam, const lt_render_params &prm, lt_film *film,
+                lt_render_stats *stats, const AdjArgs &adj, bool replay) {
     lt_device *dev = scene->dev;
     hipStream_t st = dev->stream;
     const uint32_t npix = cam.width * cam.height;
@@ -662,7 +728,8 @@ int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &
     Ctl *ctl = (Ctl *)dev->ctl.p;
     const int tblock = prm.block_size ? (int)prm.block_size : trace_default_block();
     const int tvar = prm.trace_variant ? (int)prm.trace_variant - 1 : kDefaultTraceVariant;
-    const int tgrid = trace_grid(dev->num_cus, tblock);
+    const int tgrid = trace_grid(dev->num_cus, tblock, tvar);
+
     if ((rc = dev->spill.reserve(trace_spill_bytes(tgrid, tblock)))) return rc;
     int *spill = (int *)dev->spill.p;
 
@@ -670,6 +737,8 @@ int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &
     camera_setup(cam, cs);
     WaveArgs a;
     a.sc = scene->ds;
+    a.adj = adj;
+    a.adj.inv_spp = 1.0f / (float)prm.spp;
     std::memcpy(&a.cam, &cs, sizeof(CameraArgs));
     a.ps = w.ps;
     a.npix = npix;
@@ -694,8 +763,12 @@ int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &
         ++waves;
         hipLaunchKernelGGL(k_reset_ctl, dim3(16), dim3(256), 0, st, ctl);
         tm.begin(T_GEN);
-        hipLaunchKernelGGL(k_generate, dim3(std::min<uint32_t>((n + 255) / 256, 65535)), dim3(256), 0, st, a, w.qa, n,
-                           ctl);
+        if (replay)
+            hipLaunchKernelGGL(k_generate<true>, dim3(std::min<uint32_t>((n + 255) / 256, 65535)), dim3(256), 0, st,
+                               a, w.qa, n, ctl);
+        else
+            hipLaunchKernelGGL(k_generate<false>, dim3(std::min<uint32_t>((n + 255) / 256, 65535)), dim3(256), 0, st,
+                               a, w.qa, n, ctl);
         tm.end(T_GEN);
         LT_HIP(hipGetLastError());
         Queue cur = w.qa, nxt = w.qb;
@@ -716,8 +789,12 @@ int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &
             tm.end(T_EXT);
             ++ext_launches;
             tm.begin(T_SHADE);
-            hipLaunchKernelGGL(k_shade, dim3(sgrid), dim3(256), 0, st, a, b, cur, (const Hit *)w.hits, nxt, w.nee,
-                               w.nee_rays, w.nee_hits, w.nee_ignore, w.nee_cidx, ctl);
+            if (replay)
+                hipLaunchKernelGGL(k_shade<true>, dim3(sgrid), dim3(256), 0, st, a, b, cur, (const Hit *)w.hits, nxt,
+                                   w.nee, w.nee_rays, w.nee_hits, w.nee_ignore, w.nee_cidx, ctl);
+            else
+                hipLaunchKernelGGL(k_shade<false>, dim3(sgrid), dim3(256), 0, st, a, b, cur, (const Hit *)w.hits, nxt,
+                                   w.nee, w.nee_rays, w.nee_hits, w.nee_ignore, w.nee_cidx, ctl);
             tm.end(T_SHADE);
             LT_HIP(hipGetLastError());
             /* next-event rays: two per shaded slot; inactive ones have tfar < 0 */
@@ -734,18 +811,25 @@ int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &
             tm.end(T_SHADOW);
             ++sh_launches;
             tm.begin(T_RESOLVE);
-            hipLaunchKernelGGL(k_resolve, dim3(sgrid), dim3(256), 0, st, a, b, cur, w.nee, (const Ray *)w.nee_rays,
-                               (const Hit *)w.nee_hits, (const Hit *)w.nee_trace_hits, (const uint32_t *)w.nee_occl,
-                               (const uint32_t *)w.nee_cidx, ctl);
+            if (replay)
+                hipLaunchKernelGGL(k_resolve<true>, dim3(sgrid), dim3(256), 0, st, a, b, cur, w.nee,
+                                   (const Ray *)w.nee_rays, (const Hit *)w.nee_hits, (const Hit *)w.nee_trace_hits,
+                                   (const uint32_t *)w.nee_occl, (const uint32_t *)w.nee_cidx, ctl);
+            else
+                hipLaunchKernelGGL(k_resolve<false>, dim3(sgrid), dim3(256), 0, st, a, b, cur, w.nee,
+                                   (const Ray *)w.nee_rays, (const Hit *)w.nee_hits, (const Hit *)w.nee_trace_hits,
+                                   (const uint32_t *)w.nee_occl, (const uint32_t *)w.nee_cidx, ctl);
             tm.end(T_RESOLVE);
             LT_HIP(hipGetLastError());
             std::swap(cur, nxt);
         }
-        tm.begin(T_ACC);
-        hipLaunchKernelGGL(k_accumulate, dim3(std::min<uint32_t>((npix + 255) / 256, 65535)), dim3(256), 0, st, a, ns,
-                           film->sum.as<float>(), film->bounces.as<uint32_t>(), film->count.as<uint32_t>());
-        tm.end(T_ACC);
-        LT_HIP(hipGetLastError());
+        if (!replay) {
+            tm.begin(T_ACC);
+            hipLaunchKernelGGL(k_accumulate, dim3(std::min<uint32_t>((npix + 255) / 256, 65535)), dim3(256), 0, st, a,
+                               ns, film->sum.as<float>(), film->bounces.as<uint32_t>(), film->count.as<uint32_t>());
+            tm.end(T_ACC);
+            LT_HIP(hipGetLastError());
+        }
         if (stats) {
             std::vector<uint32_t> counts(prm.max_depth + 1), ncounts(prm.max_depth + 1);
             LT_HIP(hipMemcpyAsync(counts.data(), ctl->count, sizeof(uint32_t) * (prm.max_depth + 1),
@@ -785,6 +869,32 @@ int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &
         stats->waves = waves;
     }
     return LT_OK;
+}
+
+int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &prm, lt_film *film,
+                 lt_render_stats *stats) {
+    AdjArgs adj = {};
+    return render_impl(scene, cam, prm, film, stats, adj, false);
+}
+
+int render_record(lt_scene *scene, const lt_camera &cam, const lt_render_params &prm, lt_film *film,
+                  float *sample_radiance, lt_render_stats *stats) {
+    AdjArgs adj = {};
+    adj.sample_rad = sample_radiance;
+    return render_impl(scene, cam, prm, film, stats, adj, false);
+}
+
+int render_adjoint(lt_scene *scene, const lt_camera &cam, const lt_render_params &prm, const float *sample_radiance,
+                   const float *pixel_grad, float *grad_texels, float *grad_voxels, lt_render_stats *stats) {
+    if (prm.rr_start < prm.max_depth)
+        return set_error(LT_E_INVALID, "lt_adjoint: Russian roulette makes the estimator non-differentiable; "
+                                       "set rr_start >= max_depth");
+    AdjArgs adj = {};
+    adj.sample_rad = const_cast<float *>(sample_radiance);
+    adj.pixel_grad = pixel_grad;
+    adj.grad_texels = grad_texels;
+    adj.grad_voxels = grad_voxels;
+    return render_impl(scene, cam, prm, nullptr, stats, adj, true);
 }
 
 } // namespace lt

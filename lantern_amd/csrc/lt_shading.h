@@ -79,16 +79,22 @@ LT_D int supported_lobes(int bsdf) {
 
 /* ConstantTexture / UVTexture / ImageTexture (+ ImageCache::SampleImage:
  * nearest texel by std::round, v flipped; indices clamped to the image). */
-LT_D V3 albedo(const DScene &sc, const DMaterial &m, const Interaction &it) {
-    if (m.texture == TEX_CONSTANT) return v3(m.albedo[0], m.albedo[1], m.albedo[2]);
-    if (m.texture == TEX_UV) return v3(it.tu, it.tv, 0.0f);
+/* Offset in the texel pool of the texel an image texture returns at the
+ * interaction's uv. */
+LT_D size_t texel_offset(const DScene &sc, const DMaterial &m, const Interaction &it) {
     const DTexture t = sc.textures[m.texture];
     float u = it.tu, v = 1.0f - it.tv;
     long x = (long)roundf(u * (float)t.w);
     long y = (long)roundf(v * (float)t.h);
     x = x < 0 ? 0 : (x > (long)t.w - 1 ? (long)t.w - 1 : x);
     y = y < 0 ? 0 : (y > (long)t.h - 1 ? (long)t.h - 1 : y);
-    const float *p = sc.texels + t.offset + ((size_t)y * t.w + (size_t)x) * 3;
+    return t.offset + ((size_t)y * t.w + (size_t)x) * 3;
+}
+
+LT_D V3 albedo(const DScene &sc, const DMaterial &m, const Interaction &it) {
+    if (m.texture == TEX_CONSTANT) return v3(m.albedo[0], m.albedo[1], m.albedo[2]);
+    if (m.texture == TEX_UV) return v3(it.tu, it.tv, 0.0f);
+    const float *p = sc.texels + texel_offset(sc, m, it);
     return v3(p[0], p[1], p[2]);
 }
 
@@ -189,18 +195,24 @@ LT_D void interp_uv(const DScene &sc, uint32_t gprim, float u, float v, float &o
 }
 
 /* Nearest-voxel density of a grid medium (no reference counterpart). */
-LT_D float grid_density(const DScene &sc, const DGrid &g, float px, float py, float pz) {
+/* Voxel (pool offset) holding point p, or -1 outside the grid's box. */
+LT_D long grid_voxel(const DGrid &g, float px, float py, float pz) {
     float p[3] = {px, py, pz};
     uint32_t n[3] = {g.nx, g.ny, g.nz};
     int ix[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        if (!(p[a] >= g.bmin[a] && p[a] <= g.bmax[a])) return 0.0f;
+        if (!(p[a] >= g.bmin[a] && p[a] <= g.bmax[a])) return -1;
         float f = (p[a] - g.bmin[a]) / (g.bmax[a] - g.bmin[a]) * (float)n[a];
         int i = (int)f;
         ix[a] = i < 0 ? 0 : (i > (int)n[a] - 1 ? (int)n[a] - 1 : i);
     }
-    return sc.voxels[g.offset + ((size_t)ix[2] * g.ny + ix[1]) * g.nx + ix[0]];
+    return (long)(g.offset + ((size_t)ix[2] * g.ny + ix[1]) * g.nx + ix[0]);
+}
+
+LT_D float grid_density(const DScene &sc, const DGrid &g, float px, float py, float pz) {
+    const long v = grid_voxel(g, px, py, pz);
+    return v < 0 ? 0.0f : sc.voxels[v];
 }
 
 /* Span [t0, t1) of o + t d inside the grid's box, intersected with

@@ -393,10 +393,16 @@ int launch(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, uint32_
 
 int trace_default_block() { return 128; }
 
-/* Enough blocks for 32 waves per CU (the hardware maximum); blocks that
- * cannot be resident start after others finish and then find the work
- * counter exhausted. */
-int trace_grid(int num_cus, int block) { return std::max(1, num_cus) * std::max(1, 32 / (block / 64)); }
+/* Resident blocks per CU: limited by the LDS stack of the launched variant
+ * (binary: LDS_STACK x 4 B, 8-wide: LDS8 x 8 B per lane, of 160 KB) and by
+ * 32 waves per CU. */
+int trace_grid(int num_cus, int block, int variant) {
+    const int lds8 = variant == 0 ? 16 : 8;
+    const int lds_per_lane = std::max(LDS_STACK * 4, lds8 * 8);
+    const int by_lds = (160 * 1024) / (lds_per_lane * block);
+    const int by_waves = std::max(1, 32 / (block / 64));
+    return std::max(1, num_cus) * std::max(1, std::min(by_lds, by_waves));
+}
 
 size_t trace_spill_bytes(int grid, int block) {
     size_t per = std::max((size_t)(STACK_CAP - LDS_STACK) * sizeof(int), (size_t)STACK8_CAP * sizeof(int2));

@@ -345,3 +345,59 @@ def furnace_plane(bsdf=_capi.LT_BSDF_LAMBERT, albedo=(0.5, 0.5, 0.5), background
     d.params = dict(spp=spp, max_depth=max_depth, rr_start=max_depth, seed=DEFAULT_SEED)
     d.meta = dict(workload="furnace_plane")
     return d
+
+
+def value_noise_texture(res, seed, octaves=4, base=4):
+    """(res, res, 3) float32 albedo texture in [0.1, 0.9]: a sum of bilinearly
+    upsampled seeded random grids (value noise), one per channel."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = np.zeros((res, res, 3))
+    c = (np.arange(res) + 0.5) / res
+    amp, tot = 1.0, 0.0
+    for o in range(octaves):
+        n = base << o
+        g = rng.uniform(0.0, 1.0, (n + 1, n + 1, 3))
+        f = c * n
+        i = np.minimum(f.astype(np.int64), n - 1)
+        t = f - i
+        a = g[i][:, i] * ((1 - t)[:, None, None] * (1 - t)[None, :, None])
+        a += g[i + 1][:, i] * (t[:, None, None] * (1 - t)[None, :, None])
+        a += g[i][:, i + 1] * ((1 - t)[:, None, None] * t[None, :, None])
+        a += g[i + 1][:, i + 1] * (t[:, None, None] * t[None, :, None])
+        out += amp * a
+        tot += amp
+        amp *= 0.5
+    return (0.1 + 0.8 * out / tot).astype(np.float32)
+
+
+def diff_cube(width=256, height=256, spp=64, seed=DEFAULT_SEED, tex_res=1024, grid_res=64, max_depth=6,
+              sigma_scale=4.0):
+    """The differentiable workload: a 12-triangle cube (CreateBox 1.2) with a
+    tex_res^2 x 3 FP32 albedo texture (Lambert), inside a 2.4 CreateBox of
+    IOR 1.0 (an index-matched boundary) holding a grid_res^3 fBm smoke
+    (sigma_t = 4 density, albedo 0.9, HG g = 0.3); one quad light at y = 3,
+    environment 0.5. Russian roulette off (the adjoint needs it off).
+    Parameters: the texture and the density grid."""
+    d = SceneDesc()
+    d.textures.append(value_noise_texture(tex_res, seed))
+    dens = fbm_grid(grid_res, seed)
+    d.grids.append(dict(density=dens, bmin=(-1.2, -1.2, -1.2), bmax=(1.2, 1.2, 1.2)))
+    d.media.append(dict(type=_capi.LT_MEDIUM_GRID, grid=0, sigma_t_scale=sigma_scale, albedo=0.9, hg_g=0.3))
+    cube_mat = d.add_material(bsdf=_capi.LT_BSDF_LAMBERT, albedo=(1.0, 1.0, 1.0), texture=0)
+    fog_mat = d.add_material(bsdf=_capi.LT_BSDF_DIELECTRIC, albedo=(1.0, 1.0, 1.0), ior=1.0, medium=0)
+    light_mat = d.add_material(bsdf=_capi.LT_BSDF_LAMBERT, albedo=(0.0, 0.0, 0.0))
+    cp, cn, cuv, ci = box(1.2, 1.2, 1.2)
+    d.add_mesh(cp, ci, cn, cuv, cube_mat)
+    fp_, fn, fuv, fi = box(2.4, 2.4, 2.4)
+    d.add_mesh(fp_, fi, fn, fuv, fog_mat)
+    lp, ln, luv, li, power = quad_light(2.0, 3.0, 8.0)
+    d.add_mesh(lp, li, ln, luv, light_mat, emission=((1.0, 1.0, 1.0), power))
+    d.background = (0.5, 0.5, 0.5)
+    d.camera = dict(phi=1.1, theta=0.7, radius=5.0, width=width, height=height, target=(0.0, 0.0, 0.0),
+                    fov=math.pi / 4, filter=0)
+    d.params = dict(spp=spp, max_depth=max_depth, rr_start=max_depth, seed=seed)
+    d.meta = dict(workload="diff_cube", seed=seed, tex_res=tex_res, grid_res=grid_res, sigma_t_scale=sigma_scale,
+                  single_scatter_albedo=0.9, hg_g=0.3, fog_box=2.4, cube=1.2, light_height=3.0, light_le=8.0,
+                  background=0.5, camera_phi=1.1, camera_theta=0.7, camera_radius=5.0, max_depth=max_depth,
+                  russian_roulette="off")
+    return d

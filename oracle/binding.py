@@ -93,6 +93,11 @@ def lib():
             "orc_sample_hg": (None, [C.c_float, _pf, C.c_float, C.c_float, _pf]),
             "orc_bsdf_sample": (C.c_int32, [C.POINTER(orc_material_desc), _pf, _pf, C.c_float, _pf, C.c_uint32, _pf]),
             "orc_bsdf_eval": (None, [C.POINTER(orc_material_desc), _pf, _pf, _pf, _pf]),
+            "orc_scene_update_texture": (C.c_int32, [C.c_void_p, C.c_int32, _pf]),
+            "orc_scene_update_grid": (C.c_int32, [C.c_void_p, C.c_int32, _pf, C.c_int32]),
+            "orc_pool_sizes": (None, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+            "orc_adjoint_f64": (None, [C.c_void_p, C.POINTER(orc_camera_desc), C.POINTER(orc_render_params), _pd,
+                                       _pd, _pd]),
             "orc_hardware_threads": (C.c_uint32, []),
         }
         for name, (res, args) in sig.items():
@@ -200,6 +205,39 @@ class OracleScene:
         out = np.zeros(rays.shape[0], np.float32)
         lib().orc_transmittance(self.h, medium, fp(rays), rays.shape[0], seed, method, fp(out))
         return out
+
+    def update_texture(self, tex_id, rgb):
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        assert lib().orc_scene_update_texture(self.h, tex_id, fp(rgb)) == 0
+
+    def update_grid(self, grid_id, density, freeze_sampling=False):
+        density = np.ascontiguousarray(density, np.float32)
+        assert lib().orc_scene_update_grid(self.h, grid_id, fp(density), 1 if freeze_sampling else 0) == 0
+
+    def pool_sizes(self):
+        t, v = C.c_uint64(), C.c_uint64()
+        lib().orc_pool_sizes(self.h, C.byref(t), C.byref(v))
+        return t.value, v.value
+
+    def adjoint(self, cam, params, pixel_grad, threads=0, frame0=0, stream=0):
+        """FP64 path-replay adjoint; pixel_grad (h, w, 3) = dLoss/d(pixel mean).
+        Returns (grad_texels, grad_voxels) over the parameter pools."""
+        nt, nv = self.pool_sizes()
+        gt = np.zeros(max(nt, 1), np.float64)
+        gv = np.zeros(max(nv, 1), np.float64)
+        p = orc_render_params()
+        p.spp = params["spp"]
+        p.max_depth = params.get("max_depth", 1500)
+        p.rr_start = params.get("rr_start", 4)
+        p.seed = params.get("seed", 0x4C414E54)
+        p.frame0 = frame0
+        p.rng = ORC_RNG_PHILOX
+        p.threads = threads if threads > 0 else default_threads()
+        p.stream = stream
+        g = np.ascontiguousarray(pixel_grad, np.float64)
+        lib().orc_adjoint_f64(self.h, C.byref(camera(cam)), C.byref(p), g.ctypes.data_as(_pd),
+                              gt.ctypes.data_as(_pd), gv.ctypes.data_as(_pd))
+        return gt[:nt], gv[:nv]
 
     def render(self, cam, params, precision=32, rng=ORC_RNG_PHILOX, threads=0, window=None, frame0=0, stream=0):
         """Returns (sum (h,w,3), bounces (h,w), count (h,w), seconds, rays)."""

@@ -500,12 +500,14 @@ struct Light {
     float sphere[4];
     u32 geom;
 };
-struct Texture { u32 w, h; std::vector<float> rgb; };
+struct Texture { u32 w, h; std::vector<float> rgb; size_t pool; /* offset in the texel pool (floats) */ };
 struct Grid {
     u32 nx, ny, nz;
-    std::vector<float> d;
+    std::vector<float> d;    /* densities the estimator weights with */
+    std::vector<float> ds;   /* densities the tracking decisions use (equal to d unless frozen) */
     float bmin[3], bmax[3];
-    float maxd;
+    float maxd;              /* majorant: max of ds */
+    size_t pool;             /* offset in the voxel pool */
 };
 struct Medium {
     int type;
@@ -915,6 +917,16 @@ template <class R> inline V3<R> albedo(const orc_scene &s, const orc_material_de
     return V3<R>(p[0], p[1], p[2]);
 }
 
+template <class R> inline size_t texel_pool_offset(const orc_scene &s, const orc_material_desc &m, R tu, R tv) {
+    const Texture &t = s.texs[m.texture];
+    R u = tu, v = R(1) - tv;
+    long x = (long)std::round(u * (R)t.w);
+    long y = (long)std::round(v * (R)t.h);
+    x = std::min<long>(std::max<long>(x, 0), (long)t.w - 1);
+    y = std::min<long>(std::max<long>(y, 0), (long)t.h - 1);
+    return t.pool + ((size_t)y * t.w + (size_t)x) * 3;
+}
+
 inline int supported_lobes(int bsdf) {
     switch (bsdf) {
     case ORC_BSDF_LAMBERT: return LOBE_DIFFUSE;
@@ -1127,18 +1139,41 @@ V3<R> sample_one_light(const orc_scene &s, Sampler &smp, Interaction<R> it, cons
 }
 
 /* Heterogeneous grid: nearest-voxel density (no reference counterpart). */
-inline float grid_density(const Grid &g, float px, float py, float pz) {
+/* Voxel of point p (nearest-voxel lookup), -1 outside the grid's box. */
+inline long grid_voxel(const Grid &g, float px, float py, float pz) {
     float p[3] = {px, py, pz};
     int ix[3];
     u32 n[3] = {g.nx, g.ny, g.nz};
     for (int a = 0; a < 3; ++a) {
-        if (!(p[a] >= g.bmin[a] && p[a] <= g.bmax[a])) return 0.0f;
+        if (!(p[a] >= g.bmin[a] && p[a] <= g.bmax[a])) return -1;
         float f = (p[a] - g.bmin[a]) / (g.bmax[a] - g.bmin[a]) * (float)n[a];
         int i = (int)f;
         ix[a] = std::min(std::max(i, 0), (int)n[a] - 1);
     }
-    return g.d[((size_t)ix[2] * g.ny + ix[1]) * g.nx + ix[0]];
+    return (long)(((size_t)ix[2] * g.ny + ix[1]) * g.nx + ix[0]);
 }
+inline float grid_density(const Grid &g, float px, float py, float pz) {
+    long v = grid_voxel(g, px, py, pz);
+    return v < 0 ? 0.0f : g.ds[v];
+}
+
+/* Path-replay adjoint state of one path (no reference counterpart): the
+ * path's total radiance from its forward pass, dLoss/dI of its pixel / spp,
+ * and the gradient pools it adds into. The radiance still to come at any
+ * point of the replay is total - color. */
+template <class R> struct Adj {
+    V3<R> total;
+    R gx, gy, gz;
+    R *grad_tex;
+    R *grad_vox;
+    V3<R> weight(const V3<R> &color) const {
+        V3<R> rem = total - color;
+        return V3<R>(gx * rem.x, gy * rem.y, gz * rem.z);
+    }
+};
+
+/* Offset in the texel pool of the texel an image texture returns. */
+template <class R> inline size_t texel_pool_offset(const orc_scene &s, const orc_material_desc &m, R tu, R tv);
 
 /* Span [t0, t1) of the ray o + t d inside the grid's box, intersected with
  * [tmin, tmax] (no reference counterpart). Density is zero outside the box,
@@ -1204,7 +1239,7 @@ template <class R> inline V3<R> sample_hg(R g, const V3<R> &d, Sampler &s) {
 /* Integrator::RenderPixel (integrator.cpp:98-250) for one path. */
 template <class R>
 u32 render_pixel(const orc_scene &s, const Camera &cam, u32 px, u32 py, Sampler &smp, const orc_render_params &p,
-                 V3<R> &color_out, Stats &st) {
+                 V3<R> &color_out, Stats &st, const Adj<R> *adj = nullptr) {
     float fu = smp.next_float(), fv = smp.next_float();
     V3<float> o32, d32;
     cam.ray(px, py, fu, fv, o32, d32);
@@ -1237,11 +1272,22 @@ u32 render_pixel(const orc_scene &s, const Camera &cam, u32 px, u32 py, Sampler 
                         t = t - MathK<R>::log(R(1) - uni<R>(smp)) / sbar;
                         if (!(t < t1)) break;
                         V3<R> q = origin + dir * t;
-                        R dens = (R)grid_density(g, (float)q.x, (float)q.y, (float)q.z);
-                        if (uni<R>(smp) * sbar < (R)md.scale * dens) {
+                        long vx = grid_voxel(g, (float)q.x, (float)q.y, (float)q.z);
+                        R dens_s = vx < 0 ? R(0) : (R)g.ds[vx];   /* decides */
+                        R dens_v = vx < 0 ? R(0) : (R)g.d[vx];    /* weighs */
+                        bool real = uni<R>(smp) * sbar < (R)md.scale * dens_s;
+                        if (adj && vx >= 0) {
+                            V3<R> w = adj->weight(color);
+                            R dlogp = real ? R(1) / dens_s : -(R)md.scale / (sbar - (R)md.scale * dens_s);
+                            adj->grad_vox[g.pool + (size_t)vx] += (w.x + w.y + w.z) * dlogp;
+                        }
+                        if (real) {
+                            if (dens_v != dens_s) T = T * (dens_v / dens_s);
                             collided = true;
                             break;
                         }
+                        if (dens_v != dens_s)
+                            T = T * ((sbar - (R)md.scale * dens_v) / (sbar - (R)md.scale * dens_s));
                     }
                 }
                 if (collided) {
@@ -1281,6 +1327,16 @@ u32 render_pixel(const orc_scene &s, const Camera &cam, u32 px, u32 py, Sampler 
                 it.tu = it.tv = R(0);
             it.wo = -dir;
             it.ioro = R(0);
+            if (adj && m.texture >= 0 && m.bsdf != ORC_BSDF_GGX) {
+                /* the next-event terms here and every later term carry this albedo once */
+                size_t to = texel_pool_offset<R>(s, m, it.tu, it.tv);
+                const Texture &tx = s.texs[m.texture];
+                const float *al = &tx.rgb[to - tx.pool];
+                V3<R> w = adj->weight(color);
+                R wv[3] = {w.x, w.y, w.z};
+                for (int c = 0; c < 3; ++c)
+                    if (al[c] != 0.0f) adj->grad_tex[to + c] += wv[c] / (R)al[c];
+            }
             color = color + T * sample_one_light<R>(s, smp, it, m, light, st);
             bsdf_sample<R>(m, it, smp);
             R pdf = bsdf_pdf<R>(m, it);
@@ -1356,6 +1412,56 @@ double render_impl(const orc_scene &s, const orc_camera_desc &cd, const orc_rend
     _mm_setcsr(csr);
     if (rays) *rays = total_rays.load();
     return std::chrono::duration<double>(t1 - t0).count();
+}
+
+/* Path-replay adjoint over the whole frame set (Philox streams): for every
+ * sample, a forward pass for its total radiance, then the replay that
+ * scatters gradients. Threads keep private gradient pools, summed at the end. */
+template <class R>
+void adjoint_impl(const orc_scene &s, const orc_camera_desc &cd, const orc_render_params &p, const R *pixel_grad,
+                  R *grad_tex, R *grad_vox) {
+    Camera cam(cd);
+    const u32 W = cd.width, H = cd.height;
+    size_t ntex = s.texs.empty() ? 0 : s.texs.back().pool + s.texs.back().rgb.size();
+    size_t nvox = s.grids.empty() ? 0 : s.grids.back().pool + s.grids.back().d.size();
+    u32 nthreads = p.threads > 0 ? (u32)p.threads : std::max(1u, std::thread::hardware_concurrency());
+    std::vector<std::vector<R>> gt(nthreads, std::vector<R>(ntex, R(0))), gv(nthreads, std::vector<R>(nvox, R(0)));
+    const R inv_spp = R(1) / (R)p.spp;
+    std::atomic<u32> next(0);
+    const u32 nrows = p.spp * H;
+    auto worker = [&](u32 tid) {
+        _mm_setcsr(_mm_getcsr() | 0x8040);
+        Stats st;
+        while (true) {
+            u32 r = next.fetch_add(1);
+            if (r >= nrows) break;
+            u32 fi = r / H, y = r - fi * H;
+            u32 frame = p.frame0 + fi;
+            for (u32 x = 0; x < W; ++x) {
+                size_t k = (size_t)y * W + x;
+                PcgSampler pcg(0, 0);
+                PhiloxSampler phx(p.seed, frame, x, y, p.stream);
+                Sampler smp{1, &pcg, &phx};
+                V3<R> total;
+                render_pixel<R>(s, cam, x, y, smp, p, total, st);
+                PhiloxSampler phx2(p.seed, frame, x, y, p.stream);
+                Sampler smp2{1, &pcg, &phx2};
+                Adj<R> a{total, pixel_grad[3 * k] * inv_spp, pixel_grad[3 * k + 1] * inv_spp,
+                         pixel_grad[3 * k + 2] * inv_spp, gt[tid].data(), gv[tid].data()};
+                V3<R> c2;
+                render_pixel<R>(s, cam, x, y, smp2, p, c2, st, &a);
+            }
+        }
+    };
+    unsigned int csr = _mm_getcsr();
+    std::vector<std::thread> th;
+    for (u32 i = 0; i < nthreads; ++i) th.emplace_back(worker, i);
+    for (auto &t : th) t.join();
+    _mm_setcsr(csr);
+    for (u32 i = 0; i < nthreads; ++i) {
+        for (size_t j = 0; j < ntex; ++j) grad_tex[j] += gt[i][j];
+        for (size_t j = 0; j < nvox; ++j) grad_vox[j] += gv[i][j];
+    }
 }
 
 } // namespace
@@ -1495,6 +1601,7 @@ int32_t orc_scene_add_texture(orc_scene *s, uint32_t w, uint32_t h, const float 
     t.w = w;
     t.h = h;
     t.rgb.assign(rgb, rgb + (size_t)w * h * 3);
+    t.pool = s->texs.empty() ? 0 : s->texs.back().pool + s->texs.back().rgb.size();
     s->texs.push_back(std::move(t));
     return (int32_t)s->texs.size() - 1;
 }
@@ -1503,8 +1610,10 @@ int32_t orc_scene_add_grid(orc_scene *s, uint32_t nx, uint32_t ny, uint32_t nz, 
     Grid g;
     g.nx = nx; g.ny = ny; g.nz = nz;
     g.d.assign(density, density + (size_t)nx * ny * nz);
+    g.ds = g.d;
     g.maxd = 0.0f;
     for (float v : g.d) g.maxd = std::max(g.maxd, v);
+    g.pool = s->grids.empty() ? 0 : s->grids.back().pool + s->grids.back().d.size();
     for (int a = 0; a < 3; ++a) { g.bmin[a] = bmin[a]; g.bmax[a] = bmax[a]; }
     s->grids.push_back(std::move(g));
     return (int32_t)s->grids.size() - 1;
@@ -1713,6 +1822,35 @@ void orc_bsdf_eval(const orc_material_desc *m, const float n[3], const float wo[
     V3<float> f = bsdf_eval<float>(empty, *m, it);
     out[0] = f.x; out[1] = f.y; out[2] = f.z;
     out[3] = bsdf_pdf<float>(*m, it);
+}
+
+int32_t orc_scene_update_texture(orc_scene *s, int32_t id, const float *rgb) {
+    if (id < 0 || id >= (int32_t)s->texs.size()) return -1;
+    Texture &t = s->texs[id];
+    std::copy(rgb, rgb + t.rgb.size(), t.rgb.begin());
+    return 0;
+}
+
+int32_t orc_scene_update_grid(orc_scene *s, int32_t id, const float *density, int32_t freeze_sampling) {
+    if (id < 0 || id >= (int32_t)s->grids.size()) return -1;
+    Grid &g = s->grids[id];
+    std::copy(density, density + g.d.size(), g.d.begin());
+    if (!freeze_sampling) {
+        g.ds = g.d;
+        g.maxd = 0.0f;
+        for (float v : g.ds) g.maxd = std::max(g.maxd, v);
+    }
+    return 0;
+}
+
+void orc_pool_sizes(const orc_scene *s, uint64_t *texels, uint64_t *voxels) {
+    *texels = s->texs.empty() ? 0 : s->texs.back().pool + s->texs.back().rgb.size();
+    *voxels = s->grids.empty() ? 0 : s->grids.back().pool + s->grids.back().d.size();
+}
+
+void orc_adjoint_f64(const orc_scene *s, const orc_camera_desc *cam, const orc_render_params *p,
+                     const double *pixel_grad, double *grad_texels, double *grad_voxels) {
+    adjoint_impl<double>(*s, *cam, *p, pixel_grad, grad_texels, grad_voxels);
 }
 
 uint32_t orc_hardware_threads(void) { return std::max(1u, std::thread::hardware_concurrency()); }

@@ -184,6 +184,22 @@ int32_t orc_bsdf_sample(const orc_material_desc *m, const float n[3], const floa
 /* BSDF::Eval and Pdf; out: eval rgb, pdf. */
 void orc_bsdf_eval(const orc_material_desc *m, const float n[3], const float wo[3], const float wi[3], float out[4]);
 
+/* ---- differentiable rendering (no reference counterpart) ------------------
+ * Parameters are the texel pool (textures in the order added, RGB floats) and
+ * the voxel pool (grids in the order added). */
+int32_t orc_scene_update_texture(orc_scene *s, int32_t id, const float *rgb);
+/* freeze_sampling = 1 keeps the densities (and majorant) the tracking
+ * decisions use and changes only the weights: the estimator is then a smooth
+ * function of the densities for fixed random numbers (what a central
+ * difference needs); 0 replaces both. */
+int32_t orc_scene_update_grid(orc_scene *s, int32_t id, const float *density, int32_t freeze_sampling);
+void orc_pool_sizes(const orc_scene *s, uint64_t *texels, uint64_t *voxels);
+/* Path-replay adjoint (FP64, Philox streams, params->rng ignored): adds
+ * dLoss/dparam into the pools given dLoss/d(pixel mean) (W*H*3). Russian
+ * roulette must be off (rr_start >= max_depth). */
+void orc_adjoint_f64(const orc_scene *s, const orc_camera_desc *cam, const orc_render_params *p,
+                     const double *pixel_grad, double *grad_texels, double *grad_voxels);
+
 uint32_t orc_hardware_threads(void);
 
 #ifdef __cplusplus
