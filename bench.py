@@ -54,6 +54,8 @@ def main():
     ap.add_argument("--variant", type=int, default=0, help="traversal kernel variant (0 automatic, 1..4)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rendering iteration timing")
+    ap.add_argument("--adjoint-iters", type=int, default=3)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -118,6 +120,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(desc, params, args.cpu_seconds)
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_bench(args.adjoint_iters)
 
     out = {
         "metric": "Msamples/s at fixed RMSE vs FP64 oracle; Mrays/s traversal; adjoint ms/iter",
@@ -154,8 +159,63 @@ def main():
         "mean_radiance": round(mean_radiance, 6),
         "scene_gen_s": round(gen_s, 2),
         "cpu_baseline": cpu,
+        "adjoint": adjoint,
     }
     print(json.dumps(out))
+
+
+def adjoint_bench(iters):
+    """Differentiable workload: 256x256, 64 spp, a 1024^2 x 3 albedo texture and
+    a 64^3 density grid; one iteration = forward render recording every
+    sample + L2 loss against a seed+1 target + path-replay adjoint."""
+    import torch
+
+    import lantern_amd
+    from lantern_amd import scenes
+    from lantern_amd.diff import DiffRenderer, render
+
+    d = scenes.diff_cube()
+    dt = scenes.diff_cube(seed=scenes.DEFAULT_SEED + 1)
+    with lantern_amd.Device(0) as dev:
+        st = lantern_amd.Scene(dev, dt)
+        rt = DiffRenderer(st, dt.camera, dt.params)
+        target = rt.forward(torch.tensor(dt.textures[0], device="cuda"),
+                            torch.tensor(dt.grids[0]["density"], device="cuda")).detach()
+        rt.close()
+        st.close()
+        gs = lantern_amd.Scene(dev, d)
+        r = DiffRenderer(gs, d.camera, d.params)
+        tex = torch.tensor(d.textures[0], device="cuda", requires_grad=True)
+        dens = torch.tensor(d.grids[0]["density"], device="cuda", requires_grad=True)
+
+        def step():
+            img = render(tex, dens, r)
+            loss = torch.mean((img - target) ** 2)
+            tex.grad = None
+            dens.grad = None
+            loss.backward()
+            torch.cuda.synchronize()
+            return float(loss)
+
+        step()
+        times = []
+        for _ in range(iters):
+            t0 = time.perf_counter()
+            loss = step()
+            times.append(time.perf_counter() - t0)
+        # forward and adjoint alone, by the library's own event timers
+        t0 = time.perf_counter()
+        img = r.forward(tex.detach(), dens.detach())
+        f_ms = (time.perf_counter() - t0) * 1e3
+        t0 = time.perf_counter()
+        r.backward(2.0 * (img - target) / img.numel())
+        b_ms = (time.perf_counter() - t0) * 1e3
+        r.close()
+        gs.close()
+    return {"ms_per_iter": round(1e3 * sorted(times)[len(times) // 2], 3), "forward_ms": round(f_ms, 3),
+            "adjoint_ms": round(b_ms, 3), "loss": loss,
+            "config": "diff_cube 256x256, 64 spp, max depth 6 (no RR), texture 1024x1024x3 FP32, grid 64^3 FP32; "
+                      "iteration = forward (record) + L2 loss + path-replay adjoint"}
 
 
 def copy_probe(nbytes):

@@ -235,7 +235,9 @@ LT_D bool grid_span(const DGrid &g, const V3 &o, const V3 &d, float tmin, float 
         t0 = n > t0 ? n : t0;
         t1 = f < t1 ? f : t1;
     }
-    return t0 < t1;
+    /* slivers (the segment only grazes the box within rounding of its face)
+     * count as empty, so the decision does not hang on the last bit */
+    return t1 - t0 > 1e-5f * (t1 > 1.0f ? t1 : 1.0f);
 }
 
 /* Ratio tracking (Novak et al. 2014): transmittance over [tmin, tmax] as the

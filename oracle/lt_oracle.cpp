@@ -1196,7 +1196,9 @@ template <class R> inline bool grid_span(const Grid &g, const V3<R> &o, const V3
         t0 = n > t0 ? n : t0;
         t1 = f < t1 ? f : t1;
     }
-    return t0 < t1;
+    /* slivers (the segment only grazes the box within rounding of its face)
+     * count as empty, so the decision does not hang on the last bit */
+    return t1 - t0 > R(1e-5f) * (t1 > R(1) ? t1 : R(1));
 }
 
 /* Ratio tracking (Novak et al. 2014) of the transmittance over [tmin, tmax]
