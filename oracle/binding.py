@@ -98,6 +98,8 @@ def lib():
             "orc_pool_sizes": (None, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
             "orc_adjoint_f64": (None, [C.c_void_p, C.POINTER(orc_camera_desc), C.POINTER(orc_render_params), _pd,
                                        _pd, _pd]),
+            "orc_adjoint_f32": (None, [C.c_void_p, C.POINTER(orc_camera_desc), C.POINTER(orc_render_params), _pf,
+                                       _pf, _pf]),
             "orc_hardware_threads": (C.c_uint32, []),
         }
         for name, (res, args) in sig.items():
@@ -219,12 +221,14 @@ class OracleScene:
         lib().orc_pool_sizes(self.h, C.byref(t), C.byref(v))
         return t.value, v.value
 
-    def adjoint(self, cam, params, pixel_grad, threads=0, frame0=0, stream=0):
-        """FP64 path-replay adjoint; pixel_grad (h, w, 3) = dLoss/d(pixel mean).
-        Returns (grad_texels, grad_voxels) over the parameter pools."""
+    def adjoint(self, cam, params, pixel_grad, threads=0, frame0=0, stream=0, precision=64):
+        """Path-replay adjoint (FP64, or FP32 with precision=32); pixel_grad
+        (h, w, 3) = dLoss/d(pixel mean). Returns (grad_texels, grad_voxels)
+        over the parameter pools."""
         nt, nv = self.pool_sizes()
-        gt = np.zeros(max(nt, 1), np.float64)
-        gv = np.zeros(max(nv, 1), np.float64)
+        dt = np.float64 if precision == 64 else np.float32
+        gt = np.zeros(max(nt, 1), dt)
+        gv = np.zeros(max(nv, 1), dt)
         p = orc_render_params()
         p.spp = params["spp"]
         p.max_depth = params.get("max_depth", 1500)
@@ -234,9 +238,12 @@ class OracleScene:
         p.rng = ORC_RNG_PHILOX
         p.threads = threads if threads > 0 else default_threads()
         p.stream = stream
-        g = np.ascontiguousarray(pixel_grad, np.float64)
-        lib().orc_adjoint_f64(self.h, C.byref(camera(cam)), C.byref(p), g.ctypes.data_as(_pd),
-                              gt.ctypes.data_as(_pd), gv.ctypes.data_as(_pd))
+        g = np.ascontiguousarray(pixel_grad, dt)
+        if precision == 64:
+            lib().orc_adjoint_f64(self.h, C.byref(camera(cam)), C.byref(p), g.ctypes.data_as(_pd),
+                                  gt.ctypes.data_as(_pd), gv.ctypes.data_as(_pd))
+        else:
+            lib().orc_adjoint_f32(self.h, C.byref(camera(cam)), C.byref(p), fp(g), fp(gt), fp(gv))
         return gt[:nt], gv[:nv]
 
     def render(self, cam, params, precision=32, rng=ORC_RNG_PHILOX, threads=0, window=None, frame0=0, stream=0):

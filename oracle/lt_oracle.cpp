@@ -1854,6 +1854,10 @@ void orc_adjoint_f64(const orc_scene *s, const orc_camera_desc *cam, const orc_r
                      const double *pixel_grad, double *grad_texels, double *grad_voxels) {
     adjoint_impl<double>(*s, *cam, *p, pixel_grad, grad_texels, grad_voxels);
 }
+void orc_adjoint_f32(const orc_scene *s, const orc_camera_desc *cam, const orc_render_params *p,
+                     const float *pixel_grad, float *grad_texels, float *grad_voxels) {
+    adjoint_impl<float>(*s, *cam, *p, pixel_grad, grad_texels, grad_voxels);
+}
 
 uint32_t orc_hardware_threads(void) { return std::max(1u, std::thread::hardware_concurrency()); }
 

@@ -199,6 +199,10 @@ void orc_pool_sizes(const orc_scene *s, uint64_t *texels, uint64_t *voxels);
  * roulette must be off (rr_start >= max_depth). */
 void orc_adjoint_f64(const orc_scene *s, const orc_camera_desc *cam, const orc_render_params *p,
                      const double *pixel_grad, double *grad_texels, double *grad_voxels);
+/* The same in FP32 (the GPU's arithmetic; texel and voxel addresses then
+ * round the same way). */
+void orc_adjoint_f32(const orc_scene *s, const orc_camera_desc *cam, const orc_render_params *p,
+                     const float *pixel_grad, float *grad_texels, float *grad_voxels);
 
 uint32_t orc_hardware_threads(void);
 

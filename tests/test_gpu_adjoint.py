@@ -3,8 +3,10 @@ FP64 adjoint, which is itself checked against central differences on the CPU.
 
 Both adjoints replay the same Philox paths, so the gradients agree to FP32
 rounding except for the rare path whose FP32 and FP64 versions take different
-branches. Bar: relative L2 error of the whole gradient <= 1e-3, and at least
-99 % of 256 random parameters within 1e-3 relative."""
+branches. Bars: at least 99 % of 256 random parameters within 1e-3 relative;
+projections on 8 random smooth fields within 1e-3; relative L2 error of the
+whole gradient <= 1e-3 against the FP64 adjoint at the small size and against
+the oracle's FP32 adjoint (the GPU's rounding) at the full size."""
 import numpy as np
 import pytest
 
@@ -32,13 +34,31 @@ def _gpu_adjoint(dev, d, pixel_grad):
     return out
 
 
-def _compare(g, ref, rng, n=256):
+def _smooth_fields(shape, rng, k=8):
+    """k random low-frequency fields over a parameter array of this shape."""
+    out = []
+    axes = [np.linspace(0.0, 1.0, n) for n in shape]
+    for _ in range(k):
+        f = np.ones(shape)
+        for a, x in enumerate(axes):
+            w = rng.uniform(0.5, 3.0)
+            ph = rng.uniform(0, 2 * np.pi)
+            sh = [1] * len(shape)
+            sh[a] = len(x)
+            f = f * (1.5 + np.cos(2 * np.pi * w * x + ph)).reshape(sh)
+        out.append(f.reshape(-1))
+    return out
+
+
+def _compare(g, ref, rng, shape, rel_l2_max=1e-3, n=256):
     rel_l2 = np.linalg.norm(g - ref) / np.linalg.norm(ref)
     idx = rng.choice(np.flatnonzero(ref != 0), min(n, np.count_nonzero(ref)), replace=False)
     ok = np.abs(g[idx] - ref[idx]) <= 1e-3 * np.abs(ref[idx]) + 1e-7 * np.abs(ref).max()
-    print(f"relative L2 {rel_l2:.2e}, params within 1e-3: {ok.mean():.4f}")
-    assert rel_l2 <= 1e-3
+    proj = [abs(np.dot(g, f) - np.dot(ref, f)) / abs(np.dot(ref, f)) for f in _smooth_fields(shape, rng)]
+    print(f"relative L2 {rel_l2:.2e}, params within 1e-3: {ok.mean():.4f}, smooth projections {max(proj):.2e}")
+    assert rel_l2 <= rel_l2_max
     assert ok.mean() >= 0.99
+    assert max(proj) <= 1e-3
 
 
 @pytest.mark.parametrize("size", ["small", "full"])
@@ -60,8 +80,18 @@ def test_adjoint_matches_fp64_oracle(gpu_device, size):
     img_g, gt_g, gv_g = _gpu_adjoint(gpu_device, d, pixel_grad)
     assert np.abs(img_g - img_o).mean() <= 1e-4 * np.abs(img_o).mean() + 1e-6
     rng = np.random.default_rng(5)
-    _compare(gt_g, gt_o, rng)
-    _compare(gv_g, gv_o, rng)
+    tshape, gshape = d.textures[0].shape, d.grids[0]["density"].shape
+    # Against FP64: at 1024^2 texels a hit whose u*W lies within FP32 rounding
+    # of a texel edge lands in the neighbouring texel (the image does not see
+    # it; that texel's gradient entry does), so the whole-vector L2 bar holds
+    # at the small size only; random parameters and smooth projections hold
+    # at both.
+    _compare(gt_g, gt_o, rng, tshape, 1e-3 if size == "small" else 0.1)
+    _compare(gv_g, gv_o, rng, gshape, 1e-3 if size == "small" else 0.1)
+    # Against the FP32 restatement (same rounding): the whole vectors agree.
+    gt_f, gv_f = oc.adjoint(d.camera, d.params, pixel_grad.astype(np.float32), precision=32)
+    _compare(gt_g, gt_f.astype(np.float64), rng, tshape)
+    _compare(gv_g, gv_f.astype(np.float64), rng, gshape)
 
 
 def test_autograd_descent_reduces_the_loss(gpu_device):
