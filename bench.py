@@ -25,6 +25,15 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 RAY_BYTES = 48
 
+# HBM bytes per launch of the closest-hit traversal kernel, measured by
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (one pass each) on this workload
+# and scaled by a 1 GiB copy probe (FETCH_SIZE x2.0, WRITE_SIZE x1.0); the
+# commands and raw figures are in profiles/k_trace8_triangle_heavy.md.
+# Keyed by (workload, layout, width, height, spp).
+TRAFFIC = {
+    ("triangle_heavy", 0, 1280, 720, 64): round((166734 * 2.0 + 48742 * 1.0) * 1024),
+}
+
 WORKLOADS = {
     "triangle_heavy": {"size": (1280, 720, 64),
                        "text": "triangle_heavy: displaced icosphere subdiv 7 (327,680 tris) + floor + quad light, "
@@ -32,6 +41,10 @@ WORKLOADS = {
     "volume_box": {"size": (512, 512, 256),
                    "text": "volume_box: 128^3 fBm5 density (sigma_t = 5 d, albedo 0.9, HG g=0.3) in an IOR 1.5 "
                            "CreateBox, floor + quad light, max depth 32, delta tracking"},
+    "batch": {"size": (0, 0, 32),
+              "text": "batch: 64 scenes, icosphere subdiv 5/6/7 (20,480-327,680 tris) + floor + quad light, "
+                      "Lambert/GGX/dielectric, W,H ~ U{64..512}, 32 spp, max depth 16, RR from depth 4, "
+                      "one launch sequence (lt_render_batch)"},
     "nine_sphere": {"size": (512, 512, 16),
                     "text": "nine_sphere: 9 x geosphere n=4 (46,080 tris) + ground + quad light, Lambert, "
                             "max depth 8"},
@@ -72,6 +85,9 @@ def main():
     import lantern_amd
     from lantern_amd import scenes
 
+    if args.workload == "batch":
+        print(json.dumps(bench_batch(args)))
+        return
     wl = WORKLOADS[args.workload]
     args.width = args.width or wl["size"][0]
     args.height = args.height or wl["size"][1]
@@ -108,12 +124,14 @@ def main():
     samples = sum(st["samples"] for st in per_step)
     closest = sum(st["closest_rays"] for st in per_step)
     shadow = sum(st["shadow_rays"] for st in per_step)
-    trav_ms = sum(st["extend_ms"] + st["shadow_ms"] for st in per_step)
-    launches = sum(st["extend_launches"] + st["shadow_launches"] for st in per_step)
+    ext_ms = sum(st["extend_ms"] for st in per_step)
+    ext_launches = sum(st["extend_launches"] for st in per_step)
     gpu_ms = sum(st["total_ms"] for st in per_step)
     msps = samples / wall / 1e6
     rays = closest + shadow
-    achieved = rays * RAY_BYTES / (trav_ms / 1e3) / 1e9 if trav_ms > 0 else 0.0
+    # dominant kernel: the closest-hit traversal (extension rays), 48 B per ray
+    achieved = closest * RAY_BYTES / (ext_ms / 1e3) / 1e9 if ext_ms > 0 else 0.0
+    traffic = TRAFFIC.get((args.workload, args.layout, args.width, args.height, args.spp))
     breakdown = {k: round(sum(st[k] for st in per_step) / args.steps, 3)
                  for k in ("generate_ms", "extend_ms", "shade_ms", "shadow_ms", "resolve_ms", "accumulate_ms")}
 
@@ -146,10 +164,13 @@ def main():
         "gpu_ms_per_step": round(gpu_ms / args.steps, 3),
         "kernel_ms_per_step": breakdown,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "k_trace (closest hit: extension + next-event rays)",
-                     "bytes_per_ray": RAY_BYTES, "launches_per_step": launches // args.steps,
-                     "avg_launch_ms": round(trav_ms / max(launches, 1), 4)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "kernel": ("k_trace8" if info["layout"] == 0 else "k_trace") + " closest hit (extension rays)",
+                     "bytes_per_ray": RAY_BYTES, "launches_per_step": ext_launches // args.steps,
+                     "rays_per_launch": round(closest / max(ext_launches, 1)),
+                     "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4)},
+        "closest_rays_per_step": closest // args.steps,
+        "shadow_rays_per_step": shadow // args.steps,
         "bvh": {"layout": "wide8" if info["layout"] == 0 else "binary",
                 "builder": "ploc" if args.builder == 0 else "lbvh", "build_iterations": info["build_iterations"],
                 "wide_nodes": info["wide_nodes"],
@@ -162,6 +183,79 @@ def main():
         "adjoint": adjoint,
     }
     print(json.dumps(out))
+
+
+def bench_batch(args):
+    import numpy as np
+
+    import lantern_amd
+    from lantern_amd import scenes
+
+    t0 = time.time()
+    descs = scenes.batch_scenes(64, spp=args.spp or 32)
+    gen_s = time.time() - t0
+    params = dict(descs[0].params)
+    params.update(wave_paths=args.wave_paths, block_size=args.block, trace_variant=args.variant)
+    with lantern_amd.Device(0) as dev:
+        tb = time.time()
+        ss = [lantern_amd.Scene(dev, d, layout=args.layout, builder=args.builder) for d in descs]
+        build_s = time.time() - tb
+        cams = [d.camera for d in descs]
+        films = [lantern_amd.Film(dev, c["width"], c["height"]) for c in cams]
+
+        def one(seq):
+            for f in films:
+                f.reset()
+            return lantern_amd.render_batch(ss, cams, params, films, sequential=seq)
+
+        for _ in range(args.warmup):
+            one(False)
+        dev.synchronize()
+        per_step = []
+        wall0 = time.perf_counter()
+        for _ in range(args.steps):
+            per_step.append(one(False))
+        dev.synchronize()
+        wall = time.perf_counter() - wall0
+        # the same work scene by scene, for comparison
+        one(True)
+        dev.synchronize()
+        ws0 = time.perf_counter()
+        one(True)
+        dev.synchronize()
+        wall_seq = time.perf_counter() - ws0
+        for f in films:
+            s, b, c = f.download()
+            assert np.all(np.isfinite(s)) and int(c.min()) == params["spp"]
+        for f in films:
+            f.close()
+        for sc in ss:
+            sc.close()
+    samples = sum(st["samples"] for st in per_step)
+    closest = sum(st["closest_rays"] for st in per_step)
+    shadow = sum(st["shadow_rays"] for st in per_step)
+    ext_ms = sum(st["extend_ms"] for st in per_step)
+    ext_launches = sum(st["extend_launches"] for st in per_step)
+    achieved = closest * RAY_BYTES / (ext_ms / 1e3) / 1e9 if ext_ms > 0 else 0.0
+    return {
+        "metric": "Msamples/s at fixed RMSE vs FP64 oracle; Mrays/s traversal; adjoint ms/iter",
+        "value": round(samples / wall / 1e6, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": WORKLOADS["batch"]["text"], "scenes": len(descs), "spp": params["spp"],
+                   "samples_per_step": samples // args.steps, "triangles": sum(d.num_triangles for d in descs),
+                   "seed": params["seed"]},
+        "mrays_per_s": round((closest + shadow) / wall / 1e6, 2),
+        "sequential_ms_per_step": round(wall_seq * 1e3, 3),
+        "batch_speedup_vs_sequential": round(wall_seq / (wall / args.steps), 3),
+        "waves_per_step": per_step[0]["waves"],
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "k_trace8 closest hit, batched (extension rays)", "bytes_per_ray": RAY_BYTES,
+                     "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4)},
+        "scene_gen_s": round(gen_s, 2), "bvh_build_s": round(build_s, 2),
+        "cpu_baseline": None,
+    }
 
 
 def adjoint_bench(iters):

@@ -245,9 +245,18 @@ typedef struct lt_render_stats {
  * must be width x height of the camera. stats may be NULL. Synchronous. */
 int lt_render(lt_scene *scene, const lt_camera *camera, const lt_render_params *params, lt_film *film,
               lt_render_stats *stats);
-/* Many scenes with their own cameras and films, one launch sequence. */
+/* Many scenes with their own cameras and films in one launch sequence: the
+ * (scene, sample) ranges are packed into waves of up to params->wave_paths
+ * paths, traversal and shading take each path's scene from a device table.
+ * Scene i draws from Philox stream params->stream + i, so its film equals
+ * lt_render of that scene alone with that stream. All scenes must share one
+ * BVH layout. Replaces a loop of Integrator::RenderFrame over several
+ * Lantern scenes (integrator.cpp:28-43), which the reference runs one by one. */
 int lt_render_batch(lt_scene *const *scenes, const lt_camera *cameras, lt_film *const *films, uint32_t n,
                     const lt_render_params *params, lt_render_stats *stats);
+/* The same result as lt_render_batch by n separate lt_render calls. */
+int lt_render_each(lt_scene *const *scenes, const lt_camera *cameras, lt_film *const *films, uint32_t n,
+                   const lt_render_params *params, lt_render_stats *stats);
 
 /* ---- differentiable rendering ------------------------------------------------
  * No counterpart in the reference (its renderer is not differentiable). The

@@ -14,7 +14,7 @@ from ._capi import (LanternError, LT_BSDF_DIELECTRIC, LT_BSDF_GGX, LT_BSDF_LAMBE
                     LT_TEX_UV, check, lib)
 from .scenes import SceneDesc
 
-__all__ = ["Device", "Scene", "Film", "render", "SceneDesc", "LanternError", "camera_struct", "params_struct",
+__all__ = ["Device", "Scene", "Film", "render", "render_batch", "SceneDesc", "LanternError", "camera_struct", "params_struct",
            "LT_INVALID_ID"]
 
 
@@ -210,6 +210,21 @@ def params_struct(p):
     r.use_graph = p.get("use_graph", 0)
     r.trace_variant = p.get("trace_variant", 0)
     return r
+
+
+def render_batch(scenes, cameras, params, films, sequential=False):
+    """Several scenes in one launch sequence (lt_render_batch); scene i uses
+    Philox stream params["stream"] + i. sequential=True renders them one by
+    one instead (lt_render_each), with identical results."""
+    n = len(scenes)
+    st = _capi.lt_render_stats()
+    hs = (C.c_void_p * n)(*[s.h.value for s in scenes])
+    fs = (C.c_void_p * n)(*[f.h.value for f in films])
+    cams = (_capi.lt_camera * n)(*[camera_struct(c) for c in cameras])
+    prm = params_struct(params)
+    fn = lib().lt_render_each if sequential else lib().lt_render_batch
+    check(fn(hs, cams, fs, n, C.byref(prm), C.byref(st)), "lt_render_batch")
+    return {k: getattr(st, k) for k, _ in st._fields_}
 
 
 def render(scene, camera, params, film):

@@ -123,6 +123,8 @@ SIGNATURES = {
                             C.POINTER(lt_render_stats)]),
     "lt_render_batch": (C.c_int, [C.POINTER(_vp), C.POINTER(lt_camera), C.POINTER(_vp), _u32,
                                   C.POINTER(lt_render_params), C.POINTER(lt_render_stats)]),
+    "lt_render_each": (C.c_int, [C.POINTER(_vp), C.POINTER(lt_camera), C.POINTER(_vp), _u32,
+                                 C.POINTER(lt_render_params), C.POINTER(lt_render_stats)]),
     "lt_geometry_geosphere": (_u32, [_f, _u32, _pf, _pf, _pu32, _pu32]),
     "lt_geometry_grid": (_u32, [_f, _f, _u32, _u32, _pf, _pf, _pf, _pu32, _pu32]),
     "lt_geometry_box": (_u32, [_f, _f, _f, _pf, _pf, _pf, _pu32, _pu32]),

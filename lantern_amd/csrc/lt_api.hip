@@ -19,6 +19,8 @@ int render_record(lt_scene *scene, const lt_camera &cam, const lt_render_params 
                   float *sample_radiance, lt_render_stats *stats);
 int render_adjoint(lt_scene *scene, const lt_camera &cam, const lt_render_params &prm, const float *sample_radiance,
                    const float *pixel_grad, float *grad_texels, float *grad_voxels, lt_render_stats *stats);
+int render_batch(lt_scene *const *scenes, const lt_camera *cams, lt_film *const *films, uint32_t nscenes,
+                 const lt_render_params &prm, lt_render_stats *stats);
 
 namespace {
 thread_local std::string g_error = "no error";
@@ -770,6 +772,17 @@ int lt_render(lt_scene *s, const lt_camera *cam, const lt_render_params *prm, lt
 int lt_render_batch(lt_scene *const *scenes, const lt_camera *cams, lt_film *const *films, uint32_t n,
                     const lt_render_params *prm, lt_render_stats *stats) {
     LT_CHECK(scenes && cams && films && prm, "lt_render_batch: NULL argument");
+    for (uint32_t i = 0; i < n; ++i) LT_CHECK(scenes[i] && films[i], "lt_render_batch: NULL scene or film");
+    if (n == 0) return LT_OK;
+    LT_HIP(hipSetDevice(scenes[0]->dev->ordinal));
+    return render_batch(scenes, cams, films, n, *prm, stats);
+}
+
+/* One lt_render per scene (stream + i): the sequential equivalent of
+ * lt_render_batch, kept for comparison. */
+int lt_render_each(lt_scene *const *scenes, const lt_camera *cams, lt_film *const *films, uint32_t n,
+                   const lt_render_params *prm, lt_render_stats *stats) {
+    LT_CHECK(scenes && cams && films && prm, "lt_render_each: NULL argument");
     lt_render_stats acc;
     std::memset(&acc, 0, sizeof acc);
     for (uint32_t i = 0; i < n; ++i) {

@@ -67,14 +67,20 @@ int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const 
  * occupancy target): 0 = (16, compiler), 1 = (8, compiler), 2 = (8, 6 waves),
  * 3 = (8, 8 waves). The binary layout ignores the variant. */
 constexpr int kTraceVariants = 4;
+/* Rays of several scenes in one traversal launch: ray i belongs to scene
+ * ray_scene[i] of the device table scenes (all with the same layout). */
+struct TraceBatch {
+    const DScene *scenes;
+    const uint32_t *ray_scene;
+};
 constexpr int kDefaultTraceVariant = 0;
 int launch_trace_closest(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, const uint32_t *count_ptr,
                          uint32_t count, uint32_t *work, int block, int grid, int *spill, uint32_t *overflow,
-                         int variant);
+                         int variant, const TraceBatch *batch = nullptr);
 /* ignore_geom (nullable): per ray, a geomID whose triangles do not occlude. */
 int launch_trace_any(hipStream_t st, const DScene &sc, const Ray *rays, uint32_t *occluded,
                      const uint32_t *ignore_geom, const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block,
-                     int grid, int *spill, uint32_t *overflow, int variant);
+                     int grid, int *spill, uint32_t *overflow, int variant, const TraceBatch *batch = nullptr);
 int trace_grid(int num_cus, int block, int variant);
 /* Transmittance of each ray's [tnear, tfar] through grid medium md: method 0
  * ratio tracking, 1 delta tracking. */
@@ -119,6 +125,7 @@ struct lt_device {
     lt::DevBuf trace_work;   /* work counter for the intersect entry points */
     lt::DevBuf spill;        /* traversal stack overflow area */
     lt::DevBuf diag;         /* [0]: rays that hit a traversal bound (stack/steps) */
+    lt::DevBuf batch_tables; /* scene, camera, film and segment tables of a batch render */
     uint32_t *pinned = nullptr;
 };
 

@@ -68,10 +68,33 @@ struct PathState {
     uint32_t *rngi;           /* draws consumed */
     uint32_t *bounces;        /* final bounce count */
 };
-/* One queue of rays (indexed by slot) with their owners. */
+/* One queue of rays (indexed by slot) with their owners (and, in a batch,
+ * the scene of each ray). */
 struct Queue {
     Ray *rays;
     uint32_t *path;
+    uint32_t *sid;
+};
+
+/* A batch wave is a list of segments: consecutive samples [s0, s0 + ns) of
+ * one scene, laid out sample-major from path index path_off (and pixel slot
+ * pix_off for the film accumulation). */
+struct Seg {
+    uint32_t scene, s0, ns, npix, w, path_off, pix_off, pad;
+};
+struct FilmPtrs {
+    float *sum;
+    uint32_t *bounces, *count;
+};
+struct BatchArgs {
+    const DScene *scenes;
+    const CameraArgs *cams;
+    const FilmPtrs *films;
+    const Seg *segs;           /* this wave's segments, ascending path_off */
+    uint32_t nsegs;
+    uint32_t npix_slots;       /* sum of the segments' npix */
+    uint32_t *path_seg;        /* per path: its segment */
+    uint32_t *nee_sid;         /* per compacted next-event ray: its scene */
 };
 /* Next-event record, indexed by the shaded slot. */
 struct NeeRec {
@@ -99,6 +122,7 @@ struct AdjArgs {
 struct WaveArgs {
     DScene sc;
     AdjArgs adj;
+    BatchArgs batch;
     CameraArgs cam;
     PathState ps;
     uint32_t npix;            /* pixels per sample */
@@ -110,12 +134,43 @@ struct WaveArgs {
     uint32_t rr_start;
 };
 
-__device__ __forceinline__ void path_coords(const WaveArgs &a, uint32_t p, uint32_t &x, uint32_t &y, uint32_t &frame) {
-    uint32_t s = p / a.npix;
-    uint32_t pix = p - s * a.npix;
-    y = pix / a.cam.w;
-    x = pix - y * a.cam.w;
-    frame = a.frame0 + a.sample_base + s;
+struct PathInfo {
+    uint32_t x, y, frame, scene;
+};
+
+/* Segment of path p: the last one starting at or before p. */
+__device__ __forceinline__ uint32_t seg_of(const BatchArgs &b, uint32_t p) {
+    uint32_t lo = 0, hi = b.nsegs;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (b.segs[mid].path_off <= p) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+/* Pixel, frame and scene of path p (seg: its segment in a batch). */
+template <bool BATCH>
+__device__ __forceinline__ PathInfo path_info(const WaveArgs &a, uint32_t p, uint32_t seg) {
+    PathInfo r;
+    if (BATCH) {
+        const Seg g = a.batch.segs[seg];
+        uint32_t local = p - g.path_off;
+        uint32_t s = local / g.npix;
+        uint32_t pix = local - s * g.npix;
+        r.y = pix / g.w;
+        r.x = pix - r.y * g.w;
+        r.frame = a.frame0 + g.s0 + s;
+        r.scene = g.scene;
+    } else {
+        uint32_t s = p / a.npix;
+        uint32_t pix = p - s * a.npix;
+        r.y = pix / a.cam.w;
+        r.x = pix - r.y * a.cam.w;
+        r.frame = a.frame0 + a.sample_base + s;
+        r.scene = 0;
+    }
+    return r;
 }
 
 /* ReconstructionFilter::Sample (reconstruction_filter.cpp:34-87). */
@@ -136,21 +191,27 @@ __device__ __forceinline__ float filter_sample(const CameraArgs &c, float x) {
     return neg ? -u : u;
 }
 
-template <bool ADJ>
+template <bool ADJ, bool BATCH>
 __global__ void k_generate(WaveArgs a, Queue q, uint32_t n, Ctl *ctl) {
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        uint32_t x, y, frame;
-        path_coords(a, p, x, y, frame);
+        const uint32_t seg = BATCH ? seg_of(a.batch, p) : 0u;
+        const PathInfo pi = path_info<BATCH>(a, p, seg);
+        const uint32_t x = pi.x, y = pi.y;
+        const CameraArgs &cam = BATCH ? a.batch.cams[pi.scene] : a.cam;
+        if (BATCH) {
+            a.batch.path_seg[p] = seg;
+            q.sid[p] = pi.scene;
+        }
         PathRng rng;
-        rng.init(x, y, a.stream, a.seed, frame, 0);
+        rng.init(x, y, a.stream + pi.scene, a.seed, pi.frame, 0);
         float fu = rng.next_float(), fv = rng.next_float();
-        float u = filter_sample(a.cam, fu), v = filter_sample(a.cam, fv);
-        V3 view = v3((((float)x + 0.5f + u) / (float)a.cam.w * 2.0f - 1.0f) * a.cam.tanx,
-                     -((((float)y + 0.5f + v) / (float)a.cam.h) * 2.0f - 1.0f) * a.cam.tany, -1.0f);
-        V3 mx = v3(a.cam.mx[0], a.cam.mx[1], a.cam.mx[2]);
-        V3 my = v3(a.cam.my[0], a.cam.my[1], a.cam.my[2]);
-        V3 mz = v3(a.cam.mz[0], a.cam.mz[1], a.cam.mz[2]);
-        float4 r0 = make_float4(a.cam.origin[0], a.cam.origin[1], a.cam.origin[2], 0.0f);
+        float u = filter_sample(cam, fu), v = filter_sample(cam, fv);
+        V3 view = v3((((float)x + 0.5f + u) / (float)cam.w * 2.0f - 1.0f) * cam.tanx,
+                     -((((float)y + 0.5f + v) / (float)cam.h) * 2.0f - 1.0f) * cam.tany, -1.0f);
+        V3 mx = v3(cam.mx[0], cam.mx[1], cam.mx[2]);
+        V3 my = v3(cam.my[0], cam.my[1], cam.my[2]);
+        V3 mz = v3(cam.mz[0], cam.mz[1], cam.mz[2]);
+        float4 r0 = make_float4(cam.origin[0], cam.origin[1], cam.origin[2], 0.0f);
         float4 r1 = make_float4(dot(view, mx), dot(view, my), dot(view, mz), __builtin_huge_valf());
         float4 *rp = (float4 *)(q.rays + p);
         rp[0] = r0;
@@ -215,7 +276,7 @@ __device__ __forceinline__ V3 adj_weight(const WaveArgs &a, uint32_t pix, const 
  * ADJ: the replay of the adjoint pass; contributions are subtracted from the
  * radiance still to come instead of added, and parameter factors scatter
  * their gradients. */
-template <bool ADJ>
+template <bool ADJ, bool BATCH>
 __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next,
                                                NeeRec nee, Ray *nee_rays, Hit *nee_hits, uint32_t *nee_ignore,
                                                uint32_t *nee_cidx, Ctl *ctl) {
@@ -223,7 +284,6 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    const DScene &sc = a.sc;
     const float inf = __builtin_huge_valf();
     for (uint32_t base = wave * 64; base < count; base += nwaves * 64) {
         const uint32_t q = base + lane;
@@ -234,13 +294,16 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
         V3 new_o = v3s(0.0f), new_d = v3s(0.0f);
         V3 P = v3s(0.0f), wi1 = v3s(0.0f), wi2 = v3s(0.0f);
         uint32_t ignore_geom = 0xFFFFFFFFu;
+        uint32_t scene = 0;
         if (q < count) {
             p = cur.path[q];
-            uint32_t x, y, frame;
-            path_coords(a, p, x, y, frame);
+            const PathInfo pi = path_info<BATCH>(a, p, BATCH ? a.batch.path_seg[p] : 0u);
+            const uint32_t x = pi.x, y = pi.y;
+            scene = pi.scene;
+            const DScene &sc = BATCH ? a.batch.scenes[scene] : a.sc;
             const uint32_t pix = y * a.cam.w + x;
             PathRng rng;
-            rng.init(x, y, a.stream, a.seed, frame, a.ps.rngi[p]);
+            rng.init(x, y, a.stream + scene, a.seed, pi.frame, a.ps.rngi[p]);
             const float4 *rp = (const float4 *)(cur.rays + q);
             float4 ra = rp[0], rb = rp[1];
             const Hit h = hits[q];
@@ -453,6 +516,7 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
             uint32_t off = (uint32_t)__popcll(mask & below);
             uint32_t s = slot0 + off;
             next.path[s] = p;
+            if (BATCH) next.sid[s] = scene;
             store_ray(next.rays + s, new_o, new_d, 0.001f, inf);
         }
         if (nee1 || nee2) {
@@ -460,12 +524,14 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
             if (nee1) {
                 store_ray(nee_rays + c, P, wi1, 0.001f, tfar1);
                 nee_ignore[c] = ignore_geom;
+                if (BATCH) a.batch.nee_sid[c] = scene;
                 nee_cidx[2 * (size_t)q] = c;
                 ++c;
             }
             if (nee2) {
                 store_ray(nee_rays + c, P, wi2, 0.001f, tfar2);
                 nee_ignore[c] = ignore_geom;
+                if (BATCH) a.batch.nee_sid[c] = scene;
                 nee_cidx[2 * (size_t)q + 1] = c;
             }
         }
@@ -474,15 +540,16 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
 
 /* The casts of AreaLight::SampleLi / PdfLi resolved (area_light.cpp:58-71,
  * 100-109) and the MIS sums of EstimateDirect (integrator.cpp:287-314). */
-template <bool ADJ>
+template <bool ADJ, bool BATCH>
 __global__ __launch_bounds__(256) void k_resolve(WaveArgs a, uint32_t b, Queue cur, NeeRec nee, const Ray *nee_rays,
                                                  const Hit *nee_hits, const Hit *nee_trace_hits,
                                                  const uint32_t *nee_occl, const uint32_t *nee_cidx, Ctl *ctl) {
     const uint32_t count = ctl->count[b];
-    const DScene &sc = a.sc;
+    
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
         uint32_t info = nee.info[q];
         if (!(info & 0xC0000000u)) continue;
+        const DScene &sc = BATCH ? a.batch.scenes[cur.sid[q]] : a.sc;
         uint32_t li = info & 0x3FFFFFFFu;
         const DLight L = sc.lights[li];
         V3 le = v3(L.le[0], L.le[1], L.le[2]);
@@ -564,6 +631,36 @@ __global__ void k_accumulate(WaveArgs a, uint32_t nsamples, float *sum, uint32_t
     }
 }
 
+/* The same for a batch wave: one thread per pixel slot of every segment. */
+__global__ void k_accumulate_batch(WaveArgs a) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < a.batch.npix_slots; j += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = a.batch.nsegs;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (a.batch.segs[mid].pix_off <= j) lo = mid;
+            else hi = mid;
+        }
+        const Seg g = a.batch.segs[lo];
+        const uint32_t pix = j - g.pix_off;
+        const FilmPtrs f = a.batch.films[g.scene];
+        float sx = f.sum[3 * (size_t)pix], sy = f.sum[3 * (size_t)pix + 1], sz = f.sum[3 * (size_t)pix + 2];
+        uint32_t bb = f.bounces[pix], cc = f.count[pix];
+        for (uint32_t s = 0; s < g.ns; ++s) {
+            size_t p = (size_t)g.path_off + (size_t)s * g.npix + pix;
+            sx = sx + a.ps.cx[p];
+            sy = sy + a.ps.cy[p];
+            sz = sz + a.ps.cz[p];
+            bb += a.ps.bounces[p];
+            cc += 1u;
+        }
+        f.sum[3 * (size_t)pix] = sx;
+        f.sum[3 * (size_t)pix + 1] = sy;
+        f.sum[3 * (size_t)pix + 2] = sz;
+        f.bounces[pix] = bb;
+        f.count[pix] = cc;
+    }
+}
+
 __global__ void k_reset_ctl(Ctl *ctl) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 2048) {
@@ -578,7 +675,8 @@ __global__ void k_reset_ctl(Ctl *ctl) {
 constexpr size_t kPathBytes = 4 * 11;                 /* path state */
 constexpr size_t kQueueBytes = sizeof(Ray) + 4;       /* per queue slot */
 constexpr size_t kNeeBytes = 4 * 14 + 2 * sizeof(Ray) + 4 * sizeof(Hit) + 2 * 4 + 2 * 4 + 2 * 4;
-constexpr size_t kBytesPerPath = kPathBytes + 2 * kQueueBytes + sizeof(Hit) + kNeeBytes;
+constexpr size_t kBatchBytes = 4 + 2 * 4 + 2 * 4;       /* path_seg, queue sids, next-event sids */
+constexpr size_t kBytesPerPath = kPathBytes + 2 * kQueueBytes + sizeof(Hit) + kNeeBytes + kBatchBytes;
 
 struct Workspace {
     PathState ps;
@@ -591,10 +689,12 @@ struct Workspace {
     uint32_t *nee_ignore;    /* per compacted next-event ray: the light's geomID */
     Hit *nee_trace_hits;     /* per compacted next-event ray (closest-hit mode) */
     uint32_t *nee_cidx;      /* per (slot, ray): index of its compacted ray */
+    uint32_t *path_seg;      /* batch: per path, its segment */
+    uint32_t *nee_sid;       /* batch: per compacted next-event ray, its scene */
 };
 
 int carve(lt_device *dev, size_t n, Workspace &w) {
-    size_t bytes = n * kBytesPerPath + 64 * 64;
+    size_t bytes = n * kBytesPerPath + 64 * 256;
     if (dev->ws_paths < n) {
         int rc = dev->ws.alloc(bytes);
         if (rc) return rc;
@@ -627,6 +727,10 @@ int carve(lt_device *dev, size_t n, Workspace &w) {
     w.nee_ignore = (uint32_t *)take(2 * 4);
     w.nee_trace_hits = (Hit *)take(2 * sizeof(Hit));
     w.nee_cidx = (uint32_t *)take(2 * 4);
+    w.qa.sid = (uint32_t *)take(4);
+    w.qb.sid = (uint32_t *)take(4);
+    w.path_seg = (uint32_t *)take(4);
+    w.nee_sid = (uint32_t *)take(2 * 4);
     return LT_OK;
 }
 
@@ -708,45 +812,33 @@ void camera_setup(const lt_camera &c, CameraSetup &o) {
     o.cdf[32] = 1.0f;
 }
 
-/* Renders one scene; see the file comment for the kernel sequence. With
- * replay set, runs the adjoint pass instead (no film). */
-int render_impl(lt_scene *scene, const lt_camera &cam, const lt_render_params &prm, lt_film *film,
-                lt_render_stats *stats, const AdjArgs &adj, bool replay) {
-    lt_device *dev = scene->dev;
+namespace {
+
+/* One wave of the launch sequence. */
+struct WavePlan {
+    uint32_t n;             /* paths */
+    uint32_t sample_base;   /* single scene: first sample of the wave */
+    uint32_t ns;            /* single scene: samples in the wave */
+    uint32_t seg0, nsegs;   /* batch: the wave's segments in the plan table */
+    uint32_t npix_slots;    /* batch: pixel slots of those segments */
+};
+
+/* The wavefront launch sequence over all planned waves; see the file comment.
+ * ADJ: the adjoint replay (no film). BATCH: several scenes per wave. */
+template <bool ADJ, bool BATCH>
+int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, const Seg *d_segs,
+              const lt_render_params &prm, lt_film *film, lt_render_stats *stats, Workspace &w) {
     hipStream_t st = dev->stream;
-    const uint32_t npix = cam.width * cam.height;
-    if (npix == 0 || prm.spp == 0) return LT_OK;
-    if (prm.max_depth == 0 || prm.max_depth > 2000) return set_error(LT_E_INVALID, "max_depth must be in 1..2000");
-    size_t budget = prm.wave_paths ? prm.wave_paths : (size_t)16 << 20;
-    uint32_t spw = (uint32_t)std::max<size_t>(1, std::min<size_t>(prm.spp, budget / npix));
-    size_t nmax = (size_t)spw * npix;
-    if (nmax > 0xFFFFFFFFull / 2) return set_error(LT_E_INVALID, "wave too large");
-    Workspace w;
-    int rc = carve(dev, nmax, w);
-    if (rc) return rc;
+    int rc;
     if ((rc = dev->ctl.reserve(sizeof(Ctl)))) return rc;
     Ctl *ctl = (Ctl *)dev->ctl.p;
     const int tblock = prm.block_size ? (int)prm.block_size : trace_default_block();
     const int tvar = prm.trace_variant ? (int)prm.trace_variant - 1 : kDefaultTraceVariant;
+    if (tvar < 0 || tvar >= kTraceVariants) return set_error(LT_E_INVALID, "unknown traversal variant");
     const int tgrid = trace_grid(dev->num_cus, tblock, tvar);
-
     if ((rc = dev->spill.reserve(trace_spill_bytes(tgrid, tblock)))) return rc;
     int *spill = (int *)dev->spill.p;
-
-    CameraSetup cs;
-    camera_setup(cam, cs);
-    WaveArgs a;
-    a.sc = scene->ds;
-    a.adj = adj;
-    a.adj.inv_spp = 1.0f / (float)prm.spp;
-    std::memcpy(&a.cam, &cs, sizeof(CameraArgs));
-    a.ps = w.ps;
-    a.npix = npix;
-    a.frame0 = prm.frame0;
-    a.seed = prm.seed;
-    a.stream = prm.stream;
-    a.max_depth = prm.max_depth;
-    a.rr_start = prm.rr_start;
+    uint32_t *diag = dev->diag.as<uint32_t>();
 
     const int sgrid = std::max(1, dev->num_cus) * 8;
     EventTimer tm(stats != nullptr, st);
@@ -754,21 +846,22 @@ int render_impl(lt_scene *scene, const lt_camera &cam, const lt_render_params &p
     LT_HIP(hipEventCreate(&e0));
     LT_HIP(hipEventCreate(&e1));
     LT_HIP(hipEventRecord(e0, st));
-    uint64_t closest = 0, shadow = 0, segments = 0;
+    uint64_t closest = 0, shadow = 0, segments = 0, samples = 0;
     uint32_t ext_launches = 0, sh_launches = 0, waves = 0;
-    for (uint32_t s0 = 0; s0 < prm.spp; s0 += spw) {
-        uint32_t ns = std::min(spw, prm.spp - s0);
-        uint32_t n = ns * npix;
-        a.sample_base = s0;
+    for (const WavePlan &wp : plan) {
+        const uint32_t n = wp.n;
+        a.sample_base = wp.sample_base;
+        if (BATCH) {
+            a.batch.segs = d_segs + wp.seg0;
+            a.batch.nsegs = wp.nsegs;
+            a.batch.npix_slots = wp.npix_slots;
+        }
+        samples += n;
         ++waves;
         hipLaunchKernelGGL(k_reset_ctl, dim3(16), dim3(256), 0, st, ctl);
         tm.begin(T_GEN);
-        if (replay)
-            hipLaunchKernelGGL(k_generate<true>, dim3(std::min<uint32_t>((n + 255) / 256, 65535)), dim3(256), 0, st,
-                               a, w.qa, n, ctl);
-        else
-            hipLaunchKernelGGL(k_generate<false>, dim3(std::min<uint32_t>((n + 255) / 256, 65535)), dim3(256), 0, st,
-                               a, w.qa, n, ctl);
+        hipLaunchKernelGGL((k_generate<ADJ, BATCH>), dim3(std::min<uint32_t>((n + 255) / 256, 65535)), dim3(256), 0,
+                           st, a, w.qa, n, ctl);
         tm.end(T_GEN);
         LT_HIP(hipGetLastError());
         Queue cur = w.qa, nxt = w.qb;
@@ -782,51 +875,50 @@ int render_impl(lt_scene *scene, const lt_camera &cam, const lt_render_params &p
                 if (live == 0) break;
             }
             if (wslot + 2 > 4096) return set_error(LT_E_STATE, "too many traversal launches in one wave");
+            TraceBatch tb_ext = {a.batch.scenes, cur.sid};
+            TraceBatch tb_nee = {a.batch.scenes, a.batch.nee_sid};
             tm.begin(T_EXT);
             if ((rc = launch_trace_closest(st, a.sc, cur.rays, w.hits, &ctl->count[b], 0, &ctl->work[wslot++], tblock,
-                                           tgrid, spill, dev->diag.as<uint32_t>(), tvar)))
+                                           tgrid, spill, diag, tvar, BATCH ? &tb_ext : nullptr)))
                 return rc;
             tm.end(T_EXT);
             ++ext_launches;
             tm.begin(T_SHADE);
-            if (replay)
-                hipLaunchKernelGGL(k_shade<true>, dim3(sgrid), dim3(256), 0, st, a, b, cur, (const Hit *)w.hits, nxt,
-                                   w.nee, w.nee_rays, w.nee_hits, w.nee_ignore, w.nee_cidx, ctl);
-            else
-                hipLaunchKernelGGL(k_shade<false>, dim3(sgrid), dim3(256), 0, st, a, b, cur, (const Hit *)w.hits, nxt,
-                                   w.nee, w.nee_rays, w.nee_hits, w.nee_ignore, w.nee_cidx, ctl);
+            hipLaunchKernelGGL((k_shade<ADJ, BATCH>), dim3(sgrid), dim3(256), 0, st, a, b, cur, (const Hit *)w.hits,
+                               nxt, w.nee, w.nee_rays, w.nee_hits, w.nee_ignore, w.nee_cidx, ctl);
             tm.end(T_SHADE);
             LT_HIP(hipGetLastError());
-            /* next-event rays: two per shaded slot; inactive ones have tfar < 0 */
-            tm.begin(T_SHADOW);
             /* next-event rays: occlusion up to the light hit (small lights) or
              * the reference's closest-hit query */
+            tm.begin(T_SHADOW);
             if (a.sc.light_first)
                 rc = launch_trace_any(st, a.sc, w.nee_rays, w.nee_occl, w.nee_ignore, &ctl->nee_count[b], 0,
-                                      &ctl->work[wslot++], tblock, tgrid, spill, dev->diag.as<uint32_t>(), tvar);
+                                      &ctl->work[wslot++], tblock, tgrid, spill, diag, tvar,
+                                      BATCH ? &tb_nee : nullptr);
             else
                 rc = launch_trace_closest(st, a.sc, w.nee_rays, w.nee_trace_hits, &ctl->nee_count[b], 0,
-                                          &ctl->work[wslot++], tblock, tgrid, spill, dev->diag.as<uint32_t>(), tvar);
+                                          &ctl->work[wslot++], tblock, tgrid, spill, diag, tvar,
+                                          BATCH ? &tb_nee : nullptr);
             if (rc) return rc;
             tm.end(T_SHADOW);
             ++sh_launches;
             tm.begin(T_RESOLVE);
-            if (replay)
-                hipLaunchKernelGGL(k_resolve<true>, dim3(sgrid), dim3(256), 0, st, a, b, cur, w.nee,
-                                   (const Ray *)w.nee_rays, (const Hit *)w.nee_hits, (const Hit *)w.nee_trace_hits,
-                                   (const uint32_t *)w.nee_occl, (const uint32_t *)w.nee_cidx, ctl);
-            else
-                hipLaunchKernelGGL(k_resolve<false>, dim3(sgrid), dim3(256), 0, st, a, b, cur, w.nee,
-                                   (const Ray *)w.nee_rays, (const Hit *)w.nee_hits, (const Hit *)w.nee_trace_hits,
-                                   (const uint32_t *)w.nee_occl, (const uint32_t *)w.nee_cidx, ctl);
+            hipLaunchKernelGGL((k_resolve<ADJ, BATCH>), dim3(sgrid), dim3(256), 0, st, a, b, cur, w.nee,
+                               (const Ray *)w.nee_rays, (const Hit *)w.nee_hits, (const Hit *)w.nee_trace_hits,
+                               (const uint32_t *)w.nee_occl, (const uint32_t *)w.nee_cidx, ctl);
             tm.end(T_RESOLVE);
             LT_HIP(hipGetLastError());
             std::swap(cur, nxt);
         }
-        if (!replay) {
+        if (!ADJ) {
             tm.begin(T_ACC);
-            hipLaunchKernelGGL(k_accumulate, dim3(std::min<uint32_t>((npix + 255) / 256, 65535)), dim3(256), 0, st, a,
-                               ns, film->sum.as<float>(), film->bounces.as<uint32_t>(), film->count.as<uint32_t>());
+            if (BATCH)
+                hipLaunchKernelGGL(k_accumulate_batch, dim3(std::min<uint32_t>((wp.npix_slots + 255) / 256, 65535)),
+                                   dim3(256), 0, st, a);
+            else
+                hipLaunchKernelGGL(k_accumulate, dim3(std::min<uint32_t>((a.npix + 255) / 256, 65535)), dim3(256), 0,
+                                   st, a, wp.ns, film->sum.as<float>(), film->bounces.as<uint32_t>(),
+                                   film->count.as<uint32_t>());
             tm.end(T_ACC);
             LT_HIP(hipGetLastError());
         }
@@ -853,7 +945,7 @@ int render_impl(lt_scene *scene, const lt_camera &cam, const lt_render_params &p
     if ((rc = check_traversal_bounds(dev))) return rc;
     if (stats) {
         std::memset(stats, 0, sizeof *stats);
-        stats->samples = (uint64_t)npix * prm.spp;
+        stats->samples = samples;
         stats->closest_rays = closest;
         stats->shadow_rays = shadow;
         stats->path_segments = segments;
@@ -869,6 +961,167 @@ int render_impl(lt_scene *scene, const lt_camera &cam, const lt_render_params &p
         stats->waves = waves;
     }
     return LT_OK;
+}
+
+size_t wave_budget(const lt_render_params &prm) { return prm.wave_paths ? prm.wave_paths : (size_t)16 << 20; }
+
+int check_params(const lt_render_params &prm) {
+    if (prm.max_depth == 0 || prm.max_depth > 2000) return set_error(LT_E_INVALID, "max_depth must be in 1..2000");
+    return LT_OK;
+}
+
+} // namespace
+
+/* Renders one scene; see the file comment for the kernel sequence. With
+ * replay set, runs the adjoint pass instead (no film). */
+int render_impl(lt_scene *scene, const lt_camera &cam, const lt_render_params &prm, lt_film *film,
+                lt_render_stats *stats, const AdjArgs &adj, bool replay) {
+    lt_device *dev = scene->dev;
+    const uint32_t npix = cam.width * cam.height;
+    if (npix == 0 || prm.spp == 0) return LT_OK;
+    int rc = check_params(prm);
+    if (rc) return rc;
+    uint32_t spw = (uint32_t)std::max<size_t>(1, std::min<size_t>(prm.spp, wave_budget(prm) / npix));
+    size_t nmax = (size_t)spw * npix;
+    if (nmax > 0xFFFFFFFFull / 2) return set_error(LT_E_INVALID, "wave too large");
+    Workspace w;
+    if ((rc = carve(dev, nmax, w))) return rc;
+    std::vector<WavePlan> plan;
+    for (uint32_t s0 = 0; s0 < prm.spp; s0 += spw) {
+        WavePlan wp = {};
+        wp.ns = std::min(spw, prm.spp - s0);
+        wp.n = wp.ns * npix;
+        wp.sample_base = s0;
+        plan.push_back(wp);
+    }
+    CameraSetup cs;
+    camera_setup(cam, cs);
+    WaveArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.sc = scene->ds;
+    a.adj = adj;
+    a.adj.inv_spp = 1.0f / (float)prm.spp;
+    std::memcpy(&a.cam, &cs, sizeof(CameraArgs));
+    a.ps = w.ps;
+    a.npix = npix;
+    a.frame0 = prm.frame0;
+    a.seed = prm.seed;
+    a.stream = prm.stream;
+    a.max_depth = prm.max_depth;
+    a.rr_start = prm.rr_start;
+    if (replay) return run_waves<true, false>(dev, a, plan, nullptr, prm, film, stats, w);
+    return run_waves<false, false>(dev, a, plan, nullptr, prm, film, stats, w);
+}
+
+/* Several scenes in one launch sequence: their (scene, sample) ranges are
+ * packed into waves of up to wave_paths paths; scene i draws from Philox
+ * stream params.stream + i, so its film equals that of lt_render of scene i
+ * alone with that stream. */
+int render_batch(lt_scene *const *scenes, const lt_camera *cams, lt_film *const *films, uint32_t nscenes,
+                 const lt_render_params &prm, lt_render_stats *stats) {
+    if (nscenes == 0 || prm.spp == 0) return LT_OK;
+    int rc = check_params(prm);
+    if (rc) return rc;
+    lt_device *dev = scenes[0]->dev;
+    hipStream_t st = dev->stream;
+    std::vector<DScene> tbl(nscenes);
+    std::vector<CameraArgs> cam_tbl(nscenes);
+    std::vector<FilmPtrs> film_tbl(nscenes);
+    bool light_first = true;
+    for (uint32_t i = 0; i < nscenes; ++i) {
+        lt_scene *s = scenes[i];
+        if (!s->committed) return set_error(LT_E_STATE, "lt_render_batch: a scene is not committed");
+        if (s->dev != dev || films[i]->dev != dev) return set_error(LT_E_INVALID, "lt_render_batch: mixed devices");
+        if (s->layout != scenes[0]->layout)
+            return set_error(LT_E_INVALID, "lt_render_batch: scenes must share one BVH layout");
+        if (films[i]->w != cams[i].width || films[i]->h != cams[i].height)
+            return set_error(LT_E_INVALID, "lt_render_batch: film and camera sizes differ");
+        if (cams[i].filter < 0 || cams[i].filter > 2) return set_error(LT_E_INVALID, "lt_render_batch: unknown filter");
+        tbl[i] = s->ds;
+        light_first = light_first && s->ds.light_first;
+        CameraSetup cs;
+        camera_setup(cams[i], cs);
+        std::memcpy(&cam_tbl[i], &cs, sizeof(CameraArgs));
+        film_tbl[i] = {films[i]->sum.as<float>(), films[i]->bounces.as<uint32_t>(), films[i]->count.as<uint32_t>()};
+    }
+    /* the next-event query kind is per launch: all scenes use light-first or none */
+    for (DScene &d : tbl) d.light_first = light_first ? 1u : 0u;
+
+    /* plan: pack whole samples of each scene into waves */
+    const size_t budget = wave_budget(prm);
+    std::vector<Seg> segs;
+    std::vector<WavePlan> plan;
+    WavePlan cur = {};
+    auto close_wave = [&]() {
+        if (cur.n) {
+            plan.push_back(cur);
+            cur = {};
+            cur.seg0 = (uint32_t)segs.size();
+        }
+    };
+    size_t nmax = 0;
+    for (uint32_t i = 0; i < nscenes; ++i) {
+        const uint32_t npix = cams[i].width * cams[i].height;
+        if (npix == 0) continue;
+        uint32_t s0 = 0;
+        while (s0 < prm.spp) {
+            size_t room = budget > cur.n ? budget - cur.n : 0;
+            uint32_t k = (uint32_t)std::min<size_t>(prm.spp - s0, room / npix);
+            if (k == 0) {
+                if (cur.n) {
+                    close_wave();
+                    continue;
+                }
+                k = 1;   /* one sample of a scene larger than the budget */
+            }
+            Seg g = {};
+            g.scene = i;
+            g.s0 = s0;
+            g.ns = k;
+            g.npix = npix;
+            g.w = cams[i].width;
+            g.path_off = cur.n;
+            g.pix_off = cur.npix_slots;
+            segs.push_back(g);
+            cur.nsegs++;
+            cur.n += k * npix;
+            cur.npix_slots += npix;
+            nmax = std::max<size_t>(nmax, cur.n);
+            s0 += k;
+        }
+    }
+    close_wave();
+    if (plan.empty()) return LT_OK;
+    if (nmax > 0xFFFFFFFFull / 2) return set_error(LT_E_INVALID, "wave too large");
+    Workspace w;
+    if ((rc = carve(dev, nmax, w))) return rc;
+    const size_t tbytes = nscenes * (sizeof(DScene) + sizeof(CameraArgs) + sizeof(FilmPtrs)) + segs.size() * sizeof(Seg)
+                          + 1024;
+    if ((rc = dev->batch_tables.reserve(tbytes))) return rc;
+    char *base = (char *)dev->batch_tables.p;
+    auto place = [&](const void *src, size_t bytes) -> void * {
+        char *dst = base;
+        base += (bytes + 255) & ~(size_t)255;
+        (void)hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+        return dst;
+    };
+    WaveArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.sc = tbl[0];
+    a.batch.scenes = (const DScene *)place(tbl.data(), tbl.size() * sizeof(DScene));
+    a.batch.cams = (const CameraArgs *)place(cam_tbl.data(), cam_tbl.size() * sizeof(CameraArgs));
+    a.batch.films = (const FilmPtrs *)place(film_tbl.data(), film_tbl.size() * sizeof(FilmPtrs));
+    const Seg *d_segs = (const Seg *)place(segs.data(), segs.size() * sizeof(Seg));
+    LT_HIP(hipGetLastError());
+    a.batch.path_seg = w.path_seg;
+    a.batch.nee_sid = w.nee_sid;
+    a.ps = w.ps;
+    a.frame0 = prm.frame0;
+    a.seed = prm.seed;
+    a.stream = prm.stream;
+    a.max_depth = prm.max_depth;
+    a.rr_start = prm.rr_start;
+    return run_waves<false, true>(dev, a, plan, d_segs, prm, nullptr, stats, w);
 }
 
 int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &prm, lt_film *film,

@@ -61,18 +61,18 @@ LT_D bool slab(float lox, float hix, float loy, float hiy, float loz, float hiz,
     return tn <= tf;
 }
 
-template <bool ANY, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, const Ray *__restrict__ rays, Hit *__restrict__ hits,
-                                                 uint32_t *__restrict__ occluded, const uint32_t *ignore_geom,
-                                                 const uint32_t *count_ptr, uint32_t count_imm, uint32_t *work,
-                                                 int *spill, uint32_t *overflow) {
+/* BATCH: rays of several scenes in one launch; ray i belongs to scene
+ * batch.ray_scene[i] of the table batch.scenes (same layout for all). */
+template <bool ANY, int BLOCK, bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, TraceBatch batch, const Ray *__restrict__ rays,
+                                                 Hit *__restrict__ hits, uint32_t *__restrict__ occluded,
+                                                 const uint32_t *ignore_geom, const uint32_t *count_ptr,
+                                                 uint32_t count_imm, uint32_t *work, int *spill, uint32_t *overflow) {
     __shared__ int lds[LDS_STACK * BLOCK];
     const uint32_t n = count_ptr ? *count_ptr : count_imm;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     int *gstack = spill + (size_t)(blockIdx.x * BLOCK + tid) * (STACK_CAP - LDS_STACK);
-    const float4 *__restrict__ nodes = sc.nodes;
-    const float4 *__restrict__ tris = sc.tris;
 
     while (true) {
         uint32_t base = 0;
@@ -94,7 +94,16 @@ __global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, const Ray *__restric
         float best_t = r.tfar, best_u = 0.0f, best_v = 0.0f;
         uint32_t best_prim = 0xFFFFFFFFu;
         const uint32_t ignore = (ANY && ignore_geom) ? ignore_geom[i] : 0xFFFFFFFFu;
-        bool done = !(r.tfar >= 0.0f) || sc.ntris == 0;
+        const float4 *__restrict__ nodes = sc.nodes;
+        const float4 *__restrict__ tris = sc.tris;
+        uint32_t ntris = sc.ntris;
+        if (BATCH) {
+            const DScene *bs = batch.scenes + batch.ray_scene[i];
+            nodes = bs->nodes;
+            tris = bs->tris;
+            ntris = bs->ntris;
+        }
+        bool done = !(r.tfar >= 0.0f) || ntris == 0;
 
         int sp = 0;
         int node = 0;
@@ -158,7 +167,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, const Ray *__restric
                         }
                     }
                     if (__float_as_uint(b.w) & 1u) break;
-                    if (++k >= (int)sc.ntris) break;
+                    if (++k >= (int)ntris) break;
                 }
                 if (done) break;
             }
@@ -191,19 +200,17 @@ LT_D float qbyte(uint32_t w, int j) { return (float)((w >> (8 * (j & 3))) & 0xFF
 
 /* LDS8: stack entries kept in LDS per lane; WAVES: occupancy the register
  * allocator targets (0: the compiler's choice). */
-template <bool ANY, int BLOCK, int LDS8, int WAVES>
+template <bool ANY, int BLOCK, int LDS8, int WAVES, bool BATCH>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVES : 1)))
-void k_trace8(DScene sc, const Ray *__restrict__ rays, Hit *__restrict__ hits, uint32_t *__restrict__ occluded,
-              const uint32_t *ignore_geom, const uint32_t *count_ptr, uint32_t count_imm, uint32_t *work, int2 *spill,
-              uint32_t *overflow) {
+void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__restrict__ hits,
+              uint32_t *__restrict__ occluded, const uint32_t *ignore_geom, const uint32_t *count_ptr,
+              uint32_t count_imm, uint32_t *work, int2 *spill, uint32_t *overflow) {
     __shared__ int lds_node[LDS8 * BLOCK];
     __shared__ float lds_t[LDS8 * BLOCK];
     const uint32_t n = count_ptr ? *count_ptr : count_imm;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     int2 *gstack = spill + (size_t)(blockIdx.x * BLOCK + tid) * STACK8_CAP;
-    const float4 *__restrict__ wn = sc.wnodes;
-    const float4 *__restrict__ wt = sc.wtris;
 
     while (true) {
         uint32_t base = 0;
@@ -225,7 +232,16 @@ void k_trace8(DScene sc, const Ray *__restrict__ rays, Hit *__restrict__ hits, u
         float best_t = r.tfar, best_u = 0.0f, best_v = 0.0f;
         uint32_t best_prim = 0xFFFFFFFFu;
         const uint32_t ignore = (ANY && ignore_geom) ? ignore_geom[i] : 0xFFFFFFFFu;
-        bool done = !(r.tfar >= 0.0f) || sc.ntris == 0;
+        const float4 *__restrict__ wn = sc.wnodes;
+        const float4 *__restrict__ wt = sc.wtris;
+        uint32_t ntris = sc.ntris;
+        if (BATCH) {
+            const DScene *bs = batch.scenes + batch.ray_scene[i];
+            wn = bs->wnodes;
+            wt = bs->wtris;
+            ntris = bs->ntris;
+        }
+        bool done = !(r.tfar >= 0.0f) || ntris == 0;
 
         int sp = 0;
         int node = 0;
@@ -354,15 +370,14 @@ void k_trace8(DScene sc, const Ray *__restrict__ rays, Hit *__restrict__ hits, u
     }
 }
 
-template <bool ANY>
-int launch(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, uint32_t *occl, const uint32_t *ign,
-           const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block, int grid, int *spill,
-           uint32_t *overflow, int variant) {
-    if (variant < 0 || variant >= kTraceVariants) return set_error(LT_E_INVALID, "unknown traversal variant");
+template <bool ANY, bool BATCH>
+int launch_b(hipStream_t st, const DScene &sc, const TraceBatch &tb, const Ray *rays, Hit *hits, uint32_t *occl,
+             const uint32_t *ign, const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block, int grid,
+             int *spill, uint32_t *overflow, int variant) {
     if (sc.layout == 0) {
         int2 *sp2 = (int2 *)spill;
         const int v = variant;
-#define LT_T8(B, L, W) hipLaunchKernelGGL((k_trace8<ANY, B, L, W>), dim3(grid), dim3(B), 0, st, sc, rays, hits, occl, ign, count_ptr, count, work, sp2, overflow)
+#define LT_T8(B, L, W) hipLaunchKernelGGL((k_trace8<ANY, B, L, W, BATCH>), dim3(grid), dim3(B), 0, st, sc, tb, rays, hits, occl, ign, count_ptr, count, work, sp2, overflow)
 #define LT_T8B(B)                          \
         if (v == 0) LT_T8(B, 16, 0);       \
         else if (v == 1) LT_T8(B, 8, 0);   \
@@ -380,13 +395,26 @@ int launch(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, uint32_
         return LT_OK;
     }
     switch (block) {
-    case 64: hipLaunchKernelGGL((k_trace<ANY, 64>), dim3(grid), dim3(64), 0, st, sc, rays, hits, occl, ign, count_ptr, count, work, spill, overflow); break;
-    case 128: hipLaunchKernelGGL((k_trace<ANY, 128>), dim3(grid), dim3(128), 0, st, sc, rays, hits, occl, ign, count_ptr, count, work, spill, overflow); break;
-    case 256: hipLaunchKernelGGL((k_trace<ANY, 256>), dim3(grid), dim3(256), 0, st, sc, rays, hits, occl, ign, count_ptr, count, work, spill, overflow); break;
+    case 64: hipLaunchKernelGGL((k_trace<ANY, 64, BATCH>), dim3(grid), dim3(64), 0, st, sc, tb, rays, hits, occl, ign, count_ptr, count, work, spill, overflow); break;
+    case 128: hipLaunchKernelGGL((k_trace<ANY, 128, BATCH>), dim3(grid), dim3(128), 0, st, sc, tb, rays, hits, occl, ign, count_ptr, count, work, spill, overflow); break;
+    case 256: hipLaunchKernelGGL((k_trace<ANY, 256, BATCH>), dim3(grid), dim3(256), 0, st, sc, tb, rays, hits, occl, ign, count_ptr, count, work, spill, overflow); break;
     default: return set_error(LT_E_INVALID, "traversal block size must be 64, 128 or 256");
     }
     LT_HIP(hipGetLastError());
     return LT_OK;
+}
+
+template <bool ANY>
+int launch(hipStream_t st, const DScene &sc, const TraceBatch *tb, const Ray *rays, Hit *hits, uint32_t *occl,
+           const uint32_t *ign, const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block, int grid,
+           int *spill, uint32_t *overflow, int variant) {
+    if (variant < 0 || variant >= kTraceVariants) return set_error(LT_E_INVALID, "unknown traversal variant");
+    if (tb && tb->scenes)
+        return launch_b<ANY, true>(st, sc, *tb, rays, hits, occl, ign, count_ptr, count, work, block, grid, spill,
+                                   overflow, variant);
+    TraceBatch none = {};
+    return launch_b<ANY, false>(st, sc, none, rays, hits, occl, ign, count_ptr, count, work, block, grid, spill,
+                                overflow, variant);
 }
 
 } // namespace
@@ -411,15 +439,15 @@ size_t trace_spill_bytes(int grid, int block) {
 
 int launch_trace_closest(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, const uint32_t *count_ptr,
                          uint32_t count, uint32_t *work, int block, int grid, int *spill, uint32_t *overflow,
-                         int variant) {
-    return launch<false>(st, sc, rays, hits, nullptr, nullptr, count_ptr, count, work, block, grid, spill, overflow,
-                         variant);
+                         int variant, const TraceBatch *batch) {
+    return launch<false>(st, sc, batch, rays, hits, nullptr, nullptr, count_ptr, count, work, block, grid, spill,
+                         overflow, variant);
 }
 int launch_trace_any(hipStream_t st, const DScene &sc, const Ray *rays, uint32_t *occluded,
                      const uint32_t *ignore_geom, const uint32_t *count_ptr, uint32_t count, uint32_t *work, int block,
-                     int grid, int *spill, uint32_t *overflow, int variant) {
-    return launch<true>(st, sc, rays, nullptr, occluded, ignore_geom, count_ptr, count, work, block, grid, spill,
-                        overflow, variant);
+                     int grid, int *spill, uint32_t *overflow, int variant, const TraceBatch *batch) {
+    return launch<true>(st, sc, batch, rays, nullptr, occluded, ignore_geom, count_ptr, count, work, block, grid,
+                        spill, overflow, variant);
 }
 
 } // namespace lt

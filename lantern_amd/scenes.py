@@ -401,3 +401,45 @@ def diff_cube(width=256, height=256, spp=64, seed=DEFAULT_SEED, tex_res=1024, gr
                   background=0.5, camera_phi=1.1, camera_theta=0.7, camera_radius=5.0, max_depth=max_depth,
                   russian_roulette="off")
     return d
+
+
+def batch_scenes(n=64, seed=DEFAULT_SEED, spp=32, max_depth=16, subdivs=(5, 6, 7), size_range=(64, 512)):
+    """The batch workload: n scenes, each an icosphere of subdivision drawn
+    from `subdivs` (20,480 / 81,920 / 327,680 triangles) on a Lambertian
+    floor under a 2x2 quad light (radiance 10) and environment 0.3, with a
+    material drawn uniformly from {Lambert, GGX (alpha 0.3), dielectric IOR
+    1.5} and a random albedo, image size W, H ~ U{64..512} independently, a
+    random orbit camera; 32 spp, max depth 16, Russian roulette from depth 4."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    spheres = {k: geosphere(1.0, k) for k in subdivs}
+    gp, gn, guv, gi = grid(20.0, 20.0, 2, 2)
+    lp, ln, luv, li, power = quad_light(2.0, 4.0, 10.0)
+    out = []
+    for i in range(n):
+        d = SceneDesc()
+        k = int(rng.choice(subdivs))
+        kind = int(rng.integers(0, 3))
+        alb = tuple(float(v) for v in rng.uniform(0.3, 0.9, 3))
+        if kind == 0:
+            m = d.add_material(bsdf=_capi.LT_BSDF_LAMBERT, albedo=alb)
+        elif kind == 1:
+            m = d.add_material(bsdf=_capi.LT_BSDF_GGX, albedo=alb, roughness=0.3)
+        else:
+            m = d.add_material(bsdf=_capi.LT_BSDF_DIELECTRIC, albedo=(1.0, 1.0, 1.0), ior=1.5)
+        floor = d.add_material(bsdf=_capi.LT_BSDF_LAMBERT, albedo=(0.7, 0.7, 0.7))
+        light_mat = d.add_material(bsdf=_capi.LT_BSDF_LAMBERT, albedo=(0.0, 0.0, 0.0))
+        sp, sn, si = spheres[k]
+        p2, n2 = transform(sp, sn, translate(0.0, 1.0, 0.0))
+        d.add_mesh(p2, si, n2, None, m)
+        d.add_mesh(gp, gi, gn, guv, floor)
+        d.add_mesh(lp, li, ln, luv, light_mat, emission=((1.0, 1.0, 1.0), power))
+        d.background = (0.3, 0.3, 0.3)
+        w, h = (int(v) for v in rng.integers(size_range[0], size_range[1] + 1, 2))
+        phi, theta = float(rng.uniform(0.9, 1.3)), float(rng.uniform(0.0, 2 * math.pi))
+        d.camera = dict(phi=phi, theta=theta, radius=6.0, width=w, height=h, target=(0.0, 1.0, 0.0),
+                        fov=math.pi / 4, filter=0)
+        d.params = dict(spp=spp, max_depth=max_depth, rr_start=4, seed=seed)
+        d.meta = dict(workload="batch", index=i, subdiv=k, triangles=20 << (2 * k), material=["lambert", "ggx",
+                      "dielectric"][kind], width=w, height=h)
+        out.append(d)
+    return out
