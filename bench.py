@@ -68,6 +68,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rendering iteration timing")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="batch: 0 off, 1 tune block/variant/wave size, 2 also the node width")
+    ap.add_argument("--graph", type=int, default=0, help="capture the launch sequence in a hipGraph and replay it")
     ap.add_argument("--adjoint-iters", type=int, default=3)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (HBM counter calibration)")
@@ -99,6 +102,7 @@ def main():
     params["wave_paths"] = args.wave_paths
     params["block_size"] = args.block
     params["trace_variant"] = args.variant
+    params["use_graph"] = args.graph
     with lantern_amd.Device(0) as dev:
         scene = lantern_amd.Scene(dev, desc, layout=args.layout, builder=args.builder)
         info = scene.info()
@@ -202,6 +206,13 @@ def bench_batch(args):
         build_s = time.time() - tb
         cams = [d.camera for d in descs]
         films = [lantern_amd.Film(dev, c["width"], c["height"]) for c in cams]
+        tune = None
+        if args.autotune:
+            tt = time.time()
+            tune = lantern_amd.autotune(ss, cams, params, films, try_layouts=args.autotune > 1)
+            tune["seconds"] = round(time.time() - tt, 2)
+            params = lantern_amd.tuned_params(params, tune)
+        params["use_graph"] = args.graph
 
         def one(seq):
             for f in films:
@@ -254,6 +265,7 @@ def bench_batch(args):
                      "kernel": "k_trace8 closest hit, batched (extension rays)", "bytes_per_ray": RAY_BYTES,
                      "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4)},
         "scene_gen_s": round(gen_s, 2), "bvh_build_s": round(build_s, 2),
+        "autotune": tune, "graph": bool(args.graph),
         "cpu_baseline": None,
     }
 

@@ -254,6 +254,23 @@ int lt_render(lt_scene *scene, const lt_camera *camera, const lt_render_params *
  * Lantern scenes (integrator.cpp:28-43), which the reference runs one by one. */
 int lt_render_batch(lt_scene *const *scenes, const lt_camera *cameras, lt_film *const *films, uint32_t n,
                     const lt_render_params *params, lt_render_stats *stats);
+/* Autotuner (no reference counterpart): times short probe renders of the
+ * batch over traversal block sizes {64, 128, 256}, kernel variants 1-4 (LDS
+ * stack depth 16 or 8, register/occupancy targets) and wave sizes {4M, 8M,
+ * 16M} paths; with try_layouts also both node widths (8-wide and binary; the
+ * scenes are re-committed and left in the fastest). Films are reset after. */
+typedef struct lt_tune_result {
+    uint32_t block_size;     /* best lt_render_params values */
+    uint32_t trace_variant;
+    uint32_t wave_paths;
+    uint32_t layout;         /* best lt_build_params.layout */
+    float best_ms;           /* probe time of the best and of the default configuration */
+    float default_ms;
+    uint32_t candidates;
+    uint32_t reserved[5];
+} lt_tune_result;
+int lt_autotune(lt_scene *const *scenes, const lt_camera *cameras, lt_film *const *films, uint32_t n,
+                const lt_render_params *params, int32_t try_layouts, lt_tune_result *out);
 /* The same result as lt_render_batch by n separate lt_render calls. */
 int lt_render_each(lt_scene *const *scenes, const lt_camera *cameras, lt_film *const *films, uint32_t n,
                    const lt_render_params *params, lt_render_stats *stats);

@@ -14,7 +14,7 @@ from ._capi import (LanternError, LT_BSDF_DIELECTRIC, LT_BSDF_GGX, LT_BSDF_LAMBE
                     LT_TEX_UV, check, lib)
 from .scenes import SceneDesc
 
-__all__ = ["Device", "Scene", "Film", "render", "render_batch", "SceneDesc", "LanternError", "camera_struct", "params_struct",
+__all__ = ["Device", "Scene", "Film", "render", "render_batch", "autotune", "tuned_params", "SceneDesc", "LanternError", "camera_struct", "params_struct",
            "LT_INVALID_ID"]
 
 
@@ -225,6 +225,24 @@ def render_batch(scenes, cameras, params, films, sequential=False):
     fn = lib().lt_render_each if sequential else lib().lt_render_batch
     check(fn(hs, cams, fs, n, C.byref(prm), C.byref(st)), "lt_render_batch")
     return {k: getattr(st, k) for k, _ in st._fields_}
+
+
+def autotune(scenes, cameras, params, films, try_layouts=False):
+    """lt_autotune: the fastest block size, kernel variant, wave size (and
+    node width) for this batch. Returns a dict; apply it with tuned_params."""
+    n = len(scenes)
+    out = _capi.lt_tune_result()
+    hs = (C.c_void_p * n)(*[s.h.value for s in scenes])
+    fs = (C.c_void_p * n)(*[f.h.value for f in films])
+    cams = (_capi.lt_camera * n)(*[camera_struct(c) for c in cameras])
+    prm = params_struct(params)
+    check(lib().lt_autotune(hs, cams, fs, n, C.byref(prm), 1 if try_layouts else 0, C.byref(out)), "lt_autotune")
+    return {k: getattr(out, k) for k, _ in out._fields_ if k != "reserved"}
+
+
+def tuned_params(params, tune):
+    return dict(params, block_size=tune["block_size"], trace_variant=tune["trace_variant"],
+                wave_paths=tune["wave_paths"])
 
 
 def render(scene, camera, params, film):

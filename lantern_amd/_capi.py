@@ -64,6 +64,12 @@ class lt_render_params(C.Structure):
                 ("reserved", C.c_uint32 * 6)]
 
 
+class lt_tune_result(C.Structure):
+    _fields_ = [("block_size", C.c_uint32), ("trace_variant", C.c_uint32), ("wave_paths", C.c_uint32),
+                ("layout", C.c_uint32), ("best_ms", C.c_float), ("default_ms", C.c_float),
+                ("candidates", C.c_uint32), ("reserved", C.c_uint32 * 5)]
+
+
 class lt_render_stats(C.Structure):
     _fields_ = [("samples", C.c_uint64), ("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("path_segments", C.c_uint64), ("total_ms", C.c_double), ("extend_ms", C.c_double),
@@ -123,6 +129,8 @@ SIGNATURES = {
                             C.POINTER(lt_render_stats)]),
     "lt_render_batch": (C.c_int, [C.POINTER(_vp), C.POINTER(lt_camera), C.POINTER(_vp), _u32,
                                   C.POINTER(lt_render_params), C.POINTER(lt_render_stats)]),
+    "lt_autotune": (C.c_int, [C.POINTER(_vp), C.POINTER(lt_camera), C.POINTER(_vp), _u32,
+                              C.POINTER(lt_render_params), C.c_int32, C.POINTER(lt_tune_result)]),
     "lt_render_each": (C.c_int, [C.POINTER(_vp), C.POINTER(lt_camera), C.POINTER(_vp), _u32,
                                  C.POINTER(lt_render_params), C.POINTER(lt_render_stats)]),
     "lt_geometry_geosphere": (_u32, [_f, _u32, _pf, _pf, _pu32, _pu32]),

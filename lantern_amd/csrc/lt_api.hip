@@ -261,6 +261,7 @@ int lt_device_destroy(lt_device *d) {
     if (!d) return LT_OK;
     (void)hipSetDevice(d->ordinal);
     (void)hipStreamSynchronize(d->stream);
+    if (d->graph_exec) (void)hipGraphExecDestroy((hipGraphExec_t)d->graph_exec);
     d->ws.release();
     d->ctl.release();
     d->trace_work.release();
@@ -501,6 +502,10 @@ int lt_scene_commit(lt_scene *s, const lt_build_params *params) {
     uint32_t builder = params ? params->builder : 0;
     LT_CHECK(builder <= 1, "lt_scene_commit: builder must be 0 (PLOC) or 1 (LBVH)");
     s->layout = layout;
+    s->build = params ? *params : lt_build_params{};
+    s->build.max_leaf_size = max_leaf;
+    s->build.layout = layout;
+    s->build.builder = builder;
     hipStream_t st = s->dev->stream;
     LT_HIP(hipSetDevice(s->dev->ordinal));
     int rc;
@@ -906,6 +911,73 @@ int lt_adjoint(lt_scene *s, const lt_camera *cam, const lt_render_params *prm, c
     LT_CHECK(grad_texels || s->texels.empty(), "lt_adjoint: the scene has textures but grad_texels is NULL");
     LT_CHECK(grad_voxels || s->voxels.empty(), "lt_adjoint: the scene has grids but grad_voxels is NULL");
     return render_adjoint(s, *cam, *prm, sample_radiance, pixel_grad, grad_texels, grad_voxels, stats);
+}
+
+/* ---- autotuner ------------------------------------------------------------------ */
+/* Times short probe renders (min(spp, 2) samples per pixel) of the batch for
+ * every traversal block size x kernel variant (LDS stack depth, occupancy
+ * target) x wave size, and with try_layouts for both node widths (the scenes
+ * are re-committed), then leaves the scenes in the fastest layout, resets the
+ * films and reports the fastest parameters. */
+int lt_autotune(lt_scene *const *scenes, const lt_camera *cams, lt_film *const *films, uint32_t n,
+                const lt_render_params *params, int32_t try_layouts, lt_tune_result *out) {
+    LT_CHECK(scenes && cams && films && params && out && n > 0, "lt_autotune: bad argument");
+    for (uint32_t i = 0; i < n; ++i) LT_CHECK(scenes[i] && films[i], "lt_autotune: NULL scene or film");
+    std::memset(out, 0, sizeof *out);
+    lt_render_params probe = *params;
+    probe.spp = std::min<uint32_t>(params->spp, 2u);
+    probe.use_graph = 0;
+    const uint32_t blocks[3] = {64, 128, 256};
+    const uint32_t waves[3] = {4u << 20, 8u << 20, 16u << 20};
+    const uint32_t orig_layout = scenes[0]->layout;
+    uint32_t layouts[2] = {orig_layout, 1u - orig_layout};
+    const int nl = try_layouts ? 2 : 1;
+    float best = 1e30f;
+    lt_render_stats st;
+    int rc;
+    for (int li = 0; li < nl; ++li) {
+        if (scenes[0]->layout != layouts[li]) {
+            for (uint32_t i = 0; i < n; ++i) {
+                lt_build_params bp = scenes[i]->build;
+                bp.layout = layouts[li];
+                if ((rc = lt_scene_commit(scenes[i], &bp))) return rc;
+            }
+        }
+        /* one untimed probe to settle allocations */
+        if ((rc = render_batch(scenes, cams, films, n, probe, &st))) return rc;
+        for (uint32_t b : blocks)
+            for (uint32_t v = 1; v <= (uint32_t)kTraceVariants; ++v)
+                for (uint32_t wv : waves) {
+                    lt_render_params p = probe;
+                    p.block_size = b;
+                    p.trace_variant = v;
+                    p.wave_paths = wv;
+                    if ((rc = render_batch(scenes, cams, films, n, p, &st))) return rc;
+                    out->candidates++;
+                    const bool is_default = b == (uint32_t)trace_default_block() &&
+                                            v == (uint32_t)kDefaultTraceVariant + 1 && wv == (16u << 20) &&
+                                            layouts[li] == orig_layout;
+                    if (is_default) out->default_ms = (float)st.total_ms;
+                    if ((float)st.total_ms < best) {
+                        best = (float)st.total_ms;
+                        out->block_size = b;
+                        out->trace_variant = v;
+                        out->wave_paths = wv;
+                        out->layout = layouts[li];
+                    }
+                }
+    }
+    out->best_ms = best;
+    if (scenes[0]->layout != out->layout) {
+        for (uint32_t i = 0; i < n; ++i) {
+            lt_build_params bp = scenes[i]->build;
+            bp.layout = out->layout;
+            if ((rc = lt_scene_commit(scenes[i], &bp))) return rc;
+        }
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        if ((rc = lt_film_reset(films[i]))) return rc;
+    return LT_OK;
 }
 
 /* ---- generators ---------------------------------------------------------------- */

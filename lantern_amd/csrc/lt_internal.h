@@ -126,6 +126,10 @@ struct lt_device {
     lt::DevBuf spill;        /* traversal stack overflow area */
     lt::DevBuf diag;         /* [0]: rays that hit a traversal bound (stack/steps) */
     lt::DevBuf batch_tables; /* scene, camera, film and segment tables of a batch render */
+    /* the last captured launch sequence, replayed while its arguments repeat */
+    void *graph_exec = nullptr;
+    std::string graph_key;
+    uint32_t graph_captures = 0;
     uint32_t *pinned = nullptr;
 };
 
@@ -152,6 +156,7 @@ struct lt_scene {
     lt::DevBuf d_pos, d_nrm, d_uv, d_vidx, d_tri_geom, d_geoms, d_mats, d_media, d_texs, d_texels, d_grids,
         d_voxels, d_lights, d_nodes, d_tris, d_light_tris, d_wnodes, d_wtris;
     uint32_t layout = 0;   /* 0: 8-wide compressed nodes, 1: binary nodes */
+    lt_build_params build = {};   /* parameters of the last commit */
     lt::BvhStats bvh;
     float build_ms = 0.0f;
     lt::DScene ds;

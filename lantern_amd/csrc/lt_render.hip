@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <string>
 
 namespace lt {
 
@@ -45,7 +46,8 @@ struct Ctl {
     uint32_t nee_count[2048];  /* next-event rays queued (compacted) at bounce b */
     uint32_t work[4096];       /* traversal work counters, one per launch */
     unsigned long long shadow_rays;
-    unsigned long long pad[7];
+    unsigned long long tally[2];   /* over all waves of a render: closest-hit rays, next-event rays */
+    unsigned long long pad[5];
 };
 
 struct CameraArgs {
@@ -661,6 +663,22 @@ __global__ void k_accumulate_batch(WaveArgs a) {
     }
 }
 
+/* Adds the wave's ray counts to the render's tally (one thread). */
+__global__ void k_tally(Ctl *ctl, uint32_t max_depth) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    unsigned long long c = 0, n = 0;
+    for (uint32_t b = 0; b < max_depth; ++b) {
+        c += ctl->count[b];
+        n += ctl->nee_count[b];
+    }
+    ctl->tally[0] += c;
+    ctl->tally[1] += n;
+}
+
+__global__ void k_reset_tally(Ctl *ctl) {
+    if (blockIdx.x == 0 && threadIdx.x < 2) ctl->tally[threadIdx.x] = 0ull;
+}
+
 __global__ void k_reset_ctl(Ctl *ctl) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 2048) {
@@ -839,104 +857,143 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
     if ((rc = dev->spill.reserve(trace_spill_bytes(tgrid, tblock)))) return rc;
     int *spill = (int *)dev->spill.p;
     uint32_t *diag = dev->diag.as<uint32_t>();
-
+    const bool graph = prm.use_graph != 0;
     const int sgrid = std::max(1, dev->num_cus) * 8;
-    EventTimer tm(stats != nullptr, st);
+    /* per-kernel event timing only outside graphs */
+    EventTimer tm(stats != nullptr && !graph, st);
+    uint32_t ext_launches = 0, sh_launches = 0;
+    uint64_t samples = 0;
+
+    /* The launch sequence of all waves. With early_exit, a wave stops after
+     * every 8th bounce once no path is alive (a host round trip); a captured
+     * graph runs all max_depth bounces, whose kernels return at once when
+     * their live count is zero. */
+    auto enqueue = [&](bool early_exit) -> int {
+        int r;
+        hipLaunchKernelGGL(k_reset_tally, dim3(1), dim3(64), 0, st, ctl);
+        for (const WavePlan &wp : plan) {
+            const uint32_t n = wp.n;
+            a.sample_base = wp.sample_base;
+            if (BATCH) {
+                a.batch.segs = d_segs + wp.seg0;
+                a.batch.nsegs = wp.nsegs;
+                a.batch.npix_slots = wp.npix_slots;
+            }
+            samples += n;
+            hipLaunchKernelGGL(k_reset_ctl, dim3(16), dim3(256), 0, st, ctl);
+            tm.begin(T_GEN);
+            hipLaunchKernelGGL((k_generate<ADJ, BATCH>), dim3(std::min<uint32_t>((n + 255) / 256, 65535)), dim3(256),
+                               0, st, a, w.qa, n, ctl);
+            tm.end(T_GEN);
+            LT_HIP(hipGetLastError());
+            Queue cur = w.qa, nxt = w.qb;
+            uint32_t wslot = 0;
+            for (uint32_t b = 0; b < prm.max_depth; ++b) {
+                if (early_exit && b > 0 && (b % 8) == 0) {
+                    uint32_t live = 0;
+                    LT_HIP(hipMemcpyAsync(&live, &ctl->count[b], sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+                    LT_HIP(hipStreamSynchronize(st));
+                    if (live == 0) break;
+                }
+                if (wslot + 2 > 4096) return set_error(LT_E_STATE, "too many traversal launches in one wave");
+                TraceBatch tb_ext = {a.batch.scenes, cur.sid};
+                TraceBatch tb_nee = {a.batch.scenes, a.batch.nee_sid};
+                tm.begin(T_EXT);
+                if ((r = launch_trace_closest(st, a.sc, cur.rays, w.hits, &ctl->count[b], 0, &ctl->work[wslot++],
+                                              tblock, tgrid, spill, diag, tvar, BATCH ? &tb_ext : nullptr)))
+                    return r;
+                tm.end(T_EXT);
+                ++ext_launches;
+                tm.begin(T_SHADE);
+                hipLaunchKernelGGL((k_shade<ADJ, BATCH>), dim3(sgrid), dim3(256), 0, st, a, b, cur,
+                                   (const Hit *)w.hits, nxt, w.nee, w.nee_rays, w.nee_hits, w.nee_ignore, w.nee_cidx,
+                                   ctl);
+                tm.end(T_SHADE);
+                LT_HIP(hipGetLastError());
+                /* next-event rays: occlusion up to the light hit (small lights) or
+                 * the reference's closest-hit query */
+                tm.begin(T_SHADOW);
+                if (a.sc.light_first)
+                    r = launch_trace_any(st, a.sc, w.nee_rays, w.nee_occl, w.nee_ignore, &ctl->nee_count[b], 0,
+                                         &ctl->work[wslot++], tblock, tgrid, spill, diag, tvar,
+                                         BATCH ? &tb_nee : nullptr);
+                else
+                    r = launch_trace_closest(st, a.sc, w.nee_rays, w.nee_trace_hits, &ctl->nee_count[b], 0,
+                                             &ctl->work[wslot++], tblock, tgrid, spill, diag, tvar,
+                                             BATCH ? &tb_nee : nullptr);
+                if (r) return r;
+                tm.end(T_SHADOW);
+                ++sh_launches;
+                tm.begin(T_RESOLVE);
+                hipLaunchKernelGGL((k_resolve<ADJ, BATCH>), dim3(sgrid), dim3(256), 0, st, a, b, cur, w.nee,
+                                   (const Ray *)w.nee_rays, (const Hit *)w.nee_hits, (const Hit *)w.nee_trace_hits,
+                                   (const uint32_t *)w.nee_occl, (const uint32_t *)w.nee_cidx, ctl);
+                tm.end(T_RESOLVE);
+                LT_HIP(hipGetLastError());
+                std::swap(cur, nxt);
+            }
+            if (!ADJ) {
+                tm.begin(T_ACC);
+                if (BATCH)
+                    hipLaunchKernelGGL(k_accumulate_batch,
+                                       dim3(std::min<uint32_t>((wp.npix_slots + 255) / 256, 65535)), dim3(256), 0,
+                                       st, a);
+                else
+                    hipLaunchKernelGGL(k_accumulate, dim3(std::min<uint32_t>((a.npix + 255) / 256, 65535)),
+                                       dim3(256), 0, st, a, wp.ns, film->sum.as<float>(),
+                                       film->bounces.as<uint32_t>(), film->count.as<uint32_t>());
+                tm.end(T_ACC);
+                LT_HIP(hipGetLastError());
+            }
+            hipLaunchKernelGGL(k_tally, dim3(1), dim3(64), 0, st, ctl, prm.max_depth);
+        }
+        return LT_OK;
+    };
+
     hipEvent_t e0, e1;
     LT_HIP(hipEventCreate(&e0));
     LT_HIP(hipEventCreate(&e1));
-    LT_HIP(hipEventRecord(e0, st));
-    uint64_t closest = 0, shadow = 0, segments = 0, samples = 0;
-    uint32_t ext_launches = 0, sh_launches = 0, waves = 0;
-    for (const WavePlan &wp : plan) {
-        const uint32_t n = wp.n;
-        a.sample_base = wp.sample_base;
-        if (BATCH) {
-            a.batch.segs = d_segs + wp.seg0;
-            a.batch.nsegs = wp.nsegs;
-            a.batch.npix_slots = wp.npix_slots;
-        }
-        samples += n;
-        ++waves;
-        hipLaunchKernelGGL(k_reset_ctl, dim3(16), dim3(256), 0, st, ctl);
-        tm.begin(T_GEN);
-        hipLaunchKernelGGL((k_generate<ADJ, BATCH>), dim3(std::min<uint32_t>((n + 255) / 256, 65535)), dim3(256), 0,
-                           st, a, w.qa, n, ctl);
-        tm.end(T_GEN);
-        LT_HIP(hipGetLastError());
-        Queue cur = w.qa, nxt = w.qb;
-        uint32_t wslot = 0;
-        for (uint32_t b = 0; b < prm.max_depth; ++b) {
-            if (b > 0 && (b % 8) == 0) {
-                /* stop early once every path of the wave has terminated */
-                uint32_t live = 0;
-                LT_HIP(hipMemcpyAsync(&live, &ctl->count[b], sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-                LT_HIP(hipStreamSynchronize(st));
-                if (live == 0) break;
+    if (graph) {
+        /* replay the cached graph when every kernel argument is the same */
+        std::string key((const char *)&a, sizeof a);
+        for (const WavePlan &wp : plan) key.append((const char *)&wp, sizeof wp);
+        const void *ptrs[4] = {d_segs, film ? film->sum.p : nullptr, spill, ctl};
+        const int ints[5] = {tblock, tvar, tgrid, (int)prm.max_depth, (int)BATCH * 2 + (int)ADJ};
+        key.append((const char *)ptrs, sizeof ptrs);
+        key.append((const char *)ints, sizeof ints);
+        if (!dev->graph_exec || dev->graph_key != key) {
+            if (dev->graph_exec) {
+                (void)hipGraphExecDestroy((hipGraphExec_t)dev->graph_exec);
+                dev->graph_exec = nullptr;
             }
-            if (wslot + 2 > 4096) return set_error(LT_E_STATE, "too many traversal launches in one wave");
-            TraceBatch tb_ext = {a.batch.scenes, cur.sid};
-            TraceBatch tb_nee = {a.batch.scenes, a.batch.nee_sid};
-            tm.begin(T_EXT);
-            if ((rc = launch_trace_closest(st, a.sc, cur.rays, w.hits, &ctl->count[b], 0, &ctl->work[wslot++], tblock,
-                                           tgrid, spill, diag, tvar, BATCH ? &tb_ext : nullptr)))
+            LT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            rc = enqueue(false);
+            hipGraph_t g = nullptr;
+            hipError_t ce = hipStreamEndCapture(st, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
                 return rc;
-            tm.end(T_EXT);
-            ++ext_launches;
-            tm.begin(T_SHADE);
-            hipLaunchKernelGGL((k_shade<ADJ, BATCH>), dim3(sgrid), dim3(256), 0, st, a, b, cur, (const Hit *)w.hits,
-                               nxt, w.nee, w.nee_rays, w.nee_hits, w.nee_ignore, w.nee_cidx, ctl);
-            tm.end(T_SHADE);
-            LT_HIP(hipGetLastError());
-            /* next-event rays: occlusion up to the light hit (small lights) or
-             * the reference's closest-hit query */
-            tm.begin(T_SHADOW);
-            if (a.sc.light_first)
-                rc = launch_trace_any(st, a.sc, w.nee_rays, w.nee_occl, w.nee_ignore, &ctl->nee_count[b], 0,
-                                      &ctl->work[wslot++], tblock, tgrid, spill, diag, tvar,
-                                      BATCH ? &tb_nee : nullptr);
-            else
-                rc = launch_trace_closest(st, a.sc, w.nee_rays, w.nee_trace_hits, &ctl->nee_count[b], 0,
-                                          &ctl->work[wslot++], tblock, tgrid, spill, diag, tvar,
-                                          BATCH ? &tb_nee : nullptr);
-            if (rc) return rc;
-            tm.end(T_SHADOW);
-            ++sh_launches;
-            tm.begin(T_RESOLVE);
-            hipLaunchKernelGGL((k_resolve<ADJ, BATCH>), dim3(sgrid), dim3(256), 0, st, a, b, cur, w.nee,
-                               (const Ray *)w.nee_rays, (const Hit *)w.nee_hits, (const Hit *)w.nee_trace_hits,
-                               (const uint32_t *)w.nee_occl, (const uint32_t *)w.nee_cidx, ctl);
-            tm.end(T_RESOLVE);
-            LT_HIP(hipGetLastError());
-            std::swap(cur, nxt);
-        }
-        if (!ADJ) {
-            tm.begin(T_ACC);
-            if (BATCH)
-                hipLaunchKernelGGL(k_accumulate_batch, dim3(std::min<uint32_t>((wp.npix_slots + 255) / 256, 65535)),
-                                   dim3(256), 0, st, a);
-            else
-                hipLaunchKernelGGL(k_accumulate, dim3(std::min<uint32_t>((a.npix + 255) / 256, 65535)), dim3(256), 0,
-                                   st, a, wp.ns, film->sum.as<float>(), film->bounces.as<uint32_t>(),
-                                   film->count.as<uint32_t>());
-            tm.end(T_ACC);
-            LT_HIP(hipGetLastError());
-        }
-        if (stats) {
-            std::vector<uint32_t> counts(prm.max_depth + 1), ncounts(prm.max_depth + 1);
-            LT_HIP(hipMemcpyAsync(counts.data(), ctl->count, sizeof(uint32_t) * (prm.max_depth + 1),
-                                  hipMemcpyDeviceToHost, st));
-            LT_HIP(hipMemcpyAsync(ncounts.data(), ctl->nee_count, sizeof(uint32_t) * (prm.max_depth + 1),
-                                  hipMemcpyDeviceToHost, st));
-            LT_HIP(hipStreamSynchronize(st));
-            for (uint32_t b = 0; b < prm.max_depth; ++b) {
-                closest += counts[b];
-                segments += counts[b];
-                shadow += ncounts[b];
             }
+            LT_HIP(ce);
+            hipGraphExec_t ex = nullptr;
+            hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            LT_HIP(ie);
+            dev->graph_exec = ex;
+            dev->graph_key = key;
+            dev->graph_captures++;
+        } else {
+            for (const WavePlan &wp : plan) samples += wp.n;
         }
+        ext_launches = sh_launches = (uint32_t)plan.size() * prm.max_depth;
+        LT_HIP(hipEventRecord(e0, st));
+        LT_HIP(hipGraphLaunch((hipGraphExec_t)dev->graph_exec, st));
+        LT_HIP(hipEventRecord(e1, st));
+    } else {
+        LT_HIP(hipEventRecord(e0, st));
+        if ((rc = enqueue(true))) return rc;
+        LT_HIP(hipEventRecord(e1, st));
     }
-    LT_HIP(hipEventRecord(e1, st));
     LT_HIP(hipEventSynchronize(e1));
     float total = 0;
     LT_HIP(hipEventElapsedTime(&total, e0, e1));
@@ -944,11 +1001,13 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
     (void)hipEventDestroy(e1);
     if ((rc = check_traversal_bounds(dev))) return rc;
     if (stats) {
+        unsigned long long tally[2] = {0, 0};
+        LT_HIP(hipMemcpy(tally, ctl->tally, sizeof tally, hipMemcpyDeviceToHost));
         std::memset(stats, 0, sizeof *stats);
         stats->samples = samples;
-        stats->closest_rays = closest;
-        stats->shadow_rays = shadow;
-        stats->path_segments = segments;
+        stats->closest_rays = tally[0];
+        stats->shadow_rays = tally[1];
+        stats->path_segments = tally[0];
         stats->total_ms = total;
         stats->generate_ms = tm.total(T_GEN);
         stats->extend_ms = tm.total(T_EXT);
@@ -958,7 +1017,7 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
         stats->accumulate_ms = tm.total(T_ACC);
         stats->extend_launches = ext_launches;
         stats->shadow_launches = sh_launches;
-        stats->waves = waves;
+        stats->waves = (uint32_t)plan.size();
     }
     return LT_OK;
 }

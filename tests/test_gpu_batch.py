@@ -69,3 +69,61 @@ def test_batch_rejects_mixed_layouts(gpu_device):
         f.close()
     a.close()
     b.close()
+
+
+def test_graph_replays_match_eager(gpu_device):
+    # a captured launch sequence, replayed with eager work (film resets,
+    # downloads) between the replays: every replay equals the eager render
+    d = scenes.nine_sphere(64, 48, 3, subdiv=3)
+    gs = lantern_amd.Scene(gpu_device, d)
+    film = lantern_amd.Film(gpu_device, 64, 48)
+    lantern_amd.render(gs, d.camera, dict(d.params, spp=3), film)
+    ref = film.download()
+    for k in range(3):
+        film.reset()
+        st = lantern_amd.render(gs, d.camera, dict(d.params, spp=3, use_graph=1), film)
+        got = film.download()
+        for x, y in zip(ref, got):
+            assert np.array_equal(x, y), f"replay {k}"
+        assert st["samples"] == 64 * 48 * 3
+    film.close()
+    gs.close()
+    descs = scenes.batch_scenes(4, subdivs=(3,), size_range=(20, 50), spp=2, max_depth=6)
+    ss = [lantern_amd.Scene(gpu_device, x) for x in descs]
+    cams = [x.camera for x in descs]
+    fa, fb = _films(gpu_device, descs), _films(gpu_device, descs)
+    lantern_amd.render_batch(ss, cams, dict(descs[0].params, wave_paths=3000), fa)
+    for k in range(3):
+        for f in fb:
+            f.reset()
+        lantern_amd.render_batch(ss, cams, dict(descs[0].params, wave_paths=3000, use_graph=1), fb)
+        for i in range(len(descs)):
+            assert np.array_equal(fa[i].download()[0], fb[i].download()[0]), f"replay {k} scene {i}"
+    for f in fa + fb:
+        f.close()
+    for s in ss:
+        s.close()
+
+
+def test_autotune_picks_a_valid_config_with_identical_results(gpu_device):
+    descs = scenes.batch_scenes(5, subdivs=(3, 4), size_range=(32, 96), spp=4, max_depth=8)
+    ss = [lantern_amd.Scene(gpu_device, x) for x in descs]
+    cams = [x.camera for x in descs]
+    fa, fb = _films(gpu_device, descs), _films(gpu_device, descs)
+    params = dict(descs[0].params)
+    lantern_amd.render_batch(ss, cams, params, fa)
+    tune = lantern_amd.autotune(ss, cams, params, fb)
+    assert tune["candidates"] == 36
+    assert tune["block_size"] in (64, 128, 256) and 1 <= tune["trace_variant"] <= 4
+    assert tune["best_ms"] <= tune["default_ms"]
+    assert all(int(f.download()[2].max()) == 0 for f in fb)   # films reset after tuning
+    lantern_amd.render_batch(ss, cams, lantern_amd.tuned_params(params, tune), fb)
+    for i in range(len(descs)):
+        assert np.array_equal(fa[i].download()[0], fb[i].download()[0])
+    tune2 = lantern_amd.autotune(ss, cams, params, fb, try_layouts=True)
+    assert tune2["candidates"] == 72 and tune2["layout"] in (0, 1)
+    assert all(s.info()["layout"] == tune2["layout"] for s in ss)
+    for f in fa + fb:
+        f.close()
+    for s in ss:
+        s.close()
