@@ -70,7 +70,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rendering iteration timing")
     ap.add_argument("--autotune", type=int, default=1,
                     help="batch: 0 off, 1 tune block/variant/wave size, 2 also the node width")
-    ap.add_argument("--graph", type=int, default=0, help="capture the launch sequence in a hipGraph and replay it")
+    ap.add_argument("--graph", type=int, default=-1,
+                    help="capture the launch sequence in a hipGraph and replay it (-1: batch yes, single scene no, "
+                         "so that the single-scene roofline comes from per-kernel events of the timed steps)")
     ap.add_argument("--adjoint-iters", type=int, default=3)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (HBM counter calibration)")
@@ -102,7 +104,7 @@ def main():
     params["wave_paths"] = args.wave_paths
     params["block_size"] = args.block
     params["trace_variant"] = args.variant
-    params["use_graph"] = args.graph
+    params["use_graph"] = 1 if args.graph == 1 else 0
     with lantern_amd.Device(0) as dev:
         scene = lantern_amd.Scene(dev, desc, layout=args.layout, builder=args.builder)
         info = scene.info()
@@ -212,7 +214,7 @@ def bench_batch(args):
             tune = lantern_amd.autotune(ss, cams, params, films, try_layouts=args.autotune > 1)
             tune["seconds"] = round(time.time() - tt, 2)
             params = lantern_amd.tuned_params(params, tune)
-        params["use_graph"] = args.graph
+        params["use_graph"] = 1 if args.graph != 0 else 0
 
         def one(seq):
             for f in films:
@@ -228,6 +230,11 @@ def bench_batch(args):
             per_step.append(one(False))
         dev.synchronize()
         wall = time.perf_counter() - wall0
+        # per-kernel event times need the eager launch sequence: one more step
+        eager = dict(params, use_graph=0)
+        for f in films:
+            f.reset()
+        st_e = lantern_amd.render_batch(ss, cams, eager, films)
         # the same work scene by scene, for comparison
         one(True)
         dev.synchronize()
@@ -245,9 +252,8 @@ def bench_batch(args):
     samples = sum(st["samples"] for st in per_step)
     closest = sum(st["closest_rays"] for st in per_step)
     shadow = sum(st["shadow_rays"] for st in per_step)
-    ext_ms = sum(st["extend_ms"] for st in per_step)
-    ext_launches = sum(st["extend_launches"] for st in per_step)
-    achieved = closest * RAY_BYTES / (ext_ms / 1e3) / 1e9 if ext_ms > 0 else 0.0
+    ext_ms, ext_launches = st_e["extend_ms"], st_e["extend_launches"]
+    achieved = st_e["closest_rays"] * RAY_BYTES / (ext_ms / 1e3) / 1e9 if ext_ms > 0 else 0.0
     return {
         "metric": "Msamples/s at fixed RMSE vs FP64 oracle; Mrays/s traversal; adjoint ms/iter",
         "value": round(samples / wall / 1e6, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
@@ -262,7 +268,8 @@ def bench_batch(args):
         "waves_per_step": per_step[0]["waves"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "k_trace8 closest hit, batched (extension rays)", "bytes_per_ray": RAY_BYTES,
+                     "kernel": "k_trace8 closest hit, batched (extension rays); per-kernel times from one eager step "
+                               "after the timed (graph) steps", "bytes_per_ray": RAY_BYTES,
                      "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4)},
         "scene_gen_s": round(gen_s, 2), "bvh_build_s": round(build_s, 2),
         "autotune": tune, "graph": bool(args.graph),
