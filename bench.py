@@ -68,6 +68,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rendering iteration timing")
+    ap.add_argument("--no-traversal", action="store_true", help="skip the ray-query (lt_intersect_*) timing")
     ap.add_argument("--autotune", type=int, default=1,
                     help="batch: 0 off, 1 tune block/variant/wave size, 2 also the node width")
     ap.add_argument("--graph", type=int, default=-1,
@@ -124,6 +125,7 @@ def main():
         s, b, c = film.download()
         assert int(c.min()) == args.spp and np.all(np.isfinite(s)), "film incomplete or non-finite"
         mean_radiance = float(s.mean() / args.spp)
+        traversal = None if args.no_traversal else traversal_bench(dev, scene, desc)
         scene.close()
         film.close()
 
@@ -183,12 +185,77 @@ def main():
                 "wide_depth": info["wide_depth"], "nodes": info["num_nodes"], "leaves": info["num_leaves"],
                 "depth": info["max_depth"],
                 "sah_cost": round(info["sah_cost"], 3), "build_ms": round(info["build_ms"], 2)},
+        "traversal": traversal,
         "mean_radiance": round(mean_radiance, 6),
         "scene_gen_s": round(gen_s, 2),
         "cpu_baseline": cpu,
         "adjoint": adjoint,
     }
     print(json.dumps(out))
+
+
+def traversal_bench(dev, scene, desc, n=1 << 23, reps=5):
+    """Ray queries alone (lt_intersect_closest/any_device) on the workload's
+    scene: n incoherent rays (origins uniform in the scene's bounding box,
+    directions uniform on the sphere) and n coherent camera rays, resident in
+    HBM; plus the host-buffer entry point, whose rate includes the PCIe copies."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    import lantern_amd
+    from lantern_amd import _capi
+
+    L = _capi.lib()
+    rng = np.random.default_rng(1)
+    pos = np.concatenate([m["pos"] for m in desc.meshes])
+    lo, hi = pos.min(0), pos.max(0)
+    inc = np.zeros((n, 8), np.float32)
+    inc[:, 0:3] = rng.uniform(lo, hi, (n, 3))
+    dr = rng.normal(size=(n, 3))
+    inc[:, 4:7] = dr / np.linalg.norm(dr, axis=1, keepdims=True)
+    inc[:, 7] = np.inf
+    cam = desc.camera
+    w, h = cam["width"], cam["height"]
+    phi, th, r = cam["phi"], cam["theta"], cam["radius"]
+    tgt = np.array(cam["target"], np.float64)
+    org = tgt + r * np.array([np.sin(phi) * np.sin(th), np.cos(phi), np.sin(phi) * np.cos(th)])
+    z = (org - tgt) / np.linalg.norm(org - tgt)
+    x = np.cross([0.0, 1.0, 0.0], z)
+    x /= np.linalg.norm(x)
+    y = np.cross(z, x)
+    k = np.arange(n)
+    px, py = (k % w) + rng.random(n), ((k // w) % h) + rng.random(n)
+    tx, ty = np.tan(cam["fov"] / 2), np.tan(cam["fov"] / 2 * h / w)
+    vx, vy = (px / w * 2 - 1) * tx, -(py / h * 2 - 1) * ty
+    d = vx[:, None] * x + vy[:, None] * y - z
+    coh = np.zeros((n, 8), np.float32)
+    coh[:, 0:3] = org
+    coh[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    coh[:, 7] = np.inf
+    stream = C.c_void_p()
+    _capi.check(L.lt_device_stream(dev.h, C.byref(stream)), "lt_device_stream")
+    out = {}
+    for name, rays in (("incoherent", inc), ("camera", coh)):
+        dr_ = torch.from_numpy(rays).cuda()
+        dh = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+        docc = torch.empty(n, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        for kind, fn, dst in (("closest", L.lt_intersect_closest_device, dh), ("any", L.lt_intersect_any_device, docc)):
+            fn(scene.h, C.c_void_p(dr_.data_ptr()), C.c_void_p(dst.data_ptr()), n, stream)
+            dev.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                _capi.check(fn(scene.h, C.c_void_p(dr_.data_ptr()), C.c_void_p(dst.data_ptr()), n, stream), kind)
+            dev.synchronize()
+            out[f"{name}_{kind}_mrays_per_s"] = round(n * reps / (time.perf_counter() - t0) / 1e6, 1)
+    hits = np.zeros((n, 4), np.float32)
+    t0 = time.perf_counter()
+    _capi.check(L.lt_intersect_closest(scene.h, inc.ctypes.data, hits.ctypes.data, n), "lt_intersect_closest")
+    out["incoherent_closest_host_buffers_mrays_per_s"] = round(n / (time.perf_counter() - t0) / 1e6, 1)
+    out["rays"] = n
+    return out
 
 
 def bench_batch(args):
