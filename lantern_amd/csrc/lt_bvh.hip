@@ -406,7 +406,8 @@ __global__ void k_wide_pass(WideArgs w) {
         sc[k] = ldexp(1.0, ee[k]);
         pd[k] = floor((double)blo[k] / sc[k]) * sc[k];
     }
-    unsigned qlo[3][2] = {{0u, 0u}, {0u, 0u}, {0u, 0u}}, qhi[3][2] = {{0u, 0u}, {0u, 0u}, {0u, 0u}};
+    /* unused slots get inverted boxes (lo 255, hi 0): no ray enters them */
+    unsigned qlo[3][2] = {{~0u, ~0u}, {~0u, ~0u}, {~0u, ~0u}}, qhi[3][2] = {{0u, 0u}, {0u, 0u}, {0u, 0u}};
     unsigned meta[2] = {0u, 0u};
     unsigned imask = 0u;
     int rank = 0, off = 0;
@@ -418,7 +419,7 @@ __global__ void k_wide_pass(WideArgs w) {
             double qh = ceil(((double)hi[k] - pd[k]) / sc[k]);
             unsigned uql = (unsigned)fmin(fmax(ql, 0.0), 255.0);
             unsigned uqh = (unsigned)fmin(fmax(qh, 0.0), 255.0);
-            qlo[k][j >> 2] |= uql << (8 * (j & 3));
+            qlo[k][j >> 2] = (qlo[k][j >> 2] & ~(0xFFu << (8 * (j & 3)))) | (uql << (8 * (j & 3)));
             qhi[k][j >> 2] |= uqh << (8 * (j & 3));
         }
         unsigned m;
@@ -498,8 +499,10 @@ __global__ void k_wide_single_leaf(const float *pos, const uint32_t *vidx, const
     unsigned meta0 = (n << 5);   /* slot 0: leaf of n triangles at offset 0 */
     wnodes[0] = make_float4((float)pd[0], (float)pd[1], (float)pd[2], __uint_as_float(ebits));
     wnodes[1] = make_float4(__uint_as_float(0u), __uint_as_float(0u), __uint_as_float(meta0), __uint_as_float(0u));
-    wnodes[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);     /* qlo = 0 */
-    wnodes[3] = make_float4(0.0f, 0.0f, __uint_as_float(q[0]), 0.0f);
+    /* slot 0: qlo = 0; slots 1-7 unused: inverted boxes (lo 255, hi 0) */
+    wnodes[2] = make_float4(__uint_as_float(0xFFFFFF00u), __uint_as_float(~0u), __uint_as_float(0xFFFFFF00u),
+                            __uint_as_float(~0u));
+    wnodes[3] = make_float4(__uint_as_float(0xFFFFFF00u), __uint_as_float(~0u), __uint_as_float(q[0]), 0.0f);
     wnodes[4] = make_float4(__uint_as_float(q[1]), 0.0f, __uint_as_float(q[2]), 0.0f);
 }
 

@@ -195,6 +195,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, TraceBatch batch, co
  * A push beyond STACK8_CAP is counted in *overflow and the ray ends as a
  * miss, so an overflow is reported to the caller instead of going unnoticed. */
 constexpr int STACK8_CAP = 256;
+constexpr float kSlack = 0x1p-20f;
 
 LT_D float qbyte(uint32_t w, int j) { return (float)((w >> (8 * (j & 3))) & 0xFFu); }
 
@@ -229,6 +230,7 @@ void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__
         r.tfar = rb.w;
         r.tmin = fmaxf(r.tnear, 0.0f);
         r.idir = v3(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
+        const bool negx = r.idir.x < 0.0f, negy = r.idir.y < 0.0f, negz = r.idir.z < 0.0f;
         float best_t = r.tfar, best_u = 0.0f, best_v = 0.0f;
         uint32_t best_prim = 0xFFFFFFFFu;
         const uint32_t ignore = (ANY && ignore_geom) ? ignore_geom[i] : 0xFFFFFFFFu;
@@ -262,27 +264,49 @@ void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__
             const uint32_t imask = eb >> 24;
             const uint32_t cbase = __float_as_uint(n1.x), tbase = __float_as_uint(n1.y);
             const uint32_t mw[2] = {__float_as_uint(n1.z), __float_as_uint(n1.w)};
-            const uint32_t qlx[2] = {__float_as_uint(n2.x), __float_as_uint(n2.y)};
-            const uint32_t qly[2] = {__float_as_uint(n2.z), __float_as_uint(n2.w)};
-            const uint32_t qlz[2] = {__float_as_uint(n3.x), __float_as_uint(n3.y)};
-            const uint32_t qhx[2] = {__float_as_uint(n3.z), __float_as_uint(n3.w)};
-            const uint32_t qhy[2] = {__float_as_uint(n4.x), __float_as_uint(n4.y)};
-            const uint32_t qhz[2] = {__float_as_uint(n4.z), __float_as_uint(n4.w)};
+            /* Plane distances as t = q * (s * idir) + (p - o) * idir: one FMA per
+             * plane. s is a power of two, so s * idir is exact; the rounding of
+             * b = (p - o) * idir errs by at most 2u |b| (u = 2^-24) on that axis,
+             * so near planes use b - c and far planes b + c with c = 2^-20 |b|,
+             * and the final FMA's relative error is covered by 2^-20 |t| in the
+             * test below: a box the ray enters is never rejected. The near plane
+             * of each axis is the low one for a positive direction. */
+            const float ax = sx * r.idir.x, ay = sy * r.idir.y, az = sz * r.idir.z;
+            const float bx = (n0.x - r.o.x) * r.idir.x, by = (n0.y - r.o.y) * r.idir.y,
+                        bz = (n0.z - r.o.z) * r.idir.z;
+            const float cx = fabsf(bx) * kSlack, cy = fabsf(by) * kSlack, cz = fabsf(bz) * kSlack;
+            const float bnx = bx - cx, bfx = bx + cx, bny = by - cy, bfy = by + cy, bnz = bz - cz, bfz = bz + cz;
+            const uint32_t ql[3][2] = {{__float_as_uint(n2.x), __float_as_uint(n2.y)},
+                                       {__float_as_uint(n2.z), __float_as_uint(n2.w)},
+                                       {__float_as_uint(n3.x), __float_as_uint(n3.y)}};
+            const uint32_t qh[3][2] = {{__float_as_uint(n3.z), __float_as_uint(n3.w)},
+                                       {__float_as_uint(n4.x), __float_as_uint(n4.y)},
+                                       {__float_as_uint(n4.z), __float_as_uint(n4.w)}};
+            const uint32_t nqx[2] = {negx ? qh[0][0] : ql[0][0], negx ? qh[0][1] : ql[0][1]};
+            const uint32_t fqx[2] = {negx ? ql[0][0] : qh[0][0], negx ? ql[0][1] : qh[0][1]};
+            const uint32_t nqy[2] = {negy ? qh[1][0] : ql[1][0], negy ? qh[1][1] : ql[1][1]};
+            const uint32_t fqy[2] = {negy ? ql[1][0] : qh[1][0], negy ? ql[1][1] : qh[1][1]};
+            const uint32_t nqz[2] = {negz ? qh[2][0] : ql[2][0], negz ? qh[2][1] : ql[2][1]};
+            const uint32_t fqz[2] = {negz ? ql[2][0] : qh[2][0], negz ? ql[2][1] : qh[2][1]};
+            const float tfar_w = r.tfar * 1.0000005f;
             int near_node = -1;
             float near_t = 0.0f;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const uint32_t meta = (mw[j >> 2] >> (8 * (j & 3))) & 0xFFu;
                 const bool isint = (imask >> j) & 1u;
-                if (!isint && meta == 0u) continue;
-                float tn;
-                const bool hit = slab(__builtin_fmaf(qbyte(qlx[j >> 2], j), sx, n0.x),
-                                      __builtin_fmaf(qbyte(qhx[j >> 2], j), sx, n0.x),
-                                      __builtin_fmaf(qbyte(qly[j >> 2], j), sy, n0.y),
-                                      __builtin_fmaf(qbyte(qhy[j >> 2], j), sy, n0.y),
-                                      __builtin_fmaf(qbyte(qlz[j >> 2], j), sz, n0.z),
-                                      __builtin_fmaf(qbyte(qhz[j >> 2], j), sz, n0.z), r, tn);
-                if (!hit) continue;
+                /* unused slots hold inverted boxes and meta 0 (a leaf of no triangles) */
+                const float tnx = __builtin_fmaf(qbyte(nqx[j >> 2], j), ax, bnx);
+                const float tfx = __builtin_fmaf(qbyte(fqx[j >> 2], j), ax, bfx);
+                const float tny = __builtin_fmaf(qbyte(nqy[j >> 2], j), ay, bny);
+                const float tfy = __builtin_fmaf(qbyte(fqy[j >> 2], j), ay, bfy);
+                const float tnz = __builtin_fmaf(qbyte(nqz[j >> 2], j), az, bnz);
+                const float tfz = __builtin_fmaf(qbyte(fqz[j >> 2], j), az, bfz);
+                const float tn0 = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), r.tmin);
+                const float tf0 = fminf(fminf(fminf(tfx, tfy), tfz), tfar_w);
+                if (!(tn0 <= __builtin_fmaf(fabsf(tf0), kSlack, tf0))) continue;
+                /* lower bound of the entry distance, for ordering and culling on pop */
+                const float tn = __builtin_fmaf(fabsf(tn0), -kSlack, tn0);
                 if (isint) {
                     const int child = (int)(cbase + meta);
                     if (near_node < 0) {
