@@ -117,8 +117,11 @@ int lt_scene_destroy(lt_scene *scene);
  * VERTEX_ATTRIBUTE slot 0 normals float3, slot 1 uvs float2) + rtcAttachGeometry
  * (scene.cpp:451-582). Positions are already transformed (AddMesh applies the
  * JSON transform on the host, scene.cpp:456-461). verts_per_prim is 3 or 4;
- * quads are split into (0,1,2) and (0,2,3) triangles sharing one primID
- * numbering of 2*q and 2*q+1. geomIDs are sequential from 0, as rtcAttachGeometry
+ * a quad (v0, v1, v2, v3) is intersected as Embree's quad intersector does
+ * (triangles (v0, v1, v3) and (v2, v3, v1), quad_intersector_moeller.h:146-153):
+ * hits report the quad's (u, v), attributes interpolate as QuadMesh::interpolate
+ * (scene_quad_mesh.cpp:238-255), and lt_scene_split_prims maps both triangles
+ * to the quad's primID. geomIDs are sequential from 0, as rtcAttachGeometry
  * assigns them. normals/uvs may be NULL (hasNormals/hasTexCoords false). */
 int lt_scene_add_mesh(lt_scene *scene, const float *positions, size_t position_stride_bytes,
                       uint32_t num_vertices, const uint32_t *indices, uint32_t num_prims,

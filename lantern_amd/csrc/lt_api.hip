@@ -318,7 +318,7 @@ int lt_scene_add_mesh(lt_scene *s, const float *positions, size_t stride, uint32
     dg.tbase = (uint32_t)s->tri_geom.size();
     dg.material = -1;
     dg.light = -1;
-    dg.flags = (normals ? 1u : 0u) | (uvs ? 2u : 0u);
+    dg.flags = (normals ? 1u : 0u) | (uvs ? 2u : 0u) | (vpp == 4 ? 4u : 0u);
     const size_t sf = stride / sizeof(float);
     for (uint32_t i = 0; i < nv; ++i) {
         s->pos.push_back(positions[i * sf]);
@@ -327,12 +327,23 @@ int lt_scene_add_mesh(lt_scene *s, const float *positions, size_t stride, uint32
         for (int k = 0; k < 3; ++k) s->nrm.push_back(normals ? normals[3 * i + k] : 0.0f);
         for (int k = 0; k < 2; ++k) s->uv.push_back(uvs ? uvs[2 * i + k] : 0.0f);
     }
+    /* A quad (v0, v1, v2, v3) is intersected as Embree's quad intersector does
+     * (quad_intersector_moeller.h:146-153): triangles (v0, v1, v3) and
+     * (v2, v3, v1), the second reporting (1 - u, 1 - v), so hits carry the
+     * quad's own (u, v). */
     for (uint32_t p = 0; p < nprims; ++p) {
         const uint32_t *ix = indices + (size_t)p * vpp;
-        uint32_t tri[2][3] = {{ix[0], ix[1], ix[2]}, {ix[0], vpp == 4 ? ix[2] : 0u, vpp == 4 ? ix[3] : 0u}};
-        for (uint32_t t = 0; t < (vpp == 4 ? 2u : 1u); ++t) {
-            for (int k = 0; k < 3; ++k) s->vidx.push_back(vbase + tri[t][k]);
+        if (vpp == 3) {
+            for (int k = 0; k < 3; ++k) s->vidx.push_back(vbase + ix[k]);
             s->tri_geom.push_back(g);
+            s->tri_flags.push_back(0u);
+        } else {
+            const uint32_t tri[2][3] = {{ix[0], ix[1], ix[3]}, {ix[2], ix[3], ix[1]}};
+            for (uint32_t t = 0; t < 2; ++t) {
+                for (int k = 0; k < 3; ++k) s->vidx.push_back(vbase + tri[t][k]);
+                s->tri_geom.push_back(g);
+                s->tri_flags.push_back(t == 1 ? 2u : 0u);
+            }
         }
     }
     s->geoms.push_back(dg);
@@ -424,15 +435,24 @@ int lt_scene_set_emission(lt_scene *s, uint32_t geom, const float color[3], floa
     DGeom &g = s->geoms[geom];
     uint32_t vb = s->geom_vbase[geom], nv = s->geom_nv[geom];
     uint32_t t1 = geom + 1 < s->geoms.size() ? s->geoms[geom + 1].tbase : (uint32_t)s->tri_geom.size();
-    /* surface area as AddMesh sums it (scene.cpp:464-471) */
+    /* surface area as AddMesh sums it (scene.cpp:464-471); for quads as AddLMF
+     * does (scene.cpp:540-548): 0.5|(v0-v1)x(v0-v2)| + 0.5|(v0-v2)x(v0-v3)|
+     * with v3 read from Indices[i + 2], so the second term is zero (quirk A12). */
     float area = 0.0f;
     const float *P = s->pos.data();
-    for (uint32_t t = g.tbase; t < t1; ++t) {
-        const uint32_t *ix = &s->vidx[3 * (size_t)t];
-        V3 a = v3(P[3 * ix[0]], P[3 * ix[0] + 1], P[3 * ix[0] + 2]);
-        V3 b = v3(P[3 * ix[1]], P[3 * ix[1] + 1], P[3 * ix[1] + 2]);
-        V3 c = v3(P[3 * ix[2]], P[3 * ix[2] + 1], P[3 * ix[2] + 2]);
-        area += 0.5f * length(cross(a - b, a - c));
+    auto vert = [&](uint32_t i) { return v3(P[3 * i], P[3 * i + 1], P[3 * i + 2]); };
+    if (g.flags & 4u) {
+        for (uint32_t t = g.tbase; t + 1 < t1; t += 2) {
+            const uint32_t *h0 = &s->vidx[3 * (size_t)t], *h1 = &s->vidx[3 * (size_t)(t + 1)];
+            V3 a = vert(h0[0]), b = vert(h0[1]), c = vert(h1[0]), d = c;
+            area += 0.5f * length(cross(a - b, a - c)) + 0.5f * length(cross(a - c, a - d));
+        }
+    } else {
+        for (uint32_t t = g.tbase; t < t1; ++t) {
+            const uint32_t *ix = &s->vidx[3 * (size_t)t];
+            V3 a = vert(ix[0]), b = vert(ix[1]), c = vert(ix[2]);
+            area += 0.5f * length(cross(a - b, a - c));
+        }
     }
     LT_CHECK(area > 0.0f, "lt_scene_set_emission: geometry has zero area");
     DLight L;
@@ -491,7 +511,10 @@ int lt_scene_commit(lt_scene *s, const lt_build_params *params) {
             uint32_t tb = t;
             float tbf;
             std::memcpy(&tbf, &tb, 4);
-            float rec[12] = {v0.x, v0.y, v0.z, tbf, e1.x, e1.y, e1.z, 0.0f, e2.x, e2.y, e2.z, 0.0f};
+            const uint32_t fl = s->tri_flags[t] & 2u;
+            float flf;
+            std::memcpy(&flf, &fl, 4);
+            float rec[12] = {v0.x, v0.y, v0.z, tbf, e1.x, e1.y, e1.z, flf, e2.x, e2.y, e2.z, 0.0f};
             s->light_tris.insert(s->light_tris.end(), rec, rec + 12);
         }
     }
@@ -526,6 +549,7 @@ int lt_scene_commit(lt_scene *s, const lt_build_params *params) {
     if ((rc = s->d_pos.upload(s->pos.data(), s->pos.size() * 4, st)) || (rc = s->d_nrm.upload(s->nrm.data(), s->nrm.size() * 4, st)) ||
         (rc = s->d_uv.upload(s->uv.data(), s->uv.size() * 4, st)) || (rc = s->d_vidx.upload(s->vidx.data(), s->vidx.size() * 4, st)) ||
         (rc = s->d_tri_geom.upload(s->tri_geom.data(), s->tri_geom.size() * 4, st)) ||
+        (rc = s->d_tri_flags.upload(s->tri_flags.data(), s->tri_flags.size() * 4, st)) ||
         (rc = s->d_geoms.upload(s->geoms.data(), s->geoms.size() * sizeof(DGeom), st)) ||
         (rc = s->d_mats.upload(s->mats.data(), s->mats.size() * sizeof(DMaterial), st)) ||
         (rc = s->d_media.upload(s->media.data(), s->media.size() * sizeof(DMedium), st)) ||
@@ -538,7 +562,8 @@ int lt_scene_commit(lt_scene *s, const lt_build_params *params) {
         return rc;
     auto t0 = std::chrono::steady_clock::now();
     uint32_t ntris = (uint32_t)s->tri_geom.size();
-    if ((rc = build_bvh(st, s->d_pos.as<float>(), s->d_vidx.as<uint32_t>(), s->d_tri_geom.as<uint32_t>(), ntris, max_leaf,
+    if ((rc = build_bvh(st, s->d_pos.as<float>(), s->d_vidx.as<uint32_t>(), s->d_tri_geom.as<uint32_t>(),
+                        s->d_tri_flags.as<uint32_t>(), ntris, max_leaf,
                         builder, s->d_nodes, s->d_tris, s->d_wnodes, s->d_wtris, s->bvh)))
         return rc;
     LT_HIP(hipStreamSynchronize(st));
@@ -614,6 +639,7 @@ int lt_scene_split_prims(lt_scene *s, const uint32_t *gp, uint32_t n, uint32_t *
         uint32_t g = s->tri_geom[gp[i]];
         geom[i] = g;
         prim[i] = gp[i] - s->geoms[g].tbase;
+        if (s->geoms[g].flags & 4u) prim[i] >>= 1;   /* the quad of a triangle pair */
     }
     return LT_OK;
 }

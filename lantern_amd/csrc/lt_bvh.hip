@@ -231,8 +231,10 @@ __global__ void k_reach(BuildArrays a, int *emit, unsigned int *stats) {
     }
 }
 
-__global__ void k_write_tris(const float *pos, const uint32_t *vidx, const uint32_t *tri_geom, const uint64_t *keys,
-                             uint32_t n, float4 *tris) {
+/* Triangle records: (v0, prim) (e1, flags) (e2, geomID). Flag bit 0 (binary
+ * layout) ends a leaf; bit 1 marks the second half of a quad (tri_flags). */
+__global__ void k_write_tris(const float *pos, const uint32_t *vidx, const uint32_t *tri_geom, const uint32_t *tri_flags,
+                             const uint64_t *keys, uint32_t n, float4 *tris) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     uint32_t t = (uint32_t)(keys[k] & 0xFFFFFFFFull);
@@ -242,9 +244,11 @@ __global__ void k_write_tris(const float *pos, const uint32_t *vidx, const uint3
     V3 v2 = v3(pos[3 * ic], pos[3 * ic + 1], pos[3 * ic + 2]);
     V3 e1 = v0 - v1, e2 = v2 - v0;
     tris[3 * k + 0] = make_float4(v0.x, v0.y, v0.z, __uint_as_float(t));
-    tris[3 * k + 1] = make_float4(e1.x, e1.y, e1.z, __uint_as_float(0u));
+    tris[3 * k + 1] = make_float4(e1.x, e1.y, e1.z, __uint_as_float(tri_flags ? (tri_flags[t] & 2u) : 0u));
     tris[3 * k + 2] = make_float4(e2.x, e2.y, e2.z, __uint_as_float(tri_geom[t]));
 }
+
+LT_D void mark_leaf_end(float4 *tris, int k) { tris[3 * k + 1].w = __uint_as_float(__float_as_uint(tris[3 * k + 1].w) | 1u); }
 
 LT_D void emit_child(const BuildArrays &a, int c, float lo[3], float hi[3], int &ptr, int &last, bool &leaf) {
     float cost, ar;
@@ -278,8 +282,8 @@ __global__ void k_emit(BuildArrays a, const int *emit, float4 *nodes, float4 *tr
     nodes[4 * i + 1] = make_float4(l1[0], h1[0], l1[1], h1[1]);
     nodes[4 * i + 2] = make_float4(l0[2], h0[2], l1[2], h1[2]);
     nodes[4 * i + 3] = make_float4(__int_as_float(p0), __int_as_float(p1), 0.0f, 0.0f);
-    if (leaf0) tris[3 * last0 + 1].w = __uint_as_float(1u);
-    if (leaf1) tris[3 * last1 + 1].w = __uint_as_float(1u);
+    if (leaf0) mark_leaf_end(tris, last0);
+    if (leaf1) mark_leaf_end(tris, last1);
 }
 
 /* Root for scenes whose whole triangle set is one leaf: both child slots
@@ -301,7 +305,7 @@ __global__ void k_single_leaf_root(const float *pos, const uint32_t *vidx, const
     nodes[1] = make_float4(lo[0], hi[0], lo[1], hi[1]);
     nodes[2] = make_float4(lo[2], hi[2], lo[2], hi[2]);
     nodes[3] = make_float4(__int_as_float(~0), __int_as_float(~0), 0.0f, 0.0f);
-    tris[3 * (n - 1) + 1].w = __uint_as_float(1u);
+    mark_leaf_end(tris, (int)n - 1);
 }
 
 
@@ -326,6 +330,7 @@ struct WideArgs {
     float4 *wnodes;
     float4 *wtris;
     const uint32_t *tri_geom;
+    const uint32_t *tri_flags;   /* nullable: bit 1 = second half of a quad */
     unsigned *counters;   /* [0] wide nodes allocated, [1] triangles placed */
     int begin, end;
 };
@@ -436,7 +441,7 @@ __global__ void k_wide_pass(WideArgs w) {
                 V3 e1 = v0 - v1, e2 = v2 - v0;
                 size_t o = 3 * (size_t)(tri_base + off + t);
                 w.wtris[o + 0] = make_float4(v0.x, v0.y, v0.z, __uint_as_float(tid));
-                w.wtris[o + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+                w.wtris[o + 1] = make_float4(e1.x, e1.y, e1.z, __uint_as_float(w.tri_flags ? (w.tri_flags[tid] & 2u) : 0u));
                 w.wtris[o + 2] = make_float4(e2.x, e2.y, e2.z, __uint_as_float(w.tri_geom[tid]));
             }
             off += cnt;
@@ -464,7 +469,8 @@ __global__ void k_wide_pass(WideArgs w) {
 
 /* Wide root for a scene whose triangles form a single leaf (<= max_leaf). */
 __global__ void k_wide_single_leaf(const float *pos, const uint32_t *vidx, const uint32_t *tri_geom,
-                                   const uint64_t *keys, uint32_t n, float4 *wnodes, float4 *wtris) {
+                                   const uint32_t *tri_flags, const uint64_t *keys, uint32_t n, float4 *wnodes,
+                                   float4 *wtris) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     float lo[3] = {__builtin_huge_valf(), __builtin_huge_valf(), __builtin_huge_valf()};
     float hi[3] = {-__builtin_huge_valf(), -__builtin_huge_valf(), -__builtin_huge_valf()};
@@ -482,7 +488,7 @@ __global__ void k_wide_single_leaf(const float *pos, const uint32_t *vidx, const
         V3 v2 = v3(pos[3 * ic], pos[3 * ic + 1], pos[3 * ic + 2]);
         V3 e1 = v0 - v1, e2 = v2 - v0;
         wtris[3 * t + 0] = make_float4(v0.x, v0.y, v0.z, __uint_as_float(tid));
-        wtris[3 * t + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+        wtris[3 * t + 1] = make_float4(e1.x, e1.y, e1.z, __uint_as_float(tri_flags ? (tri_flags[tid] & 2u) : 0u));
         wtris[3 * t + 2] = make_float4(e2.x, e2.y, e2.z, __uint_as_float(tri_geom[tid]));
     }
     int ee[3];
@@ -711,7 +717,8 @@ static int ploc_build(hipStream_t st, const float *d_pos, const uint32_t *d_vidx
     return LT_OK;
 }
 
-int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const uint32_t *d_tri_geom, uint32_t n,
+int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const uint32_t *d_tri_geom,
+              const uint32_t *d_tri_flags, uint32_t n,
               uint32_t max_leaf, uint32_t builder, DevBuf &nodes, DevBuf &tris, DevBuf &wnodes, DevBuf &wtris,
               BvhStats &stats) {
     stats = BvhStats();
@@ -751,10 +758,11 @@ int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const 
     if ((rc = wtris.alloc(sizeof(float4) * 3 * (size_t)n))) return rc;
     if ((rc = wnodes.alloc(sizeof(float4) * 5 * (size_t)std::max<uint32_t>(1u, n - 1)))) return rc;
     if (n == 1) {
-        hipLaunchKernelGGL(k_write_tris, dim3(1), dim3(B), 0, st, d_pos, d_vidx, d_tri_geom, sk, n, (float4 *)tris.p);
+        hipLaunchKernelGGL(k_write_tris, dim3(1), dim3(B), 0, st, d_pos, d_vidx, d_tri_geom, d_tri_flags, sk, n,
+                           (float4 *)tris.p);
         hipLaunchKernelGGL(k_single_leaf_root, dim3(1), dim3(64), 0, st, d_pos, d_vidx, sk, n, (float4 *)nodes.p,
                            (float4 *)tris.p);
-        hipLaunchKernelGGL(k_wide_single_leaf, dim3(1), dim3(64), 0, st, d_pos, d_vidx, d_tri_geom, sk, n,
+        hipLaunchKernelGGL(k_wide_single_leaf, dim3(1), dim3(64), 0, st, d_pos, d_vidx, d_tri_geom, d_tri_flags, sk, n,
                            (float4 *)wnodes.p, (float4 *)wtris.p);
         LT_HIP(hipGetLastError());
         stats.wide_nodes = 1;
@@ -789,7 +797,7 @@ int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const 
     LT_HIP(hipMemsetAsync(done.p, 0, sizeof(int) * ni, st));
     LT_HIP(hipMemsetAsync(collapse.p, 0, sizeof(int) * ni, st));
     LT_HIP(hipMemsetAsync(dstats.p, 0, sizeof(unsigned) * 4, st));
-    hipLaunchKernelGGL(k_write_tris, dim3(grid_for(n, B)), dim3(B), 0, st, d_pos, d_vidx, d_tri_geom, sk, n,
+    hipLaunchKernelGGL(k_write_tris, dim3(grid_for(n, B)), dim3(B), 0, st, d_pos, d_vidx, d_tri_geom, d_tri_flags, sk, n,
                        (float4 *)tris.p);
     LT_HIP(hipGetLastError());
     BuildArrays a;
@@ -822,7 +830,7 @@ int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const 
     if (root_collapse) {
         hipLaunchKernelGGL(k_single_leaf_root, dim3(1), dim3(64), 0, st, d_pos, d_vidx, sk, n, (float4 *)nodes.p,
                            (float4 *)tris.p);
-        hipLaunchKernelGGL(k_wide_single_leaf, dim3(1), dim3(64), 0, st, d_pos, d_vidx, d_tri_geom, sk, n,
+        hipLaunchKernelGGL(k_wide_single_leaf, dim3(1), dim3(64), 0, st, d_pos, d_vidx, d_tri_geom, d_tri_flags, sk, n,
                            (float4 *)wnodes.p, (float4 *)wtris.p);
         LT_HIP(hipGetLastError());
         stats.wide_nodes = 1;
@@ -858,6 +866,7 @@ int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const 
     wa.wnodes = (float4 *)wnodes.p;
     wa.wtris = (float4 *)wtris.p;
     wa.tri_geom = d_tri_geom;
+    wa.tri_flags = d_tri_flags;
     wa.counters = (unsigned *)wcnt.p;
     int begin = 0, end = 1, levels = 0;
     while (begin < end) {

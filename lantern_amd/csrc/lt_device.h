@@ -124,8 +124,10 @@ LT_D float exp_f(float x) {
 /* Embree's Moeller-Trumbore with Ng = cross(e2, e1)
  * (triangle_intersector_moeller.h:75-113, :115-124) and the finalize of
  * MoellerTrumboreHitM (:43-48) as a correctly rounded division. */
+/* flip: the triangle is the second half (v2, v3, v1) of an Embree quad, whose
+ * hit reports U' = |den| - U, V' = |den| - V (quad_intersector_moeller.h:150-153). */
 LT_D bool moeller(const V3 &O, const V3 &D, float tnear, float tfar, const V3 &v0, const V3 &e1, const V3 &e2, float &t,
-                  float &u, float &v) {
+                  float &u, float &v, bool flip = false) {
     V3 Ng = cross(e2, e1);
     V3 C = v0 - O;
     V3 R = cross(C, D);
@@ -142,6 +144,10 @@ LT_D bool moeller(const V3 &O, const V3 &D, float tnear, float tfar, const V3 &v
     float T = dot(Ng, C);
     if (neg) T = -T;
     if (!((absDen * tnear < T) && (T <= absDen * tfar))) return false;
+    if (flip) {
+        U = absDen - U;
+        V = absDen - V;
+    }
     t = T / absDen;
     u = U / absDen;
     v = V / absDen;

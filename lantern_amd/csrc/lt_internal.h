@@ -57,7 +57,8 @@ struct BvhStats {
 
 /* Builds the binary hierarchy (nodes, tris) and its 8-wide compressed
  * collapse (wnodes, wtris). */
-int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const uint32_t *d_tri_geom, uint32_t n,
+int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const uint32_t *d_tri_geom,
+              const uint32_t *d_tri_flags, uint32_t n,
               uint32_t max_leaf, uint32_t builder, DevBuf &nodes, DevBuf &tris, DevBuf &wnodes, DevBuf &wtris,
               BvhStats &stats);
 
@@ -139,6 +140,7 @@ struct lt_scene {
     std::vector<float> pos, nrm, uv;
     std::vector<uint32_t> vidx;       /* global vertex indices, 3 per global triangle */
     std::vector<uint32_t> tri_geom;
+    std::vector<uint32_t> tri_flags;  /* per global triangle: bit 1 = second half of a quad */
     std::vector<lt::DGeom> geoms;
     std::vector<uint32_t> geom_nv, geom_vbase;
     std::vector<lt::DMaterial> mats;
@@ -153,7 +155,7 @@ struct lt_scene {
     bool params_on_device = false;   /* texels/voxels updated on the device after the last commit */
     /* device copies */
     std::vector<float> light_tris;    /* 12 floats per light triangle (v0|prim, e1, e2) */
-    lt::DevBuf d_pos, d_nrm, d_uv, d_vidx, d_tri_geom, d_geoms, d_mats, d_media, d_texs, d_texels, d_grids,
+    lt::DevBuf d_pos, d_nrm, d_uv, d_vidx, d_tri_geom, d_tri_flags, d_geoms, d_mats, d_media, d_texs, d_texels, d_grids,
         d_voxels, d_lights, d_nodes, d_tris, d_light_tris, d_wnodes, d_wtris;
     uint32_t layout = 0;   /* 0: 8-wide compressed nodes, 1: binary nodes */
     lt_build_params build = {};   /* parameters of the last commit */

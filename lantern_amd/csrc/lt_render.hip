@@ -255,7 +255,8 @@ __device__ __forceinline__ bool light_hit(const DScene &sc, const DLight &L, con
         const float4 *tp = sc.light_tris + 3 * (size_t)(L.tri_offset + k);
         float4 ta = tp[0], tb = tp[1], tc = tp[2];
         float t, u, v;
-        if (moeller(o, d, 0.001f, tfar, v3(ta.x, ta.y, ta.z), v3(tb.x, tb.y, tb.z), v3(tc.x, tc.y, tc.z), t, u, v)) {
+        if (moeller(o, d, 0.001f, tfar, v3(ta.x, ta.y, ta.z), v3(tb.x, tb.y, tb.z), v3(tc.x, tc.y, tc.z), t, u, v,
+                    (__float_as_uint(tb.w) & 2u) != 0u)) {
             tfar = t;
             h.t = t;
             h.u = u;
@@ -387,9 +388,9 @@ __global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur
                         else C = C + T * v3(L.le[0], L.le[1], L.le[2]);
                     }
                     it.P = origin + dir * h.t;
-                    if (g.flags & 1u) it.N = normalize(interp_normal(sc, h.prim, h.u, h.v));
+                    if (g.flags & 1u) it.N = normalize(interp_normal(sc, h.prim, h.u, h.v, g));
                     else it.N = normalize(v3s(0.0f));
-                    if (g.flags & 2u) interp_uv(sc, h.prim, h.u, h.v, it.tu, it.tv);
+                    if (g.flags & 2u) interp_uv(sc, h.prim, h.u, h.v, g, it.tu, it.tv);
                     else it.tu = it.tv = 0.0f;
                     it.wo = -dir;
                     it.ioro = 0.0f;
@@ -554,6 +555,7 @@ __global__ __launch_bounds__(256) void k_resolve(WaveArgs a, uint32_t b, Queue c
         const DScene &sc = BATCH ? a.batch.scenes[cur.sid[q]] : a.sc;
         uint32_t li = info & 0x3FFFFFFFu;
         const DLight L = sc.lights[li];
+        const DGeom lg = sc.geoms[L.geom];
         V3 le = v3(L.le[0], L.le[1], L.le[2]);
         V3 direct = v3s(0.0f);
         if (info & 0x40000000u) {
@@ -567,7 +569,7 @@ __global__ __launch_bounds__(256) void k_resolve(WaveArgs a, uint32_t b, Queue c
                 V3 ip = o + normalize(d) * h.t;
                 V3 dd = ip - o;
                 float d2 = dot(dd, dd);
-                float lpdf = d2 / (fabsf(dot(normalize(interp_normal(sc, h.prim, h.u, h.v)), -d)) * L.area);
+                float lpdf = d2 / (fabsf(dot(normalize(interp_normal(sc, h.prim, h.u, h.v, lg)), -d)) * L.area);
                 V3 f1 = v3(nee.f1x[q], nee.f1y[q], nee.f1z[q]);
                 float p1 = nee.p1[q];
                 if (lpdf != 0.0f && !is_black(le) && p1 != 0.0f && !is_black(f1)) {
@@ -587,7 +589,7 @@ __global__ __launch_bounds__(256) void k_resolve(WaveArgs a, uint32_t b, Queue c
                 V3 ip = o + normalize(d) * h.t;
                 V3 dd = ip - o;
                 float d2 = dot(dd, dd);
-                float lpdf = d2 / (fabsf(dot(normalize(interp_normal(sc, h.prim, h.u, h.v)), d)) * L.area);
+                float lpdf = d2 / (fabsf(dot(normalize(interp_normal(sc, h.prim, h.u, h.v, lg)), d)) * L.area);
                 if (lpdf != 0.0f) {
                     float p2 = nee.p2[q];
                     V3 f2 = v3(nee.f2x[q], nee.f2y[q], nee.f2z[q]);

@@ -176,22 +176,50 @@ LT_D void bsdf_sample(const DMaterial &m, Interaction &it, PathRng &s) {
 
 /* rtcInterpolate1: P = fma(w,p0, fma(u,p1, v*p2)), w = 1-u-v
  * (scene_triangle_mesh.cpp:241-247). */
-LT_D V3 interp_normal(const DScene &sc, uint32_t gprim, float u, float v) {
-    const uint32_t *ix = sc.tri_vidx + 3 * (size_t)gprim;
-    uint32_t a = ix[0], b = ix[1], c = ix[2];
-    const float *n = sc.normals;
-    float w = 1.0f - u - v;
-    return v3(__builtin_fmaf(w, n[3 * a], __builtin_fmaf(u, n[3 * b], v * n[3 * c])),
-              __builtin_fmaf(w, n[3 * a + 1], __builtin_fmaf(u, n[3 * b + 1], v * n[3 * c + 1])),
-              __builtin_fmaf(w, n[3 * a + 2], __builtin_fmaf(u, n[3 * b + 2], v * n[3 * c + 2])));
+/* Vertices and weights of the point (u, v) of global primitive gprim: the
+ * triangle's barycentrics, or for a quad mesh (DGeom flag 4; its quads are
+ * stored as the triangle pairs (v0, v1, v3), (v2, v3, v1)) the half of the
+ * quad that (u, v) lies on (QuadMesh::interpolate, scene_quad_mesh.cpp:238-255:
+ * u + v <= 1 uses (p0, p1, p3) with (u, v), else (p2, p3, p1) with
+ * (1 - u, 1 - v)). */
+struct InterpW {
+    uint32_t a, b, c;
+    float w, u, v;
+};
+LT_D InterpW interp_setup(const DScene &sc, uint32_t gprim, float u, float v, const DGeom &g) {
+    InterpW r;
+    if (!(g.flags & 4u)) {
+        const uint32_t *ix = sc.tri_vidx + 3 * (size_t)gprim;
+        r.a = ix[0]; r.b = ix[1]; r.c = ix[2];
+        r.u = u; r.v = v;
+    } else {
+        const uint32_t t0 = gprim - ((gprim - g.tbase) & 1u);
+        const uint32_t *ix = sc.tri_vidx + 3 * (size_t)t0;
+        const uint32_t p0 = ix[0], p1 = ix[1], p3 = ix[2], p2 = ix[3];
+        if (u + v <= 1.0f) {
+            r.a = p0; r.b = p1; r.c = p3;
+            r.u = u; r.v = v;
+        } else {
+            r.a = p2; r.b = p3; r.c = p1;
+            r.u = 1.0f - u; r.v = 1.0f - v;
+        }
+    }
+    r.w = 1.0f - r.u - r.v;
+    return r;
 }
-LT_D void interp_uv(const DScene &sc, uint32_t gprim, float u, float v, float &ou, float &ov) {
-    const uint32_t *ix = sc.tri_vidx + 3 * (size_t)gprim;
-    uint32_t a = ix[0], b = ix[1], c = ix[2];
+/* TriangleMesh/QuadMesh::interpolate: P = fma(w, p0, fma(u, p1, v * p2)). */
+LT_D V3 interp_normal(const DScene &sc, uint32_t gprim, float u, float v, const DGeom &g) {
+    const InterpW q = interp_setup(sc, gprim, u, v, g);
+    const float *n = sc.normals;
+    return v3(__builtin_fmaf(q.w, n[3 * q.a], __builtin_fmaf(q.u, n[3 * q.b], q.v * n[3 * q.c])),
+              __builtin_fmaf(q.w, n[3 * q.a + 1], __builtin_fmaf(q.u, n[3 * q.b + 1], q.v * n[3 * q.c + 1])),
+              __builtin_fmaf(q.w, n[3 * q.a + 2], __builtin_fmaf(q.u, n[3 * q.b + 2], q.v * n[3 * q.c + 2])));
+}
+LT_D void interp_uv(const DScene &sc, uint32_t gprim, float u, float v, const DGeom &g, float &ou, float &ov) {
+    const InterpW q = interp_setup(sc, gprim, u, v, g);
     const float *t = sc.uvs;
-    float w = 1.0f - u - v;
-    ou = __builtin_fmaf(w, t[2 * a], __builtin_fmaf(u, t[2 * b], v * t[2 * c]));
-    ov = __builtin_fmaf(w, t[2 * a + 1], __builtin_fmaf(u, t[2 * b + 1], v * t[2 * c + 1]));
+    ou = __builtin_fmaf(q.w, t[2 * q.a], __builtin_fmaf(q.u, t[2 * q.b], q.v * t[2 * q.c]));
+    ov = __builtin_fmaf(q.w, t[2 * q.a + 1], __builtin_fmaf(q.u, t[2 * q.b + 1], q.v * t[2 * q.c + 1]));
 }
 
 /* Nearest-voxel density of a grid medium (no reference counterpart). */

@@ -155,7 +155,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, TraceBatch batch, co
                     float t, u, v;
                     if ((!ANY || __float_as_uint(c.w) != ignore) &&
                         moeller(r.o, r.d, r.tnear, r.tfar, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t,
-                                u, v)) {
+                                u, v, (__float_as_uint(b.w) & 2u) != 0u)) {
                         r.tfar = t;
                         best_t = t;
                         best_u = u;
@@ -344,7 +344,7 @@ void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__
                         float t, u, v;
                         if ((!ANY || __float_as_uint(c.w) != ignore) &&
                             moeller(r.o, r.d, r.tnear, r.tfar, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z),
-                                    t, u, v)) {
+                                    t, u, v, (__float_as_uint(b.w) & 2u) != 0u)) {
                             r.tfar = t;
                             best_t = t;
                             best_u = u;

@@ -443,3 +443,54 @@ def batch_scenes(n=64, seed=DEFAULT_SEED, spp=32, max_depth=16, subdivs=(5, 6, 7
                       "dielectric"][kind], width=w, height=h)
         out.append(d)
     return out
+
+
+def quad_mesh(nx, nz, width, depth, bump=0.0, seed=0):
+    """An nx x nz grid of quads (4 indices each, counter-clockwise seen from
+    +y) over [-width/2, width/2] x [-depth/2, depth/2] at y = 0, vertex heights
+    perturbed by U[-bump, bump] (non-planar quads), per-vertex normals from the
+    height field and uvs in [0, 1]^2 - the layout of an LMF file with
+    VerticesPerPrimative == 4."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    xs = np.linspace(-width / 2, width / 2, nx + 1)
+    zs = np.linspace(-depth / 2, depth / 2, nz + 1)
+    X, Z = np.meshgrid(xs, zs, indexing="xy")
+    Y = rng.uniform(-bump, bump, X.shape) if bump else np.zeros_like(X)
+    pos = np.stack([X, Y, Z], -1).reshape(-1, 3).astype(np.float32)
+    gz, gx = np.gradient(Y, zs, xs)
+    n = np.stack([-gx, np.ones_like(Y), -gz], -1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    nrm = n.reshape(-1, 3).astype(np.float32)
+    U, V = np.meshgrid(np.linspace(0, 1, nx + 1), np.linspace(0, 1, nz + 1), indexing="xy")
+    uv = np.stack([U, V], -1).reshape(-1, 2).astype(np.float32)
+    idx = []
+    for j in range(nz):
+        for i in range(nx):
+            a = j * (nx + 1) + i
+            idx.append([a, a + nx + 1, a + nx + 2, a + 1])
+    return pos, nrm, uv, np.asarray(idx, np.uint32)
+
+
+def quad_scene(width=64, height=64, spp=4, seed=DEFAULT_SEED, max_depth=6):
+    """Quad meshes end to end: a bumpy 8x8-quad floor whose albedo is its
+    interpolated uv (UV texture), a Lambert box, and a quad light (one
+    CreateGrid-shaped quad given as a 4-index primitive, so its area follows
+    the reference's quad formula)."""
+    d = SceneDesc()
+    floor = d.add_material(bsdf=_capi.LT_BSDF_LAMBERT, albedo=(1.0, 1.0, 1.0), texture=_capi.LT_TEX_UV)
+    grey = d.add_material(bsdf=_capi.LT_BSDF_LAMBERT, albedo=(0.6, 0.6, 0.6))
+    light_mat = d.add_material(bsdf=_capi.LT_BSDF_LAMBERT, albedo=(0.0, 0.0, 0.0))
+    p, n, uv, q = quad_mesh(8, 8, 8.0, 8.0, bump=0.3, seed=seed)
+    d.add_mesh(p, q, n, uv, floor)
+    bp, bn, buv, bi = box(1.0, 1.0, 1.0)
+    bp, bn = transform(bp, bn, translate(0.0, 0.8, 0.0))
+    d.add_mesh(bp, bi, bn, buv, grey)
+    lp = np.array([[-1, 3, -1], [1, 3, -1], [1, 3, 1], [-1, 3, 1]], np.float32)
+    ln = np.tile(np.array([[0, -1, 0]], np.float32), (4, 1))
+    d.add_mesh(lp, np.array([[0, 1, 2, 3]], np.uint32), ln, None, light_mat, emission=((1.0, 1.0, 1.0), 40.0))
+    d.background = (0.2, 0.2, 0.2)
+    d.camera = dict(phi=0.9, theta=0.4, radius=7.0, width=width, height=height, target=(0.0, 0.5, 0.0),
+                    fov=math.pi / 4, filter=0)
+    d.params = dict(spp=spp, max_depth=max_depth, rr_start=4, seed=seed)
+    d.meta = dict(workload="quad_scene", seed=seed)
+    return d

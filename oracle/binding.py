@@ -71,6 +71,7 @@ def lib():
             "orc_scene_create": (C.c_void_p, []),
             "orc_scene_destroy": (None, [C.c_void_p]),
             "orc_scene_add_mesh": (C.c_int32, [C.c_void_p, _pf, C.c_uint32, _pu, C.c_uint32, _pf, _pf]),
+            "orc_scene_add_quads": (C.c_int32, [C.c_void_p, _pf, C.c_uint32, _pu, C.c_uint32, _pf, _pf]),
             "orc_scene_add_material": (C.c_int32, [C.c_void_p, C.POINTER(orc_material_desc)]),
             "orc_scene_add_medium": (C.c_int32, [C.c_void_p, C.POINTER(orc_medium_desc)]),
             "orc_scene_add_texture": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, _pf]),
@@ -131,7 +132,7 @@ def camera(cam):
 
 
 class OracleScene:
-    """The oracle's copy of a SceneDesc (triangles only; quads are split (0,1,2),(0,2,3))."""
+    """The oracle's copy of a SceneDesc (triangle and quad meshes)."""
 
     def __init__(self, desc):
         L = lib()
@@ -158,13 +159,12 @@ class OracleScene:
         for mesh in desc.meshes:
             pos = np.ascontiguousarray(mesh["pos"], np.float32)
             idx = np.ascontiguousarray(mesh["idx"], np.uint32)
-            if idx.shape[1] == 4:
-                idx = np.ascontiguousarray(np.concatenate([idx[:, [0, 1, 2]], idx[:, [0, 2, 3]]], axis=1).reshape(-1, 3))
             nrm = mesh.get("nrm")
             uv = mesh.get("uv")
             nrm = None if nrm is None else np.ascontiguousarray(nrm, np.float32)
             uv = None if uv is None else np.ascontiguousarray(uv, np.float32)
-            g = L.orc_scene_add_mesh(self.h, fp(pos), pos.shape[0], up(idx), idx.shape[0], fp(nrm), fp(uv))
+            add = L.orc_scene_add_quads if idx.shape[1] == 4 else L.orc_scene_add_mesh
+            g = add(self.h, fp(pos), pos.shape[0], up(idx), idx.shape[0], fp(nrm), fp(uv))
             if g < 0:
                 raise RuntimeError("orc_scene_add_mesh failed")
             L.orc_scene_set_material(self.h, g, mesh["material"])

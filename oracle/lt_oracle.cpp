@@ -489,8 +489,9 @@ float surface_area(const V3<float> *p, const u32 *idx, size_t ntris) {
 /* Scene                                                                     */
 /* ------------------------------------------------------------------------ */
 struct Geom {
-    u32 vbase, nv, tbase, nt;
+    u32 vbase, nv, tbase, nt;     /* nt: triangles (two per quad) */
     bool has_nrm, has_uv;
+    bool quad;                    /* quads stored as triangle pairs (v0, v1, v3), (v2, v3, v1) */
     int material;
     int light;
 };
@@ -516,7 +517,7 @@ struct Medium {
     int grid;
     float scale, albedo, g;
 };
-struct Tri { V3<float> v0, e1, e2, v1, v2; u32 geom, prim; };
+struct Tri { V3<float> v0, e1, e2, v1, v2; u32 geom, prim; bool flip; /* second half of a quad */ };
 
 struct BvhNode { float lo[3], hi[3]; int left; int count; }; /* count>0: leaf, left = first tri */
 
@@ -559,7 +560,7 @@ template <class R> struct HitR {
  * and writes t,u,v when the triangle is accepted against [tnear, tfar]. */
 template <class R>
 inline bool moeller(const V3<R> &O, const V3<R> &D, R tnear, R tfar, const V3<R> &v0, const V3<R> &e1,
-                    const V3<R> &e2, R &t, R &u, R &v) {
+                    const V3<R> &e2, R &t, R &u, R &v, bool flip = false) {
     V3<R> Ng = cross(e2, e1);
     V3<R> C = v0 - O;
     V3<R> Rv = cross(C, D);
@@ -573,6 +574,10 @@ inline bool moeller(const V3<R> &O, const V3<R> &D, R tnear, R tfar, const V3<R>
     R T = dot(Ng, C);
     if (neg) T = -T;
     if (!((absDen * tnear < T) && (T <= absDen * tfar))) return false;
+    if (flip) {   /* second triangle of a quad (quad_intersector_moeller.h:150-153) */
+        U = absDen - U;
+        V = absDen - V;
+    }
     t = T / absDen;
     u = U / absDen;
     v = V / absDen;
@@ -599,7 +604,7 @@ bool intersect_brute(const orc_scene &s, const RayR<R> &ray, HitR<R> &hit, u32 *
         V3<R> v0, e1, e2;
         tri_edges<R>(s.tris[i], v0, e1, e2);
         R t, u, v;
-        if (moeller(ray.o, ray.d, ray.tnear, tfar, v0, e1, e2, t, u, v)) {
+        if (moeller(ray.o, ray.d, ray.tnear, tfar, v0, e1, e2, t, u, v, s.tris[i].flip)) {
             tfar = t;
             hit.t = t; hit.u = u; hit.v = v;
             hit.geom = s.tris[i].geom;
@@ -618,7 +623,7 @@ bool intersect_brute(const orc_scene &s, const RayR<R> &ray, HitR<R> &hit, u32 *
                 V3<R> v0, e1, e2;
                 tri_edges<R>(s.tris[i], v0, e1, e2);
                 R t, u, v;
-                if (moeller(ray.o, ray.d, ray.tnear, hit.t, v0, e1, e2, t, u, v)) ++n;
+                if (moeller(ray.o, ray.d, ray.tnear, hit.t, v0, e1, e2, t, u, v, s.tris[i].flip)) ++n;
             }
         *ties = n;
     }
@@ -781,7 +786,7 @@ bool intersect_bvh(const orc_scene &s, const RayR<R> &ray, HitR<R> &hit) {
                 V3<R> v0, e1, e2;
                 tri_edges<R>(s.tris[ti], v0, e1, e2);
                 R t, u, v;
-                if (moeller(ray.o, ray.d, ray.tnear, tfar, v0, e1, e2, t, u, v)) {
+                if (moeller(ray.o, ray.d, ray.tnear, tfar, v0, e1, e2, t, u, v, s.tris[ti].flip)) {
                     tfar = t;
                     hit.t = t; hit.u = u; hit.v = v;
                     hit.geom = s.tris[ti].geom;
@@ -819,20 +824,45 @@ template <class R> inline bool intersect(const orc_scene &s, const RayR<R> &r, H
 
 /* rtcInterpolate1 / TriangleMesh::interpolate (scene_triangle_mesh.cpp:241-247):
  * P = fma(w,p0, fma(u,p1, v*p2)), w = 1-u-v. */
-template <class R> inline V3<R> interp_normal(const orc_scene &s, u32 geom, u32 prim, R u, R v) {
+/* Vertices and weights of (u, v) on primitive prim of geometry geom: the
+ * triangle's barycentrics, or for quads the half that (u, v) lies on
+ * (QuadMesh::interpolate, scene_quad_mesh.cpp:238-255). */
+template <class R> inline void interp_weights(const orc_scene &s, u32 geom, u32 prim, R u, R v, u32 ix[3], R w[3]) {
     const Geom &g = s.geoms[geom];
-    const u32 *ix = &s.idx[3 * (g.tbase + prim)];
+    if (!g.quad) {
+        const u32 *t = &s.idx[3 * (g.tbase + prim)];
+        ix[0] = t[0]; ix[1] = t[1]; ix[2] = t[2];
+        w[1] = u; w[2] = v;
+    } else {
+        const u32 *t = &s.idx[3 * (g.tbase + 2 * prim)];      /* (v0, v1, v3), then (v2, v3, v1) */
+        const u32 p0 = t[0], p1 = t[1], p3 = t[2], p2 = t[3];
+        if (u + v <= R(1)) {
+            ix[0] = p0; ix[1] = p1; ix[2] = p3;
+            w[1] = u; w[2] = v;
+        } else {
+            ix[0] = p2; ix[1] = p3; ix[2] = p1;
+            w[1] = R(1) - u; w[2] = R(1) - v;
+        }
+    }
+    w[0] = R(1) - w[1] - w[2];
+}
+
+/* rtcInterpolate1 / TriangleMesh::interpolate (scene_triangle_mesh.cpp:241-247):
+ * P = fma(w,p0, fma(u,p1, v*p2)), w = 1-u-v. */
+template <class R> inline V3<R> interp_normal(const orc_scene &s, u32 geom, u32 prim, R u, R v) {
+    u32 ix[3];
+    R w[3];
+    interp_weights<R>(s, geom, prim, u, v, ix, w);
     V3<R> p0 = cvt<R>(s.nrm[ix[0]]), p1 = cvt<R>(s.nrm[ix[1]]), p2 = cvt<R>(s.nrm[ix[2]]);
-    R w = R(1) - u - v;
-    return V3<R>(fmaR(w, p0.x, fmaR(u, p1.x, v * p2.x)), fmaR(w, p0.y, fmaR(u, p1.y, v * p2.y)),
-                 fmaR(w, p0.z, fmaR(u, p1.z, v * p2.z)));
+    return V3<R>(fmaR(w[0], p0.x, fmaR(w[1], p1.x, w[2] * p2.x)), fmaR(w[0], p0.y, fmaR(w[1], p1.y, w[2] * p2.y)),
+                 fmaR(w[0], p0.z, fmaR(w[1], p1.z, w[2] * p2.z)));
 }
 template <class R> inline void interp_uv(const orc_scene &s, u32 geom, u32 prim, R u, R v, R &a, R &b) {
-    const Geom &g = s.geoms[geom];
-    const u32 *ix = &s.idx[3 * (g.tbase + prim)];
-    R w = R(1) - u - v;
-    a = fmaR(w, (R)s.uv[2 * ix[0]], fmaR(u, (R)s.uv[2 * ix[1]], v * (R)s.uv[2 * ix[2]]));
-    b = fmaR(w, (R)s.uv[2 * ix[0] + 1], fmaR(u, (R)s.uv[2 * ix[1] + 1], v * (R)s.uv[2 * ix[2] + 1]));
+    u32 ix[3];
+    R w[3];
+    interp_weights<R>(s, geom, prim, u, v, ix, w);
+    a = fmaR(w[0], (R)s.uv[2 * ix[0]], fmaR(w[1], (R)s.uv[2 * ix[1]], w[2] * (R)s.uv[2 * ix[2]]));
+    b = fmaR(w[0], (R)s.uv[2 * ix[0] + 1], fmaR(w[1], (R)s.uv[2 * ix[1] + 1], w[2] * (R)s.uv[2 * ix[2] + 1]));
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1563,6 +1593,7 @@ int32_t orc_scene_add_mesh(orc_scene *s, const float *pos, uint32_t nv, const ui
     g.nt = ntris;
     g.has_nrm = nrm != nullptr;
     g.has_uv = uv != nullptr;
+    g.quad = false;
     g.material = -1;
     g.light = -1;
     for (u32 i = 0; i < nv; ++i) {
@@ -1579,6 +1610,21 @@ int32_t orc_scene_add_mesh(orc_scene *s, const float *pos, uint32_t nv, const ui
     s->committed = false;
     return (int32_t)s->geoms.size() - 1;
 }
+/* A quad mesh (RTC_GEOMETRY_TYPE_QUAD, scene.cpp:508-511): stored as the
+ * triangle pairs Embree's quad intersector tests, (v0, v1, v3) and (v2, v3, v1). */
+int32_t orc_scene_add_quads(orc_scene *s, const float *pos, uint32_t nv, const uint32_t *idx, uint32_t nquads,
+                            const float *nrm, const float *uv) {
+    std::vector<uint32_t> tri(6 * (size_t)nquads);
+    for (uint32_t q = 0; q < nquads; ++q) {
+        const uint32_t *x = idx + 4 * (size_t)q;
+        const uint32_t t[6] = {x[0], x[1], x[3], x[2], x[3], x[1]};
+        std::copy(t, t + 6, tri.begin() + 6 * (size_t)q);
+    }
+    int32_t g = orc_scene_add_mesh(s, pos, nv, tri.data(), 2 * nquads, nrm, uv);
+    if (g >= 0) s->geoms[g].quad = true;
+    return g;
+}
+
 int32_t orc_scene_add_material(orc_scene *s, const orc_material_desc *m) {
     s->mats.push_back(*m);
     return (int32_t)s->mats.size() - 1;
@@ -1631,7 +1677,19 @@ int32_t orc_scene_set_emission(orc_scene *s, int32_t geom, const float color[3],
     Light L;
     std::vector<u32> local(3 * g.nt);
     for (u32 i = 0; i < 3 * g.nt; ++i) local[i] = s->idx[3 * g.tbase + i] - g.vbase;
-    float area = surface_area(&s->pos[g.vbase], local.data(), g.nt);
+    float area = 0.0f;
+    if (!g.quad) {
+        area = surface_area(&s->pos[g.vbase], local.data(), g.nt);
+    } else {
+        /* AddLMF (scene.cpp:540-548): 0.5|(v0-v1)x(v0-v2)| + 0.5|(v0-v2)x(v0-v3)| with
+         * v3 = vertices[Indices[i + 2]] (quirk A12): the second term is zero. */
+        for (u32 q = 0; 2 * q < g.nt; ++q) {
+            const u32 *t = &local[6 * q];
+            V3<float> v0 = s->pos[g.vbase + t[0]], v1 = s->pos[g.vbase + t[1]], v2 = s->pos[g.vbase + t[3]];
+            V3<float> v3 = v2;
+            area += 0.5f * length(cross(v0 - v1, v0 - v2)) + 0.5f * length(cross(v0 - v2, v0 - v3));
+        }
+    }
     bounding_sphere(&s->pos[g.vbase], g.nv, L.sphere);
     /* area_light.h:16-21: Le = color * power * (1/pi) / area */
     L.le = V3<float>(color[0] * radiant_power * (float)M_1_PI / area, color[1] * radiant_power * (float)M_1_PI / area,
@@ -1660,7 +1718,8 @@ int32_t orc_scene_commit(orc_scene *s) {
             t.v1 = v1;
             t.v2 = v2;
             t.geom = gi;
-            t.prim = p;
+            t.prim = g.quad ? p >> 1 : p;
+            t.flip = g.quad && (p & 1u);
             s->tris.push_back(t);
         }
     }

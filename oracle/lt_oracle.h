@@ -109,6 +109,9 @@ void orc_scene_destroy(orc_scene *s);
 /* Positions are already transformed (AddMesh applies the transform first). */
 int32_t orc_scene_add_mesh(orc_scene *s, const float *pos, uint32_t nv, const uint32_t *idx,
                            uint32_t ntris, const float *nrm, const float *uv);
+/* Quad mesh: idx holds 4 indices per quad (Embree QUAD semantics). */
+int32_t orc_scene_add_quads(orc_scene *s, const float *pos, uint32_t nv, const uint32_t *idx, uint32_t nquads,
+                            const float *nrm, const float *uv);
 int32_t orc_scene_add_material(orc_scene *s, const orc_material_desc *m);
 int32_t orc_scene_add_medium(orc_scene *s, const orc_medium_desc *m);
 int32_t orc_scene_add_texture(orc_scene *s, uint32_t w, uint32_t h, const float *rgb);

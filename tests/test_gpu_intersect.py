@@ -188,3 +188,18 @@ def test_axis_aligned_and_far_geometry(gpu_device, layout, builder):
     nhit, _ = _compare(gs, os_, np.concatenate([r, far]))
     assert nhit > 1000
     gs.close()
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_quad_meshes_match_brute_force(gpu_device, layout):
+    # non-planar quads: Embree's (v0, v1, v3) / (v2, v3, v1) split, quad (u, v)
+    # and quad primIDs; both BVH layouts
+    d = scenes.quad_scene()
+    gs = lantern_amd.Scene(gpu_device, d, layout=layout)
+    os_ = OracleScene(d)
+    rays = _random_rays(30000, 7, lo=-4, hi=4)
+    hits, ties = _compare(gs, os_, rays)
+    assert hits > 5000
+    rays = _camera_rays(d, 20000, 8)
+    _compare(gs, os_, rays)
+    gs.close()

@@ -109,3 +109,10 @@ def test_volume_box_small_matches_oracle(gpu_device):
     d = scenes.volume_box(64, 64, 2, res=32, max_depth=32)
     g, o, stats, rays = _render_both(gpu_device, d)
     _check(g, o, frac=0.995)
+
+
+def test_quad_scene_matches_oracle(gpu_device):
+    # quad floor with uv albedo (quad interpolation), quad light (reference quad area)
+    d = scenes.quad_scene(64, 64, 4)
+    g, o, stats, rays = _render_both(gpu_device, d)
+    _check(g, o)
