@@ -1,0 +1,80 @@
+"""Shared by the facade tests: the SceneDesc that the C++ facade's JSON loader
+builds from a scene file, derived independently here (scene.cpp:154-449:
+camera defaults, BSDF and material tables, primitives through the generators,
+the row-major transform applied to positions in float32 as AddMesh does,
+normals copied as given), and the facade driver's output reader."""
+import json
+import os
+
+import numpy as np
+
+from lantern_amd import _capi, scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "facade", "facade_main")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def build_driver():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "facade")], check=True)
+    return DRIVER
+
+
+def _apply(pos, T):
+    T = np.asarray(T, np.float32).reshape(4, 4)
+    x, y, z = (pos[:, k].astype(np.float32) for k in range(3))
+    out = np.empty_like(pos, dtype=np.float32)
+    for r in range(3):
+        out[:, r] = ((x * T[r, 0] + y * T[r, 1]) + z * T[r, 2]) + np.float32(1.0) * T[r, 3]
+    return out
+
+
+def desc_from_json(path):
+    j = json.load(open(path))
+    d = scenes.SceneDesc()
+    cam = j["camera"]
+    d.camera = dict(phi=cam.get("phi", np.pi / 4), theta=cam.get("theta", 0.0), radius=cam.get("radius", 10.0),
+                    width=int(cam["client_width"]), height=int(cam["client_height"]),
+                    target=tuple(cam.get("target", (0.0, 0.0, 0.0))), fov=cam.get("fov", np.pi / 4), filter=0)
+    kinds = {"lambert": _capi.LT_BSDF_LAMBERT, "mirror": _capi.LT_BSDF_MIRROR,
+             "ideal_specular_dielectric": _capi.LT_BSDF_DIELECTRIC, "ggx_conductor": _capi.LT_BSDF_GGX}
+    bsdfs = {}
+    for b in j["bsdfs"]:
+        a = b["albedo"]
+        m = dict(bsdf=kinds[b["type"]], ior=b.get("ior", 1.0), roughness=b.get("roughness", 0.0),
+                 albedo=tuple(a.get("value", (0.0, 0.0, 0.0))),
+                 texture=_capi.LT_TEX_UV if a["type"] == "uv" else _capi.LT_TEX_CONSTANT)
+        bsdfs[b["name"]] = m
+    mats = {}
+    for m in j["materials"]:
+        mats[m["name"]] = d.add_material(**bsdfs[m["bsdf"]])
+    for p in j["primitives"]:
+        t = p["type"]
+        nrm = uv = None
+        if t == "grid":
+            pos, nrm, uv, idx = scenes.grid(p["width"], p["depth"], p["m"], p["n"])
+        elif t == "geosphere":
+            pos, nrm, idx = scenes.geosphere(p["radius"], p["n"])
+        elif t == "box":
+            pos, nrm, uv, idx = scenes.box(p["width"], p["height"], p["depth"])
+        else:
+            raise ValueError(t)
+        if "transform" in p:
+            pos = _apply(pos, p["transform"])
+        em = None
+        if "emission" in p:
+            em = (tuple(p["emission"]["color"]), p["emission"]["radiant_power"])
+        d.add_mesh(pos, idx, nrm, uv, mats[p["material"]], emission=em)
+    d.background = tuple(j.get("background_color", (0.0, 0.0, 0.0)))
+    return d
+
+
+def read_output(path):
+    raw = open(path, "rb").read()
+    w, h = np.frombuffer(raw[:8], np.uint32)
+    n = int(w) * int(h)
+    s = np.frombuffer(raw[8:8 + 12 * n], np.float32).reshape(int(h), int(w), 3)
+    b = np.frombuffer(raw[8 + 12 * n:8 + 16 * n], np.uint32).reshape(int(h), int(w))
+    c = np.frombuffer(raw[8 + 16 * n:8 + 20 * n], np.uint32).reshape(int(h), int(w))
+    return s, b, c

@@ -1,0 +1,69 @@
+"""The C++ facade's scene loader (Scene::LoadSceneFromJSON, scene.cpp:154-449)
+on the CPU: schema validation (scenes/scene.schema.json's required fields,
+enumerations and additionalProperties) runs before any GPU work, so a valid
+file gets as far as the device check and an invalid one stops with the
+loader's message."""
+import copy
+import json
+import os
+import subprocess
+
+import pytest
+
+from tests.facade_common import GOLDEN, build_driver
+
+SCENE = os.path.join(GOLDEN, "facade_scene.json")
+
+
+def _run(driver, path, tmp_path):
+    r = subprocess.run([driver, path, "1", str(tmp_path / "out.bin")], capture_output=True, text=True, timeout=120)
+    return r.returncode, r.stdout + r.stderr
+
+
+@pytest.fixture(scope="module")
+def driver():
+    return build_driver()
+
+
+def test_valid_scene_passes_validation(driver, tmp_path):
+    rc, out = _run(driver, SCENE, tmp_path)
+    if rc == 0:
+        pytest.skip("a GPU is present: the GPU facade test covers this file")
+    assert rc == 2
+    assert "Schema validation failed" not in out and "Could not parse" not in out
+    assert "no GPU device" in out
+
+
+def _mutate(tmp_path, fn):
+    j = json.load(open(SCENE))
+    fn(j)
+    p = tmp_path / "scene.json"
+    p.write_text(json.dumps(j))
+    return str(p)
+
+
+CASES = [
+    (lambda j: j.pop("camera"), '"camera" is required'),
+    (lambda j: j["camera"].pop("client_width"), "camera requires client_width"),
+    (lambda j: j.__setitem__("extra", 1), "additional property not allowed: extra"),
+    (lambda j: j["bsdfs"][0].__setitem__("type", "velvet"), "unknown bsdf type: velvet"),
+    (lambda j: j["bsdfs"][2].pop("ior"), "dielectric bsdf requires ior"),
+    (lambda j: j["bsdfs"][0]["albedo"].__setitem__("type", "noise"), "unknown albedo type: noise"),
+    (lambda j: j["primitives"][1].__setitem__("transform", [1, 0, 0]), "transform must be float4x4"),
+    (lambda j: j["primitives"][0].pop("material"), "primitive requires name and material"),
+    (lambda j: j.__setitem__("background_color", [1, 2]), "background_color must be float3"),
+]
+
+
+@pytest.mark.parametrize("fn,msg", CASES)
+def test_schema_errors_are_reported(driver, tmp_path, fn, msg):
+    rc, out = _run(driver, _mutate(tmp_path, fn), tmp_path)
+    assert rc == 2
+    assert "Schema validation failed: " + msg in out
+
+
+def test_unparsable_file_is_reported(driver, tmp_path):
+    p = tmp_path / "broken.json"
+    p.write_text('{"camera": {"client_width": 4,')
+    rc, out = _run(driver, str(p), tmp_path)
+    assert rc == 2 and "Could not parse" in out
