@@ -198,6 +198,14 @@ int lt_film_download(lt_film *film, float *rgb_sum, uint32_t *bounces, uint32_t 
 int lt_film_upload(lt_film *film, const float *rgb_sum, const uint32_t *bounces, const uint32_t *counts);
 /* Resolved image sum/count (visualizer.cpp:355-370), w*h*3 floats. */
 int lt_film_resolve(lt_film *film, float *rgb);
+/* Tone mapping of the resolved film as the reference's final resolve shader
+ * (final_resolve_ps.glsl:24-44): rgba = T(exposure * (rgb, 1)) with T the
+ * clamp to [0, 1] or the filmic curve (alpha 1). rgba: 4 floats per pixel. */
+enum { LT_TONEMAP_CLAMP = 0, LT_TONEMAP_FILMIC = 1 };
+int lt_film_tonemap(lt_film *film, int32_t op, float exposure, float *rgba);
+/* The resolved film (sum / count) as a PFM file (RGB FP32, little endian,
+ * rows bottom to top); lt_film_upload of the raw triple resumes a render. */
+int lt_film_write_pfm(lt_film *film, const char *path);
 
 /* ---- rendering ------------------------------------------------------------ */
 /* PinholeCamera(phi, theta, radius, clientWidth, clientHeight, target, fov,

@@ -186,6 +186,15 @@ class Film:
         check(lib().lt_film_resolve(self.h, _fp(out)), "lt_film_resolve")
         return out
 
+    def tonemap(self, op=0, exposure=1.0):
+        """(h, w, 4) RGBA of the final resolve shader: op 0 clamp, 1 filmic."""
+        out = np.zeros((self.h_, self.w, 4), np.float32)
+        check(lib().lt_film_tonemap(self.h, op, exposure, _fp(out)), "lt_film_tonemap")
+        return out
+
+    def save_pfm(self, path):
+        check(lib().lt_film_write_pfm(self.h, str(path).encode()), "lt_film_write_pfm")
+
     def close(self):
         if self.h:
             lib().lt_film_destroy(self.h)

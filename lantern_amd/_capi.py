@@ -16,6 +16,7 @@ LT_INVALID_ID = 0xFFFFFFFF
 LT_BSDF_LAMBERT, LT_BSDF_MIRROR, LT_BSDF_DIELECTRIC, LT_BSDF_GGX = 0, 1, 2, 3
 LT_MEDIUM_NON_SCATTERING, LT_MEDIUM_ISOTROPIC, LT_MEDIUM_GRID = 0, 1, 2
 LT_TEX_CONSTANT, LT_TEX_UV = -1, -2
+LT_TONEMAP_CLAMP, LT_TONEMAP_FILMIC = 0, 1
 LT_PARAM_TEXTURE, LT_PARAM_GRID, LT_PARAM_TEXEL_POOL, LT_PARAM_VOXEL_POOL = 0, 1, 2, 3
 
 
@@ -125,6 +126,8 @@ SIGNATURES = {
     "lt_film_download": (C.c_int, [_vp, _pf, _pu32, _pu32]),
     "lt_film_upload": (C.c_int, [_vp, _pf, _pu32, _pu32]),
     "lt_film_resolve": (C.c_int, [_vp, _pf]),
+    "lt_film_tonemap": (C.c_int, [_vp, C.c_int32, _f, _pf]),
+    "lt_film_write_pfm": (C.c_int, [_vp, C.c_char_p]),
     "lt_render": (C.c_int, [_vp, C.POINTER(lt_camera), C.POINTER(lt_render_params), _vp,
                             C.POINTER(lt_render_stats)]),
     "lt_render_batch": (C.c_int, [C.POINTER(_vp), C.POINTER(lt_camera), C.POINTER(_vp), _u32,

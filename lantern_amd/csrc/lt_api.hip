@@ -789,6 +789,39 @@ int lt_film_resolve(lt_film *f, float *rgb) {
     return LT_OK;
 }
 
+int lt_film_tonemap(lt_film *f, int32_t op, float exposure, float *rgba) {
+    LT_CHECK(f && rgba, "lt_film_tonemap: NULL argument");
+    LT_CHECK(op == LT_TONEMAP_CLAMP || op == LT_TONEMAP_FILMIC, "lt_film_tonemap: unknown operator");
+    const size_t n = (size_t)f->w * f->h;
+    hipStream_t st = f->dev->stream;
+    DevBuf out;
+    int rc;
+    if ((rc = out.alloc(n * 16))) return rc;
+    if ((rc = launch_tonemap(st, f->sum.as<float>(), f->count.as<uint32_t>(), (uint32_t)n, op, exposure,
+                             out.as<float>(), f->dev->num_cus)))
+        return rc;
+    LT_HIP(hipMemcpyAsync(rgba, out.p, n * 16, hipMemcpyDeviceToHost, st));
+    LT_HIP(hipStreamSynchronize(st));
+    return LT_OK;
+}
+
+int lt_film_write_pfm(lt_film *f, const char *path) {
+    LT_CHECK(f && path, "lt_film_write_pfm: NULL argument");
+    const size_t n = (size_t)f->w * f->h;
+    std::vector<float> rgb(n * 3);
+    int rc = lt_film_resolve(f, rgb.data());
+    if (rc) return rc;
+    FILE *fp = std::fopen(path, "wb");
+    if (!fp) return set_error(LT_E_INVALID, std::string("lt_film_write_pfm: cannot open ") + path);
+    /* PFM: "PF", width height, negative scale = little endian, rows bottom to top */
+    std::fprintf(fp, "PF\n%u %u\n-1.0\n", f->w, f->h);
+    bool ok = true;
+    for (uint32_t y = f->h; y-- > 0 && ok;)
+        ok = std::fwrite(rgb.data() + (size_t)y * f->w * 3, sizeof(float), (size_t)f->w * 3, fp) == (size_t)f->w * 3;
+    ok = (std::fclose(fp) == 0) && ok;
+    return ok ? LT_OK : set_error(LT_E_INVALID, std::string("lt_film_write_pfm: write failed: ") + path);
+}
+
 /* ---- rendering -------------------------------------------------------------- */
 int lt_render(lt_scene *s, const lt_camera *cam, const lt_render_params *prm, lt_film *film, lt_render_stats *stats) {
     LT_CHECK(s && cam && prm && film, "lt_render: NULL argument");

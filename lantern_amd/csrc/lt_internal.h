@@ -91,6 +91,9 @@ int launch_transmittance(hipStream_t st, const DScene &sc, const DMedium &md, co
 /* After a device-side density update: clamp to >= 0 and recompute the
  * grid's majorant into *maxd (a field of the device grid table). */
 int launch_grid_refresh(hipStream_t st, float *voxels, size_t n, float *maxd, int num_cus);
+/* Film resolve + tone mapping (op 0 clamp, 1 filmic) into rgba (4 floats per pixel). */
+int launch_tonemap(hipStream_t st, const float *sum, const uint32_t *count, uint32_t n, int op, float exposure,
+                   float *rgba, int num_cus);
 size_t trace_spill_bytes(int grid, int block);
 int trace_default_block();
 
