@@ -202,8 +202,12 @@ class Film:
 
 
 def camera_struct(cam):
-    return _capi.lt_camera(cam["phi"], cam["theta"], cam["radius"], cam["width"], cam["height"],
-                           (C.c_float * 3)(*cam.get("target", (0.0, 0.0, 0.0))), cam["fov"], cam.get("filter", 0))
+    """lt_camera from a camera dict (phi, theta, radius, width, height, target,
+    fov, filter, up: the orbit's up sign, +1 unless a rotation flipped it)."""
+    c = _capi.lt_camera(cam["phi"], cam["theta"], cam["radius"], cam["width"], cam["height"],
+                        (C.c_float * 3)(*cam.get("target", (0.0, 0.0, 0.0))), cam["fov"], cam.get("filter", 0))
+    c.up = cam.get("up", 1.0)
+    return c
 
 
 def params_struct(p):

@@ -9,20 +9,41 @@
  *       FrameBuffer's ColorData (float3 per pixel), Bounces and
  *       ColorSampleCount (u32 per pixel). Exit 0, or 2 if the scene does not
  *       load (the loader's message is on stdout).
+ *   facade_main <scene.json> <frames> <out.bin> op...
+ *       first moves the camera as the visualizer's mouse handlers do
+ *       (PinholeCamera::Rotate/Zoom/Pan, pinhole_camera.cpp:42-86), one op
+ *       per argument: rotate,dphi,dtheta | zoom,d | pan,dx,dy; then prints
+ *       the camera as hex floats ("camera phi theta radius tx ty tz up").
  */
 #include "lantern/lantern.h"
 
 #include <atomic>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 
 int main(int argc, char **argv) {
-    if (argc != 4) {
+    if (argc < 4) {
         std::fprintf(stderr, "usage: %s <scene.json> <frames> <out.bin>\n", argv[0]);
         return 1;
     }
     Lantern::Scene scene;
     if (!scene.LoadSceneFromJSON(argv[1])) return 2;
+    for (int k = 4; k < argc; ++k) {
+        float a = 0.0f, b = 0.0f;
+        if (std::sscanf(argv[k], "rotate,%f,%f", &a, &b) == 2) scene.Camera->Rotate(a, b);
+        else if (std::sscanf(argv[k], "zoom,%f", &a) == 1) scene.Camera->Zoom(a);
+        else if (std::sscanf(argv[k], "pan,%f,%f", &a, &b) == 2) scene.Camera->Pan(a, b);
+        else {
+            std::fprintf(stderr, "unknown camera op %s\n", argv[k]);
+            return 1;
+        }
+    }
+    if (argc > 4) {
+        const lt_camera c = scene.Camera->Desc();
+        std::printf("camera %a %a %a %a %a %a %a\n", c.phi, c.theta, c.radius, c.target[0], c.target[1], c.target[2],
+                    c.up);
+    }
     const unsigned frames = (unsigned)std::atoi(argv[2]);
     const unsigned w = scene.Camera->FrameBufferWidth, h = scene.Camera->FrameBufferHeight;
     /* RenderFrame swaps the integrator's buffer with *swap before rendering

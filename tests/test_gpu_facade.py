@@ -36,3 +36,83 @@ def test_json_scene_through_the_facade_equals_lt_render(gpu_device, tmp_path):
     assert s.mean() > 0.01
     film.close()
     gs.close()
+
+
+def _orbit_after(cam, ops):
+    """PinholeCamera::Rotate/Zoom/Pan (pinhole_camera.cpp:42-86) restated in
+    float32 with the same double-precision 2*pi comparisons."""
+    f = np.float32
+    phi, theta, radius, up = f(cam["phi"]), f(cam["theta"]), f(cam["radius"]), f(1.0)
+    target = np.array(cam["target"], np.float32)
+
+    def basis():
+        o = target + radius * np.array([np.sin(phi) * np.sin(theta), np.cos(phi), np.sin(phi) * np.cos(theta)], f)
+        z = (o - target) / np.linalg.norm(o - target)
+        x = np.cross(np.array([0, up, 0], f), z)
+        x /= np.linalg.norm(x)
+        return x, np.cross(z, x), z
+
+    xa, ya, za = basis()
+    for op in ops:
+        k, *a = op.split(",")
+        a = [f(v) for v in a]
+        if k == "rotate":
+            theta = f(theta + a[1]) if up > 0 else f(theta - a[1])
+            phi = f(phi + a[0])
+            if float(phi) > 2.0 * float(f(np.pi)):
+                phi = f(float(phi) - 2.0 * float(f(np.pi)))
+            elif float(phi) < -2.0 * float(f(np.pi)):
+                phi = f(float(phi) + 2.0 * float(f(np.pi)))
+            up = f(1.0) if (0 < phi < np.pi) or (-np.pi > phi > -2.0 * np.pi) else f(-1.0)
+            xa, ya, za = basis()
+        elif k == "zoom":
+            radius = f(radius - a[0])
+        else:
+            target = target + xa * a[0] + ya * a[1]
+    return phi, theta, radius, target, up
+
+
+def test_camera_moves_through_the_facade(gpu_device, tmp_path):
+    # the visualizer's mouse handlers: rotate past phi = pi (the up vector
+    # flips), zoom in, pan; the moved camera renders like lt_render with it
+    driver = build_driver()
+    path = os.path.join(GOLDEN, "facade_scene.json")
+    ops = ["rotate,0.4,0.3", "rotate,1.8,-0.2", "zoom,2.5", "pan,0.7,-0.4"]
+    out = tmp_path / "fb.bin"
+    r = subprocess.run([driver, path, "2", str(out)] + ops, capture_output=True, text=True, timeout=200)
+    assert r.returncode == 0, r.stdout + r.stderr
+    line = [x for x in r.stdout.splitlines() if x.startswith("camera ")][0]
+    phi, theta, radius, tx, ty, tz, up = (float.fromhex(v) for v in line.split()[1:])
+    d = desc_from_json(path)
+    ephi, etheta, eradius, etarget, eup = _orbit_after(d.camera, ops)
+    assert up == eup == -1.0                              # phi went past pi
+    np.testing.assert_allclose([phi, theta, radius], [ephi, etheta, eradius], rtol=1e-6)
+    np.testing.assert_allclose([tx, ty, tz], etarget, atol=1e-5)
+    cam = dict(d.camera, phi=phi, theta=theta, radius=radius, target=(tx, ty, tz), up=up)
+    gs = lantern_amd.Scene(gpu_device, d)
+    film = lantern_amd.Film(gpu_device, cam["width"], cam["height"])
+    lantern_amd.render(gs, cam, dict(spp=2, max_depth=1500, rr_start=4, seed=0x4C414E54), film)
+    s_f, b_f, c_f = read_output(out)
+    s, b, c = film.download()
+    assert np.array_equal(s_f, s) and np.array_equal(b_f, b) and np.array_equal(c_f, c)
+    film.close()
+    gs.close()
+
+
+def test_multi_view_batch_equals_views_one_by_one(gpu_device):
+    # several cameras on one scene through the batch path (stream + view)
+    d = desc_from_json(os.path.join(GOLDEN, "facade_scene.json"))
+    gs = lantern_amd.Scene(gpu_device, d)
+    cams = [dict(d.camera, phi=d.camera["phi"] + 0.2 * k, theta=d.camera["theta"] - 0.3 * k) for k in range(4)]
+    params = dict(spp=3, max_depth=12, rr_start=4, seed=7, stream=2)
+    fa = [lantern_amd.Film(gpu_device, c["width"], c["height"]) for c in cams]
+    lantern_amd.render_batch([gs] * 4, cams, params, fa)
+    for k, c in enumerate(cams):
+        f = lantern_amd.Film(gpu_device, c["width"], c["height"])
+        lantern_amd.render(gs, c, dict(params, stream=2 + k), f)
+        for x, y in zip(f.download(), fa[k].download()):
+            assert np.array_equal(x, y)
+        f.close()
+    for f in fa:
+        f.close()
+    gs.close()
