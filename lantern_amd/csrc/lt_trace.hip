@@ -186,11 +186,12 @@ __global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, TraceBatch batch, co
 
 
 /* ---- 8-wide compressed traversal -------------------------------------------
- * One node fetch (80 B) tests eight quantized child boxes. Leaf children are
- * intersected as soon as their box is hit (shrinking tfar for the remaining
- * slots); the nearest internal child is descended into and the others are
- * pushed with their entry distance, so a pop can discard children that lie
- * beyond the closest hit found since. Stack entries (node, t) live in LDS,
+ * One node fetch (80 B) tests eight quantized child boxes. The triangles of
+ * the hit leaf children are gathered into one mask and intersected after the
+ * box tests, one triangle per lane per iteration; then the nearest internal
+ * child is descended into unless it lies beyond the shrunk tfar, and the
+ * others are pushed with their entry distance, so a pop can discard children
+ * that lie beyond the closest hit found since. Stack entries (node, t) live in LDS,
  * [depth][lane]; deeper entries spill to a per-thread slice of global memory.
  * A push beyond STACK8_CAP is counted in *overflow and the ray ends as a
  * miss, so an overflow is reported to the caller instead of going unnoticed. */
@@ -291,6 +292,9 @@ void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__
             const float tfar_w = r.tfar * 1.0000005f;
             int near_node = -1;
             float near_t = 0.0f;
+            /* triangles of the hit leaf children, bit k = triangle tri_base + k
+             * (a node holds at most 8 leaves of at most 4 triangles) */
+            uint32_t tmask = 0u;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const uint32_t meta = (mw[j >> 2] >> (8 * (j & 3))) & 0xFFu;
@@ -337,35 +341,40 @@ void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__
                         ++sp;
                     }
                 } else {
-                    const uint32_t first = tbase + (meta & 31u), cnt = meta >> 5;
-                    for (uint32_t k = 0; k < cnt; ++k) {
-                        const float4 *tp = wt + 3 * (size_t)(first + k);
-                        const float4 a = tp[0], b = tp[1], c = tp[2];
-                        float t, u, v;
-                        if ((!ANY || __float_as_uint(c.w) != ignore) &&
-                            moeller(r.o, r.d, r.tnear, r.tfar, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z),
-                                    t, u, v, (__float_as_uint(b.w) & 2u) != 0u)) {
-                            r.tfar = t;
-                            best_t = t;
-                            best_u = u;
-                            best_v = v;
-                            best_prim = __float_as_uint(a.w);
-                            if (ANY) {
-                                done = true;
-                                break;
-                            }
-                        }
-                    }
-                    if (done) break;
+                    tmask |= ((1u << (meta >> 5)) - 1u) << (meta & 31u);
                 }
             }
             if (done) break;
-            if (near_node >= 0) {
+            /* Leaf triangles after the box tests, one per lane per iteration:
+             * lanes that hit leaves in different slots test them together,
+             * rather than one slot at a time with the other lanes idle. */
+            while (tmask) {
+                const uint32_t k = (uint32_t)__builtin_ctz(tmask);
+                tmask &= tmask - 1u;
+                const float4 *tp = wt + 3 * (size_t)(tbase + k);
+                const float4 a = tp[0], b = tp[1], c = tp[2];
+                float t, u, v;
+                if ((!ANY || __float_as_uint(c.w) != ignore) &&
+                    moeller(r.o, r.d, r.tnear, r.tfar, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, u,
+                            v, (__float_as_uint(b.w) & 2u) != 0u)) {
+                    r.tfar = t;
+                    best_t = t;
+                    best_u = u;
+                    best_v = v;
+                    best_prim = __float_as_uint(a.w);
+                    if (ANY) {
+                        done = true;
+                        break;
+                    }
+                }
+            }
+            if (done) break;
+            const float limit = r.tfar * 1.0000005f;
+            if (near_node >= 0 && near_t <= limit) {
                 node = near_node;
                 continue;
             }
             bool found = false;
-            const float limit = r.tfar * 1.0000005f;
             while (sp > 0) {
                 --sp;
                 int nd;
