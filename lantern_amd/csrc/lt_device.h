@@ -122,12 +122,13 @@ LT_D float exp_f(float x) {
 }
 
 /* Embree's Moeller-Trumbore with Ng = cross(e2, e1)
- * (triangle_intersector_moeller.h:75-113, :115-124) and the finalize of
- * MoellerTrumboreHitM (:43-48) as a correctly rounded division. */
-/* flip: the triangle is the second half (v2, v3, v1) of an Embree quad, whose
+ * (triangle_intersector_moeller.h:75-113, :115-124). moeller_raw stops before
+ * the divisions and returns the scaled T, U, V and |den|; moeller adds the
+ * finalize of MoellerTrumboreHitM (:43-48) as correctly rounded divisions.
+ * flip: the triangle is the second half (v2, v3, v1) of an Embree quad, whose
  * hit reports U' = |den| - U, V' = |den| - V (quad_intersector_moeller.h:150-153). */
-LT_D bool moeller(const V3 &O, const V3 &D, float tnear, float tfar, const V3 &v0, const V3 &e1, const V3 &e2, float &t,
-                  float &u, float &v, bool flip = false) {
+LT_D bool moeller_raw(const V3 &O, const V3 &D, float tnear, float tfar, const V3 &v0, const V3 &e1, const V3 &e2,
+                      float &T_out, float &U_out, float &V_out, float &den_out, bool flip = false) {
     V3 Ng = cross(e2, e1);
     V3 C = v0 - O;
     V3 R = cross(C, D);
@@ -148,6 +149,17 @@ LT_D bool moeller(const V3 &O, const V3 &D, float tnear, float tfar, const V3 &v
         U = absDen - U;
         V = absDen - V;
     }
+    T_out = T;
+    U_out = U;
+    V_out = V;
+    den_out = absDen;
+    return true;
+}
+
+LT_D bool moeller(const V3 &O, const V3 &D, float tnear, float tfar, const V3 &v0, const V3 &e1, const V3 &e2, float &t,
+                  float &u, float &v, bool flip = false) {
+    float T, U, V, absDen;
+    if (!moeller_raw(O, D, tnear, tfar, v0, e1, e2, T, U, V, absDen, flip)) return false;
     t = T / absDen;
     u = U / absDen;
     v = V / absDen;

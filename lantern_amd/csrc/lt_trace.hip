@@ -232,7 +232,7 @@ void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__
         r.tmin = fmaxf(r.tnear, 0.0f);
         r.idir = v3(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
         const bool negx = r.idir.x < 0.0f, negy = r.idir.y < 0.0f, negz = r.idir.z < 0.0f;
-        float best_t = r.tfar, best_u = 0.0f, best_v = 0.0f;
+        float best_t = r.tfar;
         uint32_t best_prim = 0xFFFFFFFFu;
         const uint32_t ignore = (ANY && ignore_geom) ? ignore_geom[i] : 0xFFFFFFFFu;
         const float4 *__restrict__ wn = sc.wnodes;
@@ -245,12 +245,16 @@ void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__
             ntris = bs->ntris;
         }
         bool done = !(r.tfar >= 0.0f) || ntris == 0;
+        /* closest hit so far, kept as Moeller's scaled U, V and |den|: only t is
+         * needed during traversal, u and v are divided out once at the end */
+        float best_U = 0.0f, best_V = 0.0f, best_den = 1.0f;
 
         int sp = 0;
         int node = 0;
         uint32_t steps = 0;
         while (!done) {
-            if (++steps > MAX_STEPS) {
+            /* at most 7 pushes per node: stop before the stack could overflow */
+            if (++steps > MAX_STEPS || sp > STACK8_CAP - 7) {
                 best_prim = 0xFFFFFFFFu;
                 best_t = -1.0f;
                 atomicAdd(overflow, 1u);
@@ -290,8 +294,10 @@ void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__
             const uint32_t nqz[2] = {negz ? qh[2][0] : ql[2][0], negz ? qh[2][1] : ql[2][1]};
             const uint32_t fqz[2] = {negz ? ql[2][0] : qh[2][0], negz ? ql[2][1] : qh[2][1]};
             const float tfar_w = r.tfar * 1.0000005f;
+            /* nearest internal child hit so far; the loser of each comparison
+             * is pushed */
             int near_node = -1;
-            float near_t = 0.0f;
+            float near_t = __builtin_huge_valf();
             /* triangles of the hit leaf children, bit k = triangle tri_base + k
              * (a node holds at most 8 leaves of at most 4 triangles) */
             uint32_t tmask = 0u;
@@ -308,30 +314,18 @@ void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__
                 const float tfz = __builtin_fmaf(qbyte(fqz[j >> 2], j), az, bfz);
                 const float tn0 = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), r.tmin);
                 const float tf0 = fminf(fminf(fminf(tfx, tfy), tfz), tfar_w);
-                if (!(tn0 <= __builtin_fmaf(fabsf(tf0), kSlack, tf0))) continue;
+                const bool hit = tn0 <= __builtin_fmaf(fabsf(tf0), kSlack, tf0);
                 /* lower bound of the entry distance, for ordering and culling on pop */
                 const float tn = __builtin_fmaf(fabsf(tn0), -kSlack, tn0);
-                if (isint) {
+                tmask |= (hit && !isint) ? ((1u << (meta >> 5)) - 1u) << (meta & 31u) : 0u;
+                if (hit && isint) {
                     const int child = (int)(cbase + meta);
-                    if (near_node < 0) {
-                        near_node = child;
-                        near_t = tn;
-                    } else {
-                        int pn = child;
-                        float pt = tn;
-                        if (tn < near_t) {
-                            pn = near_node;
-                            pt = near_t;
-                            near_node = child;
-                            near_t = tn;
-                        }
-                        if (sp >= STACK8_CAP) {
-                            atomicAdd(overflow, 1u);
-                            best_prim = 0xFFFFFFFFu;
-                            best_t = -1.0f;
-                            done = true;
-                            break;
-                        }
+                    const bool closer = tn < near_t;
+                    const int pn = closer ? near_node : child;
+                    const float pt = closer ? near_t : tn;
+                    near_node = closer ? child : near_node;
+                    near_t = closer ? tn : near_t;
+                    if (pn >= 0) {
                         if (sp < LDS8) {
                             lds_node[sp * BLOCK + tid] = pn;
                             lds_t[sp * BLOCK + tid] = pt;
@@ -340,11 +334,8 @@ void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__
                         }
                         ++sp;
                     }
-                } else {
-                    tmask |= ((1u << (meta >> 5)) - 1u) << (meta & 31u);
                 }
             }
-            if (done) break;
             /* Leaf triangles after the box tests, one per lane per iteration:
              * lanes that hit leaves in different slots test them together,
              * rather than one slot at a time with the other lanes idle. */
@@ -353,19 +344,20 @@ void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__
                 tmask &= tmask - 1u;
                 const float4 *tp = wt + 3 * (size_t)(tbase + k);
                 const float4 a = tp[0], b = tp[1], c = tp[2];
-                float t, u, v;
+                float T, U, V, den;
                 if ((!ANY || __float_as_uint(c.w) != ignore) &&
-                    moeller(r.o, r.d, r.tnear, r.tfar, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, u,
-                            v, (__float_as_uint(b.w) & 2u) != 0u)) {
-                    r.tfar = t;
-                    best_t = t;
-                    best_u = u;
-                    best_v = v;
+                    moeller_raw(r.o, r.d, r.tnear, r.tfar, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), T,
+                                U, V, den, (__float_as_uint(b.w) & 2u) != 0u)) {
                     best_prim = __float_as_uint(a.w);
                     if (ANY) {
                         done = true;
                         break;
                     }
+                    r.tfar = T / den;
+                    best_t = r.tfar;
+                    best_U = U;
+                    best_V = V;
+                    best_den = den;
                 }
             }
             if (done) break;
@@ -398,6 +390,11 @@ void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__
         if (ANY) {
             occluded[i] = best_prim != 0xFFFFFFFFu ? 1u : 0u;
         } else {
+            float best_u = 0.0f, best_v = 0.0f;
+            if (best_prim != 0xFFFFFFFFu) {
+                best_u = best_U / best_den;
+                best_v = best_V / best_den;
+            }
             *(float4 *)(hits + i) = make_float4(best_t, best_u, best_v, __uint_as_float(best_prim));
         }
     }
