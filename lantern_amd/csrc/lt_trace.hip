@@ -200,8 +200,214 @@ constexpr float kSlack = 0x1p-20f;
 
 LT_D float qbyte(uint32_t w, int j) { return (float)((w >> (8 * (j & 3))) & 0xFFu); }
 
+/* State of one ray in the 8-wide traversal. The closest hit so far is kept as
+ * Moeller's scaled U, V and |den|: only t is needed during traversal, u and v
+ * are divided out once at the end. */
+struct Trav8 {
+    TravRay r;
+    bool negx, negy, negz;
+    float best_t, best_U, best_V, best_den;
+    uint32_t best_prim, ignore, steps, ray;
+    const float4 *__restrict__ wn;
+    const float4 *__restrict__ wt;
+    int sp, node;
+};
+
+/* Loads ray i and sets up its traversal; returns false if the ray needs no
+ * traversal (negative tfar or an empty scene). */
+template <bool ANY, bool BATCH>
+LT_D bool trav8_begin(Trav8 &s, const DScene &sc, const TraceBatch &batch, const Ray *__restrict__ rays,
+                      const uint32_t *ignore_geom, uint32_t i) {
+    const float4 *rp = (const float4 *)(rays + i);
+    const float4 ra = rp[0], rb = rp[1];
+    s.ray = i;
+    s.r.o = v3(ra.x, ra.y, ra.z);
+    s.r.tnear = ra.w;
+    s.r.d = v3(rb.x, rb.y, rb.z);
+    s.r.tfar = rb.w;
+    s.r.tmin = fmaxf(s.r.tnear, 0.0f);
+    s.r.idir = v3(safe_rcp(s.r.d.x), safe_rcp(s.r.d.y), safe_rcp(s.r.d.z));
+    s.negx = s.r.idir.x < 0.0f;
+    s.negy = s.r.idir.y < 0.0f;
+    s.negz = s.r.idir.z < 0.0f;
+    s.best_t = s.r.tfar;
+    s.best_U = 0.0f;
+    s.best_V = 0.0f;
+    s.best_den = 1.0f;
+    s.best_prim = 0xFFFFFFFFu;
+    s.ignore = (ANY && ignore_geom) ? ignore_geom[i] : 0xFFFFFFFFu;
+    s.wn = sc.wnodes;
+    s.wt = sc.wtris;
+    uint32_t ntris = sc.ntris;
+    if (BATCH) {
+        const DScene *bs = batch.scenes + batch.ray_scene[i];
+        s.wn = bs->wnodes;
+        s.wt = bs->wtris;
+        ntris = bs->ntris;
+    }
+    s.sp = 0;
+    s.node = 0;
+    s.steps = 0;
+    return (s.r.tfar >= 0.0f) && ntris != 0;
+}
+
+template <bool ANY>
+LT_D void trav8_end(const Trav8 &s, Hit *__restrict__ hits, uint32_t *__restrict__ occluded) {
+    if (ANY) {
+        occluded[s.ray] = s.best_prim != 0xFFFFFFFFu ? 1u : 0u;
+    } else {
+        float u = 0.0f, v = 0.0f;
+        if (s.best_prim != 0xFFFFFFFFu) {
+            u = s.best_U / s.best_den;
+            v = s.best_V / s.best_den;
+        }
+        *(float4 *)(hits + s.ray) = make_float4(s.best_t, u, v, __uint_as_float(s.best_prim));
+    }
+}
+
+/* One node visit: the eight child boxes, then the triangles of the hit leaf
+ * children, then the next node (the nearest hit internal child, or a pop).
+ * Returns true when the ray is finished. */
+template <bool ANY, int BLOCK, int LDS8>
+LT_D bool trav8_visit(Trav8 &s, int *lds_node, float *lds_t, int tid, int2 *gstack, uint32_t *overflow) {
+    /* at most 7 pushes per node: stop before the stack could overflow */
+    if (++s.steps > MAX_STEPS || s.sp > STACK8_CAP - 7) {
+        s.best_prim = 0xFFFFFFFFu;
+        s.best_t = -1.0f;
+        atomicAdd(overflow, 1u);
+        return true;
+    }
+    const TravRay &r = s.r;
+    const float4 *np = s.wn + 5 * (size_t)s.node;
+    const float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3], n4 = np[4];
+    const uint32_t eb = __float_as_uint(n0.w);
+    const float sx = __uint_as_float((eb & 0xFFu) << 23);
+    const float sy = __uint_as_float(((eb >> 8) & 0xFFu) << 23);
+    const float sz = __uint_as_float(((eb >> 16) & 0xFFu) << 23);
+    const uint32_t imask = eb >> 24;
+    const uint32_t cbase = __float_as_uint(n1.x), tbase = __float_as_uint(n1.y);
+    const uint32_t mw[2] = {__float_as_uint(n1.z), __float_as_uint(n1.w)};
+    /* Plane distances as t = q * (s * idir) + (p - o) * idir: one FMA per
+     * plane. s is a power of two, so s * idir is exact; the rounding of
+     * b = (p - o) * idir errs by at most 2u |b| (u = 2^-24) on that axis,
+     * so near planes use b - c and far planes b + c with c = 2^-20 |b|,
+     * and the final FMA's relative error is covered by 2^-20 |t| in the
+     * test below: a box the ray enters is never rejected. The near plane
+     * of each axis is the low one for a positive direction. */
+    const float ax = sx * r.idir.x, ay = sy * r.idir.y, az = sz * r.idir.z;
+    const float bx = (n0.x - r.o.x) * r.idir.x, by = (n0.y - r.o.y) * r.idir.y, bz = (n0.z - r.o.z) * r.idir.z;
+    const float cx = fabsf(bx) * kSlack, cy = fabsf(by) * kSlack, cz = fabsf(bz) * kSlack;
+    const float bnx = bx - cx, bfx = bx + cx, bny = by - cy, bfy = by + cy, bnz = bz - cz, bfz = bz + cz;
+    const uint32_t ql[3][2] = {{__float_as_uint(n2.x), __float_as_uint(n2.y)},
+                               {__float_as_uint(n2.z), __float_as_uint(n2.w)},
+                               {__float_as_uint(n3.x), __float_as_uint(n3.y)}};
+    const uint32_t qh[3][2] = {{__float_as_uint(n3.z), __float_as_uint(n3.w)},
+                               {__float_as_uint(n4.x), __float_as_uint(n4.y)},
+                               {__float_as_uint(n4.z), __float_as_uint(n4.w)}};
+    const uint32_t nqx[2] = {s.negx ? qh[0][0] : ql[0][0], s.negx ? qh[0][1] : ql[0][1]};
+    const uint32_t fqx[2] = {s.negx ? ql[0][0] : qh[0][0], s.negx ? ql[0][1] : qh[0][1]};
+    const uint32_t nqy[2] = {s.negy ? qh[1][0] : ql[1][0], s.negy ? qh[1][1] : ql[1][1]};
+    const uint32_t fqy[2] = {s.negy ? ql[1][0] : qh[1][0], s.negy ? ql[1][1] : qh[1][1]};
+    const uint32_t nqz[2] = {s.negz ? qh[2][0] : ql[2][0], s.negz ? qh[2][1] : ql[2][1]};
+    const uint32_t fqz[2] = {s.negz ? ql[2][0] : qh[2][0], s.negz ? ql[2][1] : qh[2][1]};
+    const float tfar_w = r.tfar * 1.0000005f;
+    /* nearest internal child hit so far; the loser of each comparison is pushed */
+    int near_node = -1;
+    float near_t = __builtin_huge_valf();
+    /* triangles of the hit leaf children, bit k = triangle tri_base + k
+     * (a node holds at most 8 leaves of at most 4 triangles) */
+    uint32_t tmask = 0u;
+    int sp = s.sp;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const uint32_t meta = (mw[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+        const bool isint = (imask >> j) & 1u;
+        /* unused slots hold inverted boxes and meta 0 (a leaf of no triangles) */
+        const float tnx = __builtin_fmaf(qbyte(nqx[j >> 2], j), ax, bnx);
+        const float tfx = __builtin_fmaf(qbyte(fqx[j >> 2], j), ax, bfx);
+        const float tny = __builtin_fmaf(qbyte(nqy[j >> 2], j), ay, bny);
+        const float tfy = __builtin_fmaf(qbyte(fqy[j >> 2], j), ay, bfy);
+        const float tnz = __builtin_fmaf(qbyte(nqz[j >> 2], j), az, bnz);
+        const float tfz = __builtin_fmaf(qbyte(fqz[j >> 2], j), az, bfz);
+        const float tn0 = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), r.tmin);
+        const float tf0 = fminf(fminf(fminf(tfx, tfy), tfz), tfar_w);
+        const bool hit = tn0 <= __builtin_fmaf(fabsf(tf0), kSlack, tf0);
+        /* lower bound of the entry distance, for ordering and culling on pop */
+        const float tn = __builtin_fmaf(fabsf(tn0), -kSlack, tn0);
+        tmask |= (hit && !isint) ? ((1u << (meta >> 5)) - 1u) << (meta & 31u) : 0u;
+        if (hit && isint) {
+            const int child = (int)(cbase + meta);
+            const bool closer = tn < near_t;
+            const int pn = closer ? near_node : child;
+            const float pt = closer ? near_t : tn;
+            near_node = closer ? child : near_node;
+            near_t = closer ? tn : near_t;
+            if (pn >= 0) {
+                if (sp < LDS8) {
+                    lds_node[sp * BLOCK + tid] = pn;
+                    lds_t[sp * BLOCK + tid] = pt;
+                } else {
+                    gstack[sp - LDS8] = make_int2(pn, __float_as_int(pt));
+                }
+                ++sp;
+            }
+        }
+    }
+    /* Leaf triangles after the box tests, one per lane per iteration: lanes
+     * that hit leaves in different slots test them together, rather than one
+     * slot at a time with the other lanes idle. */
+    while (tmask) {
+        const uint32_t k = (uint32_t)__builtin_ctz(tmask);
+        tmask &= tmask - 1u;
+        const float4 *tp = s.wt + 3 * (size_t)(tbase + k);
+        const float4 a = tp[0], b = tp[1], c = tp[2];
+        float T, U, V, den;
+        if ((!ANY || __float_as_uint(c.w) != s.ignore) &&
+            moeller_raw(r.o, r.d, r.tnear, r.tfar, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), T, U, V,
+                        den, (__float_as_uint(b.w) & 2u) != 0u)) {
+            s.best_prim = __float_as_uint(a.w);
+            if (ANY) return true;
+            s.r.tfar = T / den;
+            s.best_t = s.r.tfar;
+            s.best_U = U;
+            s.best_V = V;
+            s.best_den = den;
+        }
+    }
+    const float limit = s.r.tfar * 1.0000005f;
+    if (near_node >= 0 && near_t <= limit) {
+        s.node = near_node;
+        s.sp = sp;
+        return false;
+    }
+    while (sp > 0) {
+        --sp;
+        int nd;
+        float t;
+        if (sp < LDS8) {
+            nd = lds_node[sp * BLOCK + tid];
+            t = lds_t[sp * BLOCK + tid];
+        } else {
+            const int2 e = gstack[sp - LDS8];
+            nd = e.x;
+            t = __int_as_float(e.y);
+        }
+        if (t <= limit) {
+            s.node = nd;
+            s.sp = sp;
+            return false;
+        }
+    }
+    s.sp = 0;
+    return true;
+}
+
 /* LDS8: stack entries kept in LDS per lane; WAVES: occupancy the register
- * allocator targets (0: the compiler's choice). */
+ * allocator targets (0: the compiler's choice). A wave takes 64 new rays when
+ * all its rays are finished. (Refilling single lanes as soon as enough of
+ * them are idle, Aila & Laine 2009, measured 1.5-2x slower here: the lanes of
+ * a wave then sit in unrelated parts of the tree, and the node fetches that
+ * the wave's lanes used to share become separate cache lines.) */
 template <bool ANY, int BLOCK, int LDS8, int WAVES, bool BATCH>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVES : 1)))
 void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__restrict__ hits,
@@ -213,7 +419,7 @@ void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     int2 *gstack = spill + (size_t)(blockIdx.x * BLOCK + tid) * STACK8_CAP;
-
+    Trav8 s;
     while (true) {
         uint32_t base = 0;
         if (lane == 0) base = atomicAdd(work, 64u);
@@ -221,182 +427,10 @@ void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__
         if (base >= n) break;
         const uint32_t i = base + lane;
         if (i >= n) continue;
-
-        const float4 *rp = (const float4 *)(rays + i);
-        float4 ra = rp[0], rb = rp[1];
-        TravRay r;
-        r.o = v3(ra.x, ra.y, ra.z);
-        r.tnear = ra.w;
-        r.d = v3(rb.x, rb.y, rb.z);
-        r.tfar = rb.w;
-        r.tmin = fmaxf(r.tnear, 0.0f);
-        r.idir = v3(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
-        const bool negx = r.idir.x < 0.0f, negy = r.idir.y < 0.0f, negz = r.idir.z < 0.0f;
-        float best_t = r.tfar;
-        uint32_t best_prim = 0xFFFFFFFFu;
-        const uint32_t ignore = (ANY && ignore_geom) ? ignore_geom[i] : 0xFFFFFFFFu;
-        const float4 *__restrict__ wn = sc.wnodes;
-        const float4 *__restrict__ wt = sc.wtris;
-        uint32_t ntris = sc.ntris;
-        if (BATCH) {
-            const DScene *bs = batch.scenes + batch.ray_scene[i];
-            wn = bs->wnodes;
-            wt = bs->wtris;
-            ntris = bs->ntris;
-        }
-        bool done = !(r.tfar >= 0.0f) || ntris == 0;
-        /* closest hit so far, kept as Moeller's scaled U, V and |den|: only t is
-         * needed during traversal, u and v are divided out once at the end */
-        float best_U = 0.0f, best_V = 0.0f, best_den = 1.0f;
-
-        int sp = 0;
-        int node = 0;
-        uint32_t steps = 0;
-        while (!done) {
-            /* at most 7 pushes per node: stop before the stack could overflow */
-            if (++steps > MAX_STEPS || sp > STACK8_CAP - 7) {
-                best_prim = 0xFFFFFFFFu;
-                best_t = -1.0f;
-                atomicAdd(overflow, 1u);
-                break;
+        if (trav8_begin<ANY, BATCH>(s, sc, batch, rays, ignore_geom, i))
+            while (!trav8_visit<ANY, BLOCK, LDS8>(s, lds_node, lds_t, tid, gstack, overflow)) {
             }
-            const float4 *np = wn + 5 * (size_t)node;
-            const float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3], n4 = np[4];
-            const uint32_t eb = __float_as_uint(n0.w);
-            const float sx = __uint_as_float((eb & 0xFFu) << 23);
-            const float sy = __uint_as_float(((eb >> 8) & 0xFFu) << 23);
-            const float sz = __uint_as_float(((eb >> 16) & 0xFFu) << 23);
-            const uint32_t imask = eb >> 24;
-            const uint32_t cbase = __float_as_uint(n1.x), tbase = __float_as_uint(n1.y);
-            const uint32_t mw[2] = {__float_as_uint(n1.z), __float_as_uint(n1.w)};
-            /* Plane distances as t = q * (s * idir) + (p - o) * idir: one FMA per
-             * plane. s is a power of two, so s * idir is exact; the rounding of
-             * b = (p - o) * idir errs by at most 2u |b| (u = 2^-24) on that axis,
-             * so near planes use b - c and far planes b + c with c = 2^-20 |b|,
-             * and the final FMA's relative error is covered by 2^-20 |t| in the
-             * test below: a box the ray enters is never rejected. The near plane
-             * of each axis is the low one for a positive direction. */
-            const float ax = sx * r.idir.x, ay = sy * r.idir.y, az = sz * r.idir.z;
-            const float bx = (n0.x - r.o.x) * r.idir.x, by = (n0.y - r.o.y) * r.idir.y,
-                        bz = (n0.z - r.o.z) * r.idir.z;
-            const float cx = fabsf(bx) * kSlack, cy = fabsf(by) * kSlack, cz = fabsf(bz) * kSlack;
-            const float bnx = bx - cx, bfx = bx + cx, bny = by - cy, bfy = by + cy, bnz = bz - cz, bfz = bz + cz;
-            const uint32_t ql[3][2] = {{__float_as_uint(n2.x), __float_as_uint(n2.y)},
-                                       {__float_as_uint(n2.z), __float_as_uint(n2.w)},
-                                       {__float_as_uint(n3.x), __float_as_uint(n3.y)}};
-            const uint32_t qh[3][2] = {{__float_as_uint(n3.z), __float_as_uint(n3.w)},
-                                       {__float_as_uint(n4.x), __float_as_uint(n4.y)},
-                                       {__float_as_uint(n4.z), __float_as_uint(n4.w)}};
-            const uint32_t nqx[2] = {negx ? qh[0][0] : ql[0][0], negx ? qh[0][1] : ql[0][1]};
-            const uint32_t fqx[2] = {negx ? ql[0][0] : qh[0][0], negx ? ql[0][1] : qh[0][1]};
-            const uint32_t nqy[2] = {negy ? qh[1][0] : ql[1][0], negy ? qh[1][1] : ql[1][1]};
-            const uint32_t fqy[2] = {negy ? ql[1][0] : qh[1][0], negy ? ql[1][1] : qh[1][1]};
-            const uint32_t nqz[2] = {negz ? qh[2][0] : ql[2][0], negz ? qh[2][1] : ql[2][1]};
-            const uint32_t fqz[2] = {negz ? ql[2][0] : qh[2][0], negz ? ql[2][1] : qh[2][1]};
-            const float tfar_w = r.tfar * 1.0000005f;
-            /* nearest internal child hit so far; the loser of each comparison
-             * is pushed */
-            int near_node = -1;
-            float near_t = __builtin_huge_valf();
-            /* triangles of the hit leaf children, bit k = triangle tri_base + k
-             * (a node holds at most 8 leaves of at most 4 triangles) */
-            uint32_t tmask = 0u;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const uint32_t meta = (mw[j >> 2] >> (8 * (j & 3))) & 0xFFu;
-                const bool isint = (imask >> j) & 1u;
-                /* unused slots hold inverted boxes and meta 0 (a leaf of no triangles) */
-                const float tnx = __builtin_fmaf(qbyte(nqx[j >> 2], j), ax, bnx);
-                const float tfx = __builtin_fmaf(qbyte(fqx[j >> 2], j), ax, bfx);
-                const float tny = __builtin_fmaf(qbyte(nqy[j >> 2], j), ay, bny);
-                const float tfy = __builtin_fmaf(qbyte(fqy[j >> 2], j), ay, bfy);
-                const float tnz = __builtin_fmaf(qbyte(nqz[j >> 2], j), az, bnz);
-                const float tfz = __builtin_fmaf(qbyte(fqz[j >> 2], j), az, bfz);
-                const float tn0 = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), r.tmin);
-                const float tf0 = fminf(fminf(fminf(tfx, tfy), tfz), tfar_w);
-                const bool hit = tn0 <= __builtin_fmaf(fabsf(tf0), kSlack, tf0);
-                /* lower bound of the entry distance, for ordering and culling on pop */
-                const float tn = __builtin_fmaf(fabsf(tn0), -kSlack, tn0);
-                tmask |= (hit && !isint) ? ((1u << (meta >> 5)) - 1u) << (meta & 31u) : 0u;
-                if (hit && isint) {
-                    const int child = (int)(cbase + meta);
-                    const bool closer = tn < near_t;
-                    const int pn = closer ? near_node : child;
-                    const float pt = closer ? near_t : tn;
-                    near_node = closer ? child : near_node;
-                    near_t = closer ? tn : near_t;
-                    if (pn >= 0) {
-                        if (sp < LDS8) {
-                            lds_node[sp * BLOCK + tid] = pn;
-                            lds_t[sp * BLOCK + tid] = pt;
-                        } else {
-                            gstack[sp - LDS8] = make_int2(pn, __float_as_int(pt));
-                        }
-                        ++sp;
-                    }
-                }
-            }
-            /* Leaf triangles after the box tests, one per lane per iteration:
-             * lanes that hit leaves in different slots test them together,
-             * rather than one slot at a time with the other lanes idle. */
-            while (tmask) {
-                const uint32_t k = (uint32_t)__builtin_ctz(tmask);
-                tmask &= tmask - 1u;
-                const float4 *tp = wt + 3 * (size_t)(tbase + k);
-                const float4 a = tp[0], b = tp[1], c = tp[2];
-                float T, U, V, den;
-                if ((!ANY || __float_as_uint(c.w) != ignore) &&
-                    moeller_raw(r.o, r.d, r.tnear, r.tfar, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), T,
-                                U, V, den, (__float_as_uint(b.w) & 2u) != 0u)) {
-                    best_prim = __float_as_uint(a.w);
-                    if (ANY) {
-                        done = true;
-                        break;
-                    }
-                    r.tfar = T / den;
-                    best_t = r.tfar;
-                    best_U = U;
-                    best_V = V;
-                    best_den = den;
-                }
-            }
-            if (done) break;
-            const float limit = r.tfar * 1.0000005f;
-            if (near_node >= 0 && near_t <= limit) {
-                node = near_node;
-                continue;
-            }
-            bool found = false;
-            while (sp > 0) {
-                --sp;
-                int nd;
-                float t;
-                if (sp < LDS8) {
-                    nd = lds_node[sp * BLOCK + tid];
-                    t = lds_t[sp * BLOCK + tid];
-                } else {
-                    int2 e = gstack[sp - LDS8];
-                    nd = e.x;
-                    t = __int_as_float(e.y);
-                }
-                if (t <= limit) {
-                    node = nd;
-                    found = true;
-                    break;
-                }
-            }
-            if (!found) break;
-        }
-        if (ANY) {
-            occluded[i] = best_prim != 0xFFFFFFFFu ? 1u : 0u;
-        } else {
-            float best_u = 0.0f, best_v = 0.0f;
-            if (best_prim != 0xFFFFFFFFu) {
-                best_u = best_U / best_den;
-                best_v = best_V / best_den;
-            }
-            *(float4 *)(hits + i) = make_float4(best_t, best_u, best_v, __uint_as_float(best_prim));
-        }
+        trav8_end<ANY>(s, hits, occluded);
     }
 }
 

@@ -116,3 +116,23 @@ def test_quad_scene_matches_oracle(gpu_device):
     d = scenes.quad_scene(64, 64, 4)
     g, o, stats, rays = _render_both(gpu_device, d)
     _check(g, o)
+
+
+def test_trace_variants_give_identical_images(gpu_device):
+    """Every traversal kernel variant (LDS stack depth, occupancy target) and
+    block size visits each ray's nodes in the same order, so the films are
+    bit-identical to the default launch's."""
+    d = scenes.triangle_heavy(128, 72, 2)
+    gs = lantern_amd.Scene(gpu_device, d)
+    film = lantern_amd.Film(gpu_device, 128, 72)
+    lantern_amd.render(gs, d.camera, dict(d.params), film)
+    ref = film.download()
+    for variant in range(1, 5):
+        for block in (64, 128, 256):
+            film.reset()
+            lantern_amd.render(gs, d.camera, dict(d.params, trace_variant=variant, block_size=block), film)
+            got = film.download()
+            for a, b in zip(got, ref):
+                assert np.array_equal(a, b), f"variant {variant} block {block}"
+    film.close()
+    gs.close()
