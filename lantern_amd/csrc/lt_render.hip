@@ -275,269 +275,353 @@ __device__ __forceinline__ V3 adj_weight(const WaveArgs &a, uint32_t pix, const 
     return v3(g[0] * C.x, g[1] * C.y, g[2] * C.z) * a.adj.inv_spp;
 }
 
+/* Shading bin of slot q: 0 a miss, 1 a path inside a medium, 2 + the BSDF
+ * type of the surface hit. */
+constexpr int kBins = 6;
+template <bool BATCH>
+__device__ __forceinline__ uint32_t shade_bin(const WaveArgs &a, const Queue &cur, const Hit *hits, uint32_t q) {
+    const uint32_t prim = hits[q].prim;
+    if (prim == 0xFFFFFFFFu) return 0u;
+    if (a.ps.medium[cur.path[q]] >= 0) return 1u;
+    const DScene &sc = BATCH ? a.batch.scenes[cur.sid[q]] : a.sc;
+    const int bsdf = sc.mats[sc.geoms[sc.tri_geom[prim]].material].bsdf;
+    return 2u + (uint32_t)min(max(bsdf, 0), kBins - 3);
+}
+
+/* k_shade works through the live slots in chunks of R * 64 consecutive
+ * slots per wave. A wave first sorts its chunk by shading bin (a counting
+ * sort with ballots, into its own part of LDS), then shades the sorted chunk
+ * 64 slots at a time, so that its lanes take the same branch (a miss, a
+ * medium, one BSDF) instead of every branch with part of the lanes. The
+ * slots of a chunk stay within a few KB of each per-slot array, so the
+ * reordering costs no extra memory traffic, and no wave waits for another.
+ * R (at most kShadeRounds) shrinks with the live count so that small
+ * bounces still spread over every wave of the grid. The result does not
+ * depend on the order: every path draws from its own random stream. */
+constexpr int kShadeBlock = 256;
+constexpr int kShadeRounds = 8;
+constexpr int kShadeChunk = kShadeRounds * 64;
+
 /* One bounce of RenderPixel for every live path (integrator.cpp:119-242).
  * ADJ: the replay of the adjoint pass; contributions are subtracted from the
  * radiance still to come instead of added, and parameter factors scatter
  * their gradients. */
 template <bool ADJ, bool BATCH>
-__global__ __launch_bounds__(256) void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next,
-                                               NeeRec nee, Ray *nee_rays, Hit *nee_hits, uint32_t *nee_ignore,
-                                               uint32_t *nee_cidx, Ctl *ctl) {
+__global__ __launch_bounds__(kShadeBlock) void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits,
+                                                       Queue next, NeeRec nee, Ray *nee_rays, Hit *nee_hits,
+                                                       uint32_t *nee_ignore, uint32_t *nee_cidx, Ctl *ctl) {
+    __shared__ uint32_t s_order_all[(kShadeBlock / 64) * kShadeChunk];
     const uint32_t count = ctl->count[b];
     const int lane = threadIdx.x & 63;
+    const float inf = __builtin_huge_valf();
+    const unsigned long long lanes_below = (1ull << lane) - 1ull;
+    uint32_t *s_order = s_order_all + (threadIdx.x >> 6) * kShadeChunk;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    const float inf = __builtin_huge_valf();
-    for (uint32_t base = wave * 64; base < count; base += nwaves * 64) {
-        const uint32_t q = base + lane;
-        bool alive = false;
-        bool nee1 = false, nee2 = false;
-        float tfar1 = 0.0f, tfar2 = 0.0f;
-        uint32_t p = 0;
-        V3 new_o = v3s(0.0f), new_d = v3s(0.0f);
-        V3 P = v3s(0.0f), wi1 = v3s(0.0f), wi2 = v3s(0.0f);
-        uint32_t ignore_geom = 0xFFFFFFFFu;
-        uint32_t scene = 0;
-        if (q < count) {
-            p = cur.path[q];
-            const PathInfo pi = path_info<BATCH>(a, p, BATCH ? a.batch.path_seg[p] : 0u);
-            const uint32_t x = pi.x, y = pi.y;
-            scene = pi.scene;
-            const DScene &sc = BATCH ? a.batch.scenes[scene] : a.sc;
-            const uint32_t pix = y * a.cam.w + x;
-            PathRng rng;
-            rng.init(x, y, a.stream + scene, a.seed, pi.frame, a.ps.rngi[p]);
-            const float4 *rp = (const float4 *)(cur.rays + q);
-            float4 ra = rp[0], rb = rp[1];
-            const Hit h = hits[q];
-            V3 T = v3(a.ps.tx[p], a.ps.ty[p], a.ps.tz[p]);
-            V3 C = v3(a.ps.cx[p], a.ps.cy[p], a.ps.cz[p]);
-            Interaction it;
-            it.iori = a.ps.iori[p];
-            it.lobe = (int)a.ps.lobe[p];
-            int medium = a.ps.medium[p];
-            bool dead = false;
-            uint32_t nee_info = 0;
-            if (h.prim == 0xFFFFFFFFu) {
-                if (ADJ) C = C - T * v3(sc.bg[0], sc.bg[1], sc.bg[2]);
-                else C = C + T * v3(sc.bg[0], sc.bg[1], sc.bg[2]);
-                dead = true;
-            } else {
-                V3 origin = v3(ra.x, ra.y, ra.z);
-                V3 dir = normalize(v3(rb.x, rb.y, rb.z));
-                bool surface = true;
-                if (medium >= 0) {                                                   /* integrator.cpp:131-163 */
-                    const DMedium md = sc.media[medium];
-                    if (md.type == MEDIUM_GRID) {
-                        const DGrid g = sc.grids[md.grid];
-                        float sbar = md.scale * g.maxd;
-                        float t = 0.0f, t1 = 0.0f;
-                        bool collided = false;
-                        if (sbar > 0.0f && grid_span(g, origin, dir, 0.0f, h.t, t, t1)) {
-                            while (true) {                                         /* delta tracking */
-                                t = t - log_f(1.0f - rng.next_float()) / sbar;
-                                if (!(t < t1)) break;
-                                V3 qp = origin + dir * t;
-                                const long vox = grid_voxel(g, qp.x, qp.y, qp.z);
-                                const float dens = vox < 0 ? 0.0f : sc.voxels[vox];
-                                const bool real = rng.next_float() * sbar < md.scale * dens;
-                                if (ADJ && vox >= 0) {
-                                    /* d log P / d density: 1/density (collision),
-                                     * -scale/(sbar - scale*density) (null) */
-                                    const V3 wgt = adj_weight(a, pix, C);
-                                    const float dl = wgt.x + wgt.y + wgt.z;
-                                    const float dlogp = real ? 1.0f / dens : -md.scale / (sbar - md.scale * dens);
-                                    unsafeAtomicAdd(a.adj.grad_voxels + vox, dl * dlogp);
-                                }
-                                if (real) {
-                                    collided = true;
-                                    break;
-                                }
-                            }
-                        }
-                        if (collided) {
-                            surface = false;
-                            T = T * md.albedo;
-                            new_o = origin + dir * t;
-                            new_d = sample_hg(md.g, dir, rng);
-                        }
-                    } else {
-                        float dist = h.t;
-                        if (md.type == MEDIUM_ISOTROPIC) {                           /* isotropic_scattering_medium.h:31-40 */
-                            float d = -log_f(rng.next_float()) / md.sigma_s;
-                            dist = d >= h.t ? h.t : d;
-                        }
-                        V3 na = -v3(md.sigma_a[0], md.sigma_a[1], md.sigma_a[2]);
-                        V3 tr = v3(exp_f(na.x * dist), exp_f(na.y * dist), exp_f(na.z * dist));
-                        T = (T * 1.0f) * tr;
-                        if (dist < h.t) {
-                            surface = false;
-                            new_o = origin + dir * dist;
-                            new_d = uniform_sphere(rng);
-                        }
-                    }
-                }
-                if (surface) {
-                    const uint32_t gi = sc.tri_geom[h.prim];
-                    const DGeom g = sc.geoms[gi];
-                    const DMaterial m = sc.mats[g.material];
-                    const int light = g.light;
-                    if ((b == 0 || (it.lobe & LOBE_SPECULAR) != 0) && light >= 0) {
-                        const DLight L = sc.lights[light];
-                        if (ADJ) C = C - T * v3(L.le[0], L.le[1], L.le[2]);
-                        else C = C + T * v3(L.le[0], L.le[1], L.le[2]);
-                    }
-                    it.P = origin + dir * h.t;
-                    if (g.flags & 1u) it.N = normalize(interp_normal(sc, h.prim, h.u, h.v, g));
-                    else it.N = normalize(v3s(0.0f));
-                    if (g.flags & 2u) interp_uv(sc, h.prim, h.u, h.v, g, it.tu, it.tv);
-                    else it.tu = it.tv = 0.0f;
-                    it.wo = -dir;
-                    it.ioro = 0.0f;
-                    P = it.P;
-                    if (ADJ && m.texture >= 0 && m.bsdf != BSDF_GGX) {
-                        /* Lambert, mirror and dielectric: the next-event terms
-                         * here and every later term carry this albedo once,
-                         * linearly (Eval/Pdf = albedo). */
-                        const size_t to = texel_offset(sc, m, it);
-                        const V3 wgt = adj_weight(a, pix, C);
-                        const float al[3] = {sc.texels[to], sc.texels[to + 1], sc.texels[to + 2]};
-                        const float wv[3] = {wgt.x, wgt.y, wgt.z};
+    const uint32_t rounds = max(1u, min((uint32_t)kShadeRounds, count / (nwaves * 64u)));
+    const uint32_t chunk = rounds * 64u;
+    for (uint32_t cbase = wave * chunk; cbase < count; cbase += nwaves * chunk) {
+        const uint32_t nvalid = min(chunk, count - cbase);
+        /* counting sort of the chunk by shading bin; cnt[k] is wave-uniform */
+        int key[kShadeRounds];
+        uint32_t rank[kShadeRounds];
+        uint32_t cnt[kBins];
 #pragma unroll
-                        for (int c = 0; c < 3; ++c)
-                            if (al[c] != 0.0f && wv[c] != 0.0f) unsafeAtomicAdd(a.adj.grad_texels + to + c, wv[c] / al[c]);
+        for (int k = 0; k < kBins; ++k) cnt[k] = 0u;
+#pragma unroll
+        for (int r = 0; r < kShadeRounds; ++r) {
+            const uint32_t pos = r * 64 + lane;
+            key[r] = pos < nvalid ? (int)shade_bin<BATCH>(a, cur, hits, cbase + pos) : -1;
+            rank[r] = 0u;
+            unsigned long long rem = __ballot(key[r] >= 0);
+            while (rem) {
+                const int k0 = __shfl(key[r], __ffsll(rem) - 1);
+                const unsigned long long m = __ballot(key[r] == k0);
+                uint32_t r0 = 0u;
+#pragma unroll
+                for (int k = 0; k < kBins; ++k)
+                    if (k0 == k) {
+                        r0 = cnt[k];
+                        cnt[k] += (uint32_t)__popcll(m);
                     }
-
-                    /* SampleOneLight / EstimateDirect on a copy of the interaction
-                     * (integrator.cpp:252-315); the two casts are deferred. */
-                    const uint32_t nl = sc.nlights;
-                    if (!(nl == 0 || (nl == 1 && light >= 0))) {
-                        uint32_t li;
-                        do {
-                            li = rng.next_discrete(nl);
-                        } while ((int)li == light);
-                        const DLight L = sc.lights[li];
-                        Interaction ic = it;
-                        V3 f1 = v3s(0.0f), f2 = v3s(0.0f);
-                        float p1 = 0.0f, p2 = 0.0f;
-                        if ((supported_lobes(m.bsdf) & ~LOBE_SPECULAR) != 0) {
-                            /* AreaLight::SampleLi up to its cast (area_light.cpp:18-56) */
-                            float dx, dy;
-                            sample_disc(rng, L.sphere[3], dx, dy);
-                            V3 cdir = v3(L.sphere[0], L.sphere[1], L.sphere[2]) - ic.P;
-                            float dist = length(cdir);
-                            V3 cn = normalize(cdir);
-                            V3 pt = rotate_to_world(dx, dy, dist - L.sphere[3], cn);
-                            V3 ldir = normalize(pt);
-                            ic.wi = ldir;
-                            if (dot(ldir, ic.N) > 0.0f) {
-                                f1 = bsdf_eval(sc, m, ic);
-                                p1 = bsdf_pdf(m, ic);
-                                nee1 = true;
-                                wi1 = ldir;
-                                tfar1 = inf;
-                                if (sc.light_first) {
-                                    Hit lh;
-                                    nee1 = light_hit(sc, L, ic.P, ldir, lh);
-                                    if (nee1) {
-                                        nee_hits[2 * (size_t)q] = lh;
-                                        tfar1 = lh.t;
+                if (key[r] == k0) rank[r] = r0 + (uint32_t)__popcll(m & lanes_below);
+                rem &= ~m;
+            }
+        }
+        uint32_t bin_start[kBins];
+        bin_start[0] = 0u;
+#pragma unroll
+        for (int k = 1; k < kBins; ++k) bin_start[k] = bin_start[k - 1] + cnt[k - 1];
+#pragma unroll
+        for (int r = 0; r < kShadeRounds; ++r) {
+            if (key[r] >= 0) {
+                uint32_t st0 = 0u;
+#pragma unroll
+                for (int k = 1; k < kBins; ++k)
+                    if (key[r] == k) st0 = bin_start[k];
+                s_order[st0 + rank[r]] = cbase + r * 64 + lane;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int r = 0; r < kShadeRounds; ++r) {
+            const uint32_t pos = r * 64 + lane;
+            const bool valid = pos < nvalid;
+            const uint32_t q = valid ? s_order[pos] : 0u;
+            bool alive = false;
+            bool nee1 = false, nee2 = false;
+            float tfar1 = 0.0f, tfar2 = 0.0f;
+            uint32_t p = 0;
+            V3 new_o = v3s(0.0f), new_d = v3s(0.0f);
+            V3 P = v3s(0.0f), wi1 = v3s(0.0f), wi2 = v3s(0.0f);
+            uint32_t ignore_geom = 0xFFFFFFFFu;
+            uint32_t scene = 0;
+            if (valid) {
+                p = cur.path[q];
+                const PathInfo pi = path_info<BATCH>(a, p, BATCH ? a.batch.path_seg[p] : 0u);
+                const uint32_t x = pi.x, y = pi.y;
+                scene = pi.scene;
+                const DScene &sc = BATCH ? a.batch.scenes[scene] : a.sc;
+                const uint32_t pix = y * a.cam.w + x;
+                PathRng rng;
+                rng.init(x, y, a.stream + scene, a.seed, pi.frame, a.ps.rngi[p]);
+                const float4 *rp = (const float4 *)(cur.rays + q);
+                float4 ra = rp[0], rb = rp[1];
+                const Hit h = hits[q];
+                V3 T = v3(a.ps.tx[p], a.ps.ty[p], a.ps.tz[p]);
+                V3 C = v3(a.ps.cx[p], a.ps.cy[p], a.ps.cz[p]);
+                Interaction it;
+                it.iori = a.ps.iori[p];
+                it.lobe = (int)a.ps.lobe[p];
+                int medium = a.ps.medium[p];
+                bool dead = false;
+                uint32_t nee_info = 0;
+                if (h.prim == 0xFFFFFFFFu) {
+                    if (ADJ) C = C - T * v3(sc.bg[0], sc.bg[1], sc.bg[2]);
+                    else C = C + T * v3(sc.bg[0], sc.bg[1], sc.bg[2]);
+                    dead = true;
+                } else {
+                    V3 origin = v3(ra.x, ra.y, ra.z);
+                    V3 dir = normalize(v3(rb.x, rb.y, rb.z));
+                    bool surface = true;
+                    if (medium >= 0) {                                                   /* integrator.cpp:131-163 */
+                        const DMedium md = sc.media[medium];
+                        if (md.type == MEDIUM_GRID) {
+                            const DGrid g = sc.grids[md.grid];
+                            float sbar = md.scale * g.maxd;
+                            float t = 0.0f, t1 = 0.0f;
+                            bool collided = false;
+                            if (sbar > 0.0f && grid_span(g, origin, dir, 0.0f, h.t, t, t1)) {
+                                while (true) {                                         /* delta tracking */
+                                    t = t - log_f(1.0f - rng.next_float()) / sbar;
+                                    if (!(t < t1)) break;
+                                    V3 qp = origin + dir * t;
+                                    const long vox = grid_voxel(g, qp.x, qp.y, qp.z);
+                                    const float dens = vox < 0 ? 0.0f : sc.voxels[vox];
+                                    const bool real = rng.next_float() * sbar < md.scale * dens;
+                                    if (ADJ && vox >= 0) {
+                                        /* d log P / d density: 1/density (collision),
+                                         * -scale/(sbar - scale*density) (null) */
+                                        const V3 wgt = adj_weight(a, pix, C);
+                                        const float dl = wgt.x + wgt.y + wgt.z;
+                                        const float dlogp = real ? 1.0f / dens : -md.scale / (sbar - md.scale * dens);
+                                        unsafeAtomicAdd(a.adj.grad_voxels + vox, dl * dlogp);
+                                    }
+                                    if (real) {
+                                        collided = true;
+                                        break;
                                     }
                                 }
                             }
-                        }
-                        bsdf_sample(m, ic, rng);
-                        f2 = bsdf_eval(sc, m, ic);
-                        p2 = bsdf_pdf(m, ic);
-                        if (p2 != 0.0f && !is_black(f2) && dot(ic.wi, ic.N) > 0.0f) {  /* AreaLight::PdfLi horizon test */
-                            nee2 = true;
-                            wi2 = ic.wi;
-                            tfar2 = inf;
-                            if (sc.light_first) {
-                                Hit lh;
-                                nee2 = light_hit(sc, L, ic.P, wi2, lh);
-                                if (nee2) {
-                                    nee_hits[2 * (size_t)q + 1] = lh;
-                                    tfar2 = lh.t;
-                                }
+                            if (collided) {
+                                surface = false;
+                                T = T * md.albedo;
+                                new_o = origin + dir * t;
+                                new_d = sample_hg(md.g, dir, rng);
+                            }
+                        } else {
+                            float dist = h.t;
+                            if (md.type == MEDIUM_ISOTROPIC) {                           /* isotropic_scattering_medium.h:31-40 */
+                                float d = -log_f(rng.next_float()) / md.sigma_s;
+                                dist = d >= h.t ? h.t : d;
+                            }
+                            V3 na = -v3(md.sigma_a[0], md.sigma_a[1], md.sigma_a[2]);
+                            V3 tr = v3(exp_f(na.x * dist), exp_f(na.y * dist), exp_f(na.z * dist));
+                            T = (T * 1.0f) * tr;
+                            if (dist < h.t) {
+                                surface = false;
+                                new_o = origin + dir * dist;
+                                new_d = uniform_sphere(rng);
                             }
                         }
-                        if (nee1 || nee2) {
-                            nee.tx[q] = T.x; nee.ty[q] = T.y; nee.tz[q] = T.z;
-                            nee.f1x[q] = f1.x; nee.f1y[q] = f1.y; nee.f1z[q] = f1.z; nee.p1[q] = p1;
-                            nee.f2x[q] = f2.x; nee.f2y[q] = f2.y; nee.f2z[q] = f2.z; nee.p2[q] = p2;
-                            nee.nl[q] = (float)nl;
-                            nee_info = li | (nee1 ? 0x40000000u : 0u) | (nee2 ? 0x80000000u : 0u);
-                            ignore_geom = L.geom;
+                    }
+                    if (surface) {
+                        const uint32_t gi = sc.tri_geom[h.prim];
+                        const DGeom g = sc.geoms[gi];
+                        const DMaterial m = sc.mats[g.material];
+                        const int light = g.light;
+                        if ((b == 0 || (it.lobe & LOBE_SPECULAR) != 0) && light >= 0) {
+                            const DLight L = sc.lights[light];
+                            if (ADJ) C = C - T * v3(L.le[0], L.le[1], L.le[2]);
+                            else C = C + T * v3(L.le[0], L.le[1], L.le[2]);
+                        }
+                        it.P = origin + dir * h.t;
+                        if (g.flags & 1u) it.N = normalize(interp_normal(sc, h.prim, h.u, h.v, g));
+                        else it.N = normalize(v3s(0.0f));
+                        if (g.flags & 2u) interp_uv(sc, h.prim, h.u, h.v, g, it.tu, it.tv);
+                        else it.tu = it.tv = 0.0f;
+                        it.wo = -dir;
+                        it.ioro = 0.0f;
+                        P = it.P;
+                        if (ADJ && m.texture >= 0 && m.bsdf != BSDF_GGX) {
+                            /* Lambert, mirror and dielectric: the next-event terms
+                             * here and every later term carry this albedo once,
+                             * linearly (Eval/Pdf = albedo). */
+                            const size_t to = texel_offset(sc, m, it);
+                            const V3 wgt = adj_weight(a, pix, C);
+                            const float al[3] = {sc.texels[to], sc.texels[to + 1], sc.texels[to + 2]};
+                            const float wv[3] = {wgt.x, wgt.y, wgt.z};
+#pragma unroll
+                            for (int c = 0; c < 3; ++c)
+                                if (al[c] != 0.0f && wv[c] != 0.0f) unsafeAtomicAdd(a.adj.grad_texels + to + c, wv[c] / al[c]);
+                        }
+
+                        /* SampleOneLight / EstimateDirect on a copy of the interaction
+                         * (integrator.cpp:252-315); the two casts are deferred. */
+                        const uint32_t nl = sc.nlights;
+                        if (!(nl == 0 || (nl == 1 && light >= 0))) {
+                            uint32_t li;
+                            do {
+                                li = rng.next_discrete(nl);
+                            } while ((int)li == light);
+                            const DLight L = sc.lights[li];
+                            Interaction ic = it;
+                            V3 f1 = v3s(0.0f), f2 = v3s(0.0f);
+                            float p1 = 0.0f, p2 = 0.0f;
+                            if ((supported_lobes(m.bsdf) & ~LOBE_SPECULAR) != 0) {
+                                /* AreaLight::SampleLi up to its cast (area_light.cpp:18-56) */
+                                float dx, dy;
+                                sample_disc(rng, L.sphere[3], dx, dy);
+                                V3 cdir = v3(L.sphere[0], L.sphere[1], L.sphere[2]) - ic.P;
+                                float dist = length(cdir);
+                                V3 cn = normalize(cdir);
+                                V3 pt = rotate_to_world(dx, dy, dist - L.sphere[3], cn);
+                                V3 ldir = normalize(pt);
+                                ic.wi = ldir;
+                                if (dot(ldir, ic.N) > 0.0f) {
+                                    f1 = bsdf_eval(sc, m, ic);
+                                    p1 = bsdf_pdf(m, ic);
+                                    nee1 = true;
+                                    wi1 = ldir;
+                                    tfar1 = inf;
+                                    if (sc.light_first) {
+                                        Hit lh;
+                                        nee1 = light_hit(sc, L, ic.P, ldir, lh);
+                                        if (nee1) {
+                                            nee_hits[2 * (size_t)q] = lh;
+                                            tfar1 = lh.t;
+                                        }
+                                    }
+                                }
+                            }
+                            bsdf_sample(m, ic, rng);
+                            f2 = bsdf_eval(sc, m, ic);
+                            p2 = bsdf_pdf(m, ic);
+                            if (p2 != 0.0f && !is_black(f2) && dot(ic.wi, ic.N) > 0.0f) {  /* AreaLight::PdfLi horizon test */
+                                nee2 = true;
+                                wi2 = ic.wi;
+                                tfar2 = inf;
+                                if (sc.light_first) {
+                                    Hit lh;
+                                    nee2 = light_hit(sc, L, ic.P, wi2, lh);
+                                    if (nee2) {
+                                        nee_hits[2 * (size_t)q + 1] = lh;
+                                        tfar2 = lh.t;
+                                    }
+                                }
+                            }
+                            if (nee1 || nee2) {
+                                nee.tx[q] = T.x; nee.ty[q] = T.y; nee.tz[q] = T.z;
+                                nee.f1x[q] = f1.x; nee.f1y[q] = f1.y; nee.f1z[q] = f1.z; nee.p1[q] = p1;
+                                nee.f2x[q] = f2.x; nee.f2y[q] = f2.y; nee.f2z[q] = f2.z; nee.p2[q] = p2;
+                                nee.nl[q] = (float)nl;
+                                nee_info = li | (nee1 ? 0x40000000u : 0u) | (nee2 ? 0x80000000u : 0u);
+                                ignore_geom = L.geom;
+                            }
+                        }
+                        /* continuation (integrator.cpp:199-231) */
+                        bsdf_sample(m, it, rng);
+                        float pdf = bsdf_pdf(m, it);
+                        if (pdf == 0.0f) {
+                            dead = true;   /* the reference divides 0/0 here */
+                        } else {
+                            T = (T * bsdf_eval(sc, m, it)) / pdf;
+                            if (it.lobe == LOBE_SPEC_TRANS) {
+                                it.iori = it.ioro;
+                                medium = m.medium;
+                            }
+                            new_o = it.P;
+                            new_d = it.wi;
                         }
                     }
-                    /* continuation (integrator.cpp:199-231) */
-                    bsdf_sample(m, it, rng);
-                    float pdf = bsdf_pdf(m, it);
-                    if (pdf == 0.0f) {
-                        dead = true;   /* the reference divides 0/0 here */
-                    } else {
-                        T = (T * bsdf_eval(sc, m, it)) / pdf;
-                        if (it.lobe == LOBE_SPEC_TRANS) {
-                            it.iori = it.ioro;
-                            medium = m.medium;
-                        }
-                        new_o = it.P;
-                        new_d = it.wi;
+                    if (!dead && b >= a.rr_start) {                                      /* integrator.cpp:235-242 */
+                        float pr = maxc(T);
+                        if (rng.next_float() > pr || !(pr > 0.0f)) dead = true;
+                        else T = T * (1.0f / pr);
                     }
                 }
-                if (!dead && b >= a.rr_start) {                                      /* integrator.cpp:235-242 */
-                    float pr = maxc(T);
-                    if (rng.next_float() > pr || !(pr > 0.0f)) dead = true;
-                    else T = T * (1.0f / pr);
+                nee.info[q] = nee_info;
+                alive = !dead && (b + 1 < a.max_depth);
+                if (!alive) a.ps.bounces[p] = dead ? b : b + 1;
+                a.ps.tx[p] = T.x; a.ps.ty[p] = T.y; a.ps.tz[p] = T.z;
+                a.ps.cx[p] = C.x; a.ps.cy[p] = C.y; a.ps.cz[p] = C.z;
+                a.ps.iori[p] = it.iori;
+                a.ps.medium[p] = medium;
+                a.ps.lobe[p] = (uint32_t)it.lobe;
+                a.ps.rngi[p] = rng.i;
+            }
+            /* wave-level compaction of the survivors and of the next-event rays */
+            const unsigned long long below = lanes_below;
+            unsigned long long mask = __ballot(alive);
+            unsigned long long nmask1 = __ballot(nee1);
+            unsigned long long nmask2 = __ballot(nee2);
+            uint32_t slot0 = 0, nslot0 = 0;
+            if (lane == 0) {
+                if (mask) slot0 = atomicAdd(&ctl->count[b + 1], (uint32_t)__popcll(mask));
+                uint32_t ns = (uint32_t)(__popcll(nmask1) + __popcll(nmask2));
+                if (ns) nslot0 = atomicAdd(&ctl->nee_count[b], ns);
+            }
+            slot0 = __shfl(slot0, 0);
+            nslot0 = __shfl(nslot0, 0);
+            if (alive) {
+                uint32_t off = (uint32_t)__popcll(mask & below);
+                uint32_t s = slot0 + off;
+                next.path[s] = p;
+                if (BATCH) next.sid[s] = scene;
+                store_ray(next.rays + s, new_o, new_d, 0.001f, inf);
+            }
+            if (nee1 || nee2) {
+                uint32_t c = nslot0 + (uint32_t)(__popcll(nmask1 & below) + __popcll(nmask2 & below));
+                if (nee1) {
+                    store_ray(nee_rays + c, P, wi1, 0.001f, tfar1);
+                    nee_ignore[c] = ignore_geom;
+                    if (BATCH) a.batch.nee_sid[c] = scene;
+                    nee_cidx[2 * (size_t)q] = c;
+                    ++c;
+                }
+                if (nee2) {
+                    store_ray(nee_rays + c, P, wi2, 0.001f, tfar2);
+                    nee_ignore[c] = ignore_geom;
+                    if (BATCH) a.batch.nee_sid[c] = scene;
+                    nee_cidx[2 * (size_t)q + 1] = c;
                 }
             }
-            nee.info[q] = nee_info;
-            alive = !dead && (b + 1 < a.max_depth);
-            if (!alive) a.ps.bounces[p] = dead ? b : b + 1;
-            a.ps.tx[p] = T.x; a.ps.ty[p] = T.y; a.ps.tz[p] = T.z;
-            a.ps.cx[p] = C.x; a.ps.cy[p] = C.y; a.ps.cz[p] = C.z;
-            a.ps.iori[p] = it.iori;
-            a.ps.medium[p] = medium;
-            a.ps.lobe[p] = (uint32_t)it.lobe;
-            a.ps.rngi[p] = rng.i;
         }
-        /* wave-level compaction of the survivors and of the next-event rays */
-        const unsigned long long below = (1ull << lane) - 1ull;
-        unsigned long long mask = __ballot(alive);
-        unsigned long long nmask1 = __ballot(nee1);
-        unsigned long long nmask2 = __ballot(nee2);
-        uint32_t slot0 = 0, nslot0 = 0;
-        if (lane == 0) {
-            if (mask) slot0 = atomicAdd(&ctl->count[b + 1], (uint32_t)__popcll(mask));
-            uint32_t ns = (uint32_t)(__popcll(nmask1) + __popcll(nmask2));
-            if (ns) nslot0 = atomicAdd(&ctl->nee_count[b], ns);
-        }
-        slot0 = __shfl(slot0, 0);
-        nslot0 = __shfl(nslot0, 0);
-        if (alive) {
-            uint32_t off = (uint32_t)__popcll(mask & below);
-            uint32_t s = slot0 + off;
-            next.path[s] = p;
-            if (BATCH) next.sid[s] = scene;
-            store_ray(next.rays + s, new_o, new_d, 0.001f, inf);
-        }
-        if (nee1 || nee2) {
-            uint32_t c = nslot0 + (uint32_t)(__popcll(nmask1 & below) + __popcll(nmask2 & below));
-            if (nee1) {
-                store_ray(nee_rays + c, P, wi1, 0.001f, tfar1);
-                nee_ignore[c] = ignore_geom;
-                if (BATCH) a.batch.nee_sid[c] = scene;
-                nee_cidx[2 * (size_t)q] = c;
-                ++c;
-            }
-            if (nee2) {
-                store_ray(nee_rays + c, P, wi2, 0.001f, tfar2);
-                nee_ignore[c] = ignore_geom;
-                if (BATCH) a.batch.nee_sid[c] = scene;
-                nee_cidx[2 * (size_t)q + 1] = c;
-            }
-        }
+        /* the next chunk's sort overwrites s_order */
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -907,7 +991,7 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
                 tm.end(T_EXT);
                 ++ext_launches;
                 tm.begin(T_SHADE);
-                hipLaunchKernelGGL((k_shade<ADJ, BATCH>), dim3(sgrid), dim3(256), 0, st, a, b, cur,
+                hipLaunchKernelGGL((k_shade<ADJ, BATCH>), dim3(sgrid), dim3(kShadeBlock), 0, st, a, b, cur,
                                    (const Hit *)w.hits, nxt, w.nee, w.nee_rays, w.nee_hits, w.nee_ignore, w.nee_cidx,
                                    ctl);
                 tm.end(T_SHADE);
