@@ -226,7 +226,8 @@ typedef struct lt_render_params {
     uint32_t seed;         /* counter-based RNG key */
     uint32_t frame0;       /* frame number of the first sample (Integrator::m_frameNumber) */
     uint32_t stream;       /* RNG stream id (scene index in a batch) */
-    uint32_t wave_paths;   /* paths in flight per wave; 0 = automatic */
+    uint32_t wave_paths;   /* paths in flight per wave; 0 = automatic (64M paths, about
+                              24 GB of workspace, or less if device memory is short) */
     uint32_t block_size;   /* traversal block size (64, 128 or 256); 0 = automatic */
     int32_t use_graph;     /* 1: capture the bounce loop in a hipGraph and replay it */
     uint32_t trace_variant;/* 0 = automatic; 1 + variant of the 8-wide traversal kernel
@@ -266,10 +267,12 @@ int lt_render(lt_scene *scene, const lt_camera *camera, const lt_render_params *
 int lt_render_batch(lt_scene *const *scenes, const lt_camera *cameras, lt_film *const *films, uint32_t n,
                     const lt_render_params *params, lt_render_stats *stats);
 /* Autotuner (no reference counterpart): times short probe renders of the
- * batch over traversal block sizes {64, 128, 256}, kernel variants 1-4 (LDS
- * stack depth 16 or 8, register/occupancy targets) and wave sizes {4M, 8M,
- * 16M} paths; with try_layouts also both node widths (8-wide and binary; the
- * scenes are re-committed and left in the fastest). Films are reset after. */
+ * batch over traversal block sizes {64, 128, 256} and kernel variants 1-4
+ * (LDS stack depth 16 or 8, register/occupancy targets); with try_layouts
+ * also both node widths (8-wide and binary; the scenes are re-committed and
+ * left in the fastest). Films are reset after. The wave size is not probed
+ * (a short probe fits any wave; larger waves measured faster on full
+ * renders): wave_paths echoes params->wave_paths. */
 typedef struct lt_tune_result {
     uint32_t block_size;     /* best lt_render_params values */
     uint32_t trace_variant;

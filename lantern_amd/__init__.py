@@ -241,8 +241,8 @@ def render_batch(scenes, cameras, params, films, sequential=False):
 
 
 def autotune(scenes, cameras, params, films, try_layouts=False):
-    """lt_autotune: the fastest block size, kernel variant, wave size (and
-    node width) for this batch. Returns a dict; apply it with tuned_params."""
+    """lt_autotune: the fastest block size and kernel variant (and node
+    width) for this batch. Returns a dict; apply it with tuned_params."""
     n = len(scenes)
     out = _capi.lt_tune_result()
     hs = (C.c_void_p * n)(*[s.h.value for s in scenes])

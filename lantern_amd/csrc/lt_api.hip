@@ -987,7 +987,6 @@ int lt_autotune(lt_scene *const *scenes, const lt_camera *cams, lt_film *const *
     probe.spp = std::min<uint32_t>(params->spp, 2u);
     probe.use_graph = 0;
     const uint32_t blocks[3] = {64, 128, 256};
-    const uint32_t waves[3] = {4u << 20, 8u << 20, 16u << 20};
     const uint32_t orig_layout = scenes[0]->layout;
     uint32_t layouts[2] = {orig_layout, 1u - orig_layout};
     const int nl = try_layouts ? 2 : 1;
@@ -1005,27 +1004,24 @@ int lt_autotune(lt_scene *const *scenes, const lt_camera *cams, lt_film *const *
         /* one untimed probe to settle allocations */
         if ((rc = render_batch(scenes, cams, films, n, probe, &st))) return rc;
         for (uint32_t b : blocks)
-            for (uint32_t v = 1; v <= (uint32_t)kTraceVariants; ++v)
-                for (uint32_t wv : waves) {
-                    lt_render_params p = probe;
-                    p.block_size = b;
-                    p.trace_variant = v;
-                    p.wave_paths = wv;
-                    if ((rc = render_batch(scenes, cams, films, n, p, &st))) return rc;
-                    out->candidates++;
-                    const bool is_default = b == (uint32_t)trace_default_block() &&
-                                            v == (uint32_t)kDefaultTraceVariant + 1 && wv == (16u << 20) &&
-                                            layouts[li] == orig_layout;
-                    if (is_default) out->default_ms = (float)st.total_ms;
-                    if ((float)st.total_ms < best) {
-                        best = (float)st.total_ms;
-                        out->block_size = b;
-                        out->trace_variant = v;
-                        out->wave_paths = wv;
-                        out->layout = layouts[li];
-                    }
+            for (uint32_t v = 1; v <= (uint32_t)kTraceVariants; ++v) {
+                lt_render_params p = probe;
+                p.block_size = b;
+                p.trace_variant = v;
+                if ((rc = render_batch(scenes, cams, films, n, p, &st))) return rc;
+                out->candidates++;
+                const bool is_default = b == (uint32_t)trace_default_block() &&
+                                        v == (uint32_t)kDefaultTraceVariant + 1 && layouts[li] == orig_layout;
+                if (is_default) out->default_ms = (float)st.total_ms;
+                if ((float)st.total_ms < best) {
+                    best = (float)st.total_ms;
+                    out->block_size = b;
+                    out->trace_variant = v;
+                    out->layout = layouts[li];
                 }
+            }
     }
+    out->wave_paths = params->wave_paths;
     out->best_ms = best;
     if (scenes[0]->layout != out->layout) {
         for (uint32_t i = 0; i < n; ++i) {

@@ -75,6 +75,8 @@ struct TraceBatch {
     const uint32_t *ray_scene;
 };
 constexpr int kDefaultTraceVariant = 0;
+/* Default paths in flight per render wave (lt_render_params.wave_paths 0). */
+constexpr size_t kDefaultWavePaths = (size_t)64 << 20;
 int launch_trace_closest(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, const uint32_t *count_ptr,
                          uint32_t count, uint32_t *work, int block, int grid, int *spill, uint32_t *overflow,
                          int variant, const TraceBatch *batch = nullptr);

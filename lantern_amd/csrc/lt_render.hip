@@ -1108,7 +1108,17 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
     return LT_OK;
 }
 
-size_t wave_budget(const lt_render_params &prm) { return prm.wave_paths ? prm.wave_paths : (size_t)16 << 20; }
+/* Paths in flight per wave. The default is kDefaultWavePaths (about 24 GB of
+ * workspace: fewer, fuller launches and shorter tails than small waves), or
+ * what half of the device memory not yet in use by other allocations holds,
+ * if that is less. */
+size_t wave_budget(lt_device *dev, const lt_render_params &prm) {
+    if (prm.wave_paths) return prm.wave_paths;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return (size_t)16 << 20;
+    const size_t usable = (free_b + dev->ws_paths * kBytesPerPath) / 2;
+    return std::max<size_t>((size_t)1 << 20, std::min<size_t>(kDefaultWavePaths, usable / kBytesPerPath));
+}
 
 int check_params(const lt_render_params &prm) {
     if (prm.max_depth == 0 || prm.max_depth > 2000) return set_error(LT_E_INVALID, "max_depth must be in 1..2000");
@@ -1126,7 +1136,7 @@ int render_impl(lt_scene *scene, const lt_camera &cam, const lt_render_params &p
     if (npix == 0 || prm.spp == 0) return LT_OK;
     int rc = check_params(prm);
     if (rc) return rc;
-    uint32_t spw = (uint32_t)std::max<size_t>(1, std::min<size_t>(prm.spp, wave_budget(prm) / npix));
+    uint32_t spw = (uint32_t)std::max<size_t>(1, std::min<size_t>(prm.spp, wave_budget(dev, prm) / npix));
     size_t nmax = (size_t)spw * npix;
     if (nmax > 0xFFFFFFFFull / 2) return set_error(LT_E_INVALID, "wave too large");
     Workspace w;
@@ -1193,7 +1203,7 @@ int render_batch(lt_scene *const *scenes, const lt_camera *cams, lt_film *const 
     for (DScene &d : tbl) d.light_first = light_first ? 1u : 0u;
 
     /* plan: pack whole samples of each scene into waves */
-    const size_t budget = wave_budget(prm);
+    const size_t budget = wave_budget(scenes[0]->dev, prm);
     std::vector<Seg> segs;
     std::vector<WavePlan> plan;
     WavePlan cur = {};
