@@ -37,6 +37,35 @@ constexpr int LDS_STACK = 24;
 constexpr int STACK_CAP = 96;
 constexpr uint32_t MAX_STEPS = 1u << 22;
 
+/* Hands out rays to the waves of a persistent launch: each atomic on the
+ * launch's work counter gives a wave R * 64 consecutive rays, which it traces
+ * 64 at a time. One device-wide counter takes on the order of 10^8 atomics
+ * per second, which a launch of cheap rays fetched 64 at a time reaches;
+ * R = kFeedRounds cuts the atomics to a quarter (at most a quarter of the
+ * launch's rays per wave, so that every wave still fetches about four times
+ * or more). Handing out consecutive rays in fetch order keeps the rays in
+ * flight neighbours in the queue (and mostly in the image), whose shared BVH
+ * nodes then stay in L2: dealing the rays out statically in stripes measured
+ * 60% slower on the triangle-heavy scene, and sizing each wave's fetch by
+ * the time its last rounds took 4% slower (a wave that drew cheap rays then
+ * takes a long run of expensive ones). */
+constexpr uint32_t kFeedRounds = 4;
+struct RayFeed {
+    uint32_t n, chunk;
+    LT_D RayFeed(uint32_t count, int block) {
+        const uint32_t nwaves = (gridDim.x * (uint32_t)block) >> 6;
+        n = count;
+        chunk = 64u * max(1u, min(kFeedRounds, count / (nwaves * 64u * 4u)));
+    }
+    /* first ray of the wave's next chunk; false when none are left */
+    LT_D bool next(uint32_t *work, int lane, uint32_t &base) {
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(work, chunk);
+        base = __shfl(b, 0);
+        return base < n;
+    }
+};
+
 struct TravRay {
     V3 o, d, idir;
     float tnear, tmin, tfar;
@@ -74,12 +103,17 @@ __global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, TraceBatch batch, co
     const int lane = tid & 63;
     int *gstack = spill + (size_t)(blockIdx.x * BLOCK + tid) * (STACK_CAP - LDS_STACK);
 
+    RayFeed feed(n, BLOCK);
+    uint32_t base;
+    uint32_t r = feed.chunk;
     while (true) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(work, 64u);
-        base = __shfl(base, 0);
-        if (base >= n) break;
-        const uint32_t i = base + lane;
+        /* the next 64 rays of the wave's chunk, or a new chunk */
+        if (r == feed.chunk) {
+            if (!feed.next(work, lane, base)) break;
+            r = 0;
+        }
+        const uint32_t i = base + r + lane;
+        r += 64u;
         if (i >= n) continue;
 
         const float4 *rp = (const float4 *)(rays + i);
@@ -403,11 +437,13 @@ LT_D bool trav8_visit(Trav8 &s, int *lds_node, float *lds_t, int tid, int2 *gsta
 }
 
 /* LDS8: stack entries kept in LDS per lane; WAVES: occupancy the register
- * allocator targets (0: the compiler's choice). A wave takes 64 new rays when
- * all its rays are finished. (Refilling single lanes as soon as enough of
- * them are idle, Aila & Laine 2009, measured 1.5-2x slower here: the lanes of
- * a wave then sit in unrelated parts of the tree, and the node fetches that
- * the wave's lanes used to share become separate cache lines.) */
+ * allocator targets (0: the compiler's choice). Rays go to waves 64 at a
+ * time, see next_rays. (Refilling single lanes as soon as enough of them are
+ * idle, Aila & Laine 2009, measured 1.5-2x slower here: the lanes of a wave
+ * then sit in unrelated parts of the tree, and the node fetches that the
+ * wave's lanes used to share become separate cache lines. Ordering each
+ * wave's rays by direction octant helped a scene of short random rays and
+ * cost the triangle-heavy scene 5%.) */
 template <bool ANY, int BLOCK, int LDS8, int WAVES, bool BATCH>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVES : 1)))
 void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__restrict__ hits,
@@ -419,18 +455,18 @@ void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     int2 *gstack = spill + (size_t)(blockIdx.x * BLOCK + tid) * STACK8_CAP;
+    RayFeed feed(n, BLOCK);
     Trav8 s;
-    while (true) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(work, 64u);
-        base = __shfl(base, 0);
-        if (base >= n) break;
-        const uint32_t i = base + lane;
-        if (i >= n) continue;
-        if (trav8_begin<ANY, BATCH>(s, sc, batch, rays, ignore_geom, i))
-            while (!trav8_visit<ANY, BLOCK, LDS8>(s, lds_node, lds_t, tid, gstack, overflow)) {
-            }
-        trav8_end<ANY>(s, hits, occluded);
+    uint32_t base;
+    while (feed.next(work, lane, base)) {
+        for (uint32_t r = 0; r < feed.chunk; r += 64u) {
+            const uint32_t i = base + r + lane;
+            if (i >= n) break;
+            if (trav8_begin<ANY, BATCH>(s, sc, batch, rays, ignore_geom, i))
+                while (!trav8_visit<ANY, BLOCK, LDS8>(s, lds_node, lds_t, tid, gstack, overflow)) {
+                }
+            trav8_end<ANY>(s, hits, occluded);
+        }
     }
 }
 
