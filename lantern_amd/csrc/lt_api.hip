@@ -245,7 +245,7 @@ int lt_device_create(int ordinal, lt_device **out) {
         delete d;
         return hip_error(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
     }
-    int rc = d->trace_work.alloc(64);
+    int rc = d->trace_work.alloc(kWorkWords * sizeof(uint32_t));
     if (!rc) rc = d->diag.alloc(64);
     if (!rc && hipMemset(d->diag.p, 0, 64) != hipSuccess) rc = set_error(LT_E_HIP, "hipMemset of the diagnostics word failed");
     if (rc) {
@@ -655,7 +655,7 @@ static int intersect_device(lt_scene *s, const lt_ray *rays, void *out, uint32_t
     int grid = trace_grid(d->num_cus, block, kDefaultTraceVariant);
     int rc = d->spill.reserve(trace_spill_bytes(grid, block));
     if (rc) return rc;
-    LT_HIP(hipMemsetAsync(d->trace_work.p, 0, sizeof(uint32_t), st));
+    LT_HIP(hipMemsetAsync(d->trace_work.p, 0, kWorkWords * sizeof(uint32_t), st));
     if (any)
         return launch_trace_any(st, s->ds, (const Ray *)rays, (uint32_t *)out, nullptr, nullptr, n,
                                 d->trace_work.as<uint32_t>(), block, grid, (int *)d->spill.p, d->diag.as<uint32_t>(),

@@ -75,6 +75,12 @@ struct TraceBatch {
     const uint32_t *ray_scene;
 };
 constexpr int kDefaultTraceVariant = 0;
+/* Work counters of one traversal launch: kWorkCounters counters (one per
+ * XCD of the MI355X), kWorkStride words apart so that each sits in a cache
+ * line of its own; all zero at launch. */
+constexpr int kWorkCounters = 8;
+constexpr int kWorkStride = 32;
+constexpr int kWorkWords = kWorkCounters * kWorkStride;
 /* Default paths in flight per render wave (lt_render_params.wave_paths 0). */
 constexpr size_t kDefaultWavePaths = (size_t)64 << 20;
 int launch_trace_closest(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, const uint32_t *count_ptr,

@@ -44,7 +44,7 @@ namespace {
 struct Ctl {
     uint32_t count[2048];      /* live paths entering bounce b */
     uint32_t nee_count[2048];  /* next-event rays queued (compacted) at bounce b */
-    uint32_t work[4096];       /* traversal work counters, one per launch */
+    uint32_t work[4096 * kWorkWords]; /* traversal work counters, kWorkWords per launch */
     unsigned long long shadow_rays;
     unsigned long long tally[2];   /* over all waves of a render: closest-hit rays, next-event rays */
     unsigned long long pad[5];
@@ -765,13 +765,14 @@ __global__ void k_reset_tally(Ctl *ctl) {
     if (blockIdx.x == 0 && threadIdx.x < 2) ctl->tally[threadIdx.x] = 0ull;
 }
 
-__global__ void k_reset_ctl(Ctl *ctl) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+/* Zeroes the counts and the work counters of the first nlaunch launches. */
+__global__ void k_reset_ctl(Ctl *ctl, uint32_t nlaunch) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 2048) {
         ctl->count[i] = 0;
         ctl->nee_count[i] = 0;
     }
-    if (i < 4096) ctl->work[i] = 0;
+    for (uint32_t j = i; j < nlaunch * (uint32_t)kWorkWords; j += gridDim.x * blockDim.x) ctl->work[j] = 0;
     if (i == 0) ctl->shadow_rays = 0;
 }
 
@@ -966,7 +967,7 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
                 a.batch.npix_slots = wp.npix_slots;
             }
             samples += n;
-            hipLaunchKernelGGL(k_reset_ctl, dim3(16), dim3(256), 0, st, ctl);
+            hipLaunchKernelGGL(k_reset_ctl, dim3(64), dim3(256), 0, st, ctl, 2u * prm.max_depth);
             tm.begin(T_GEN);
             hipLaunchKernelGGL((k_generate<ADJ, BATCH>), dim3(std::min<uint32_t>((n + 255) / 256, 65535)), dim3(256),
                                0, st, a, w.qa, n, ctl);
@@ -985,7 +986,7 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
                 TraceBatch tb_ext = {a.batch.scenes, cur.sid};
                 TraceBatch tb_nee = {a.batch.scenes, a.batch.nee_sid};
                 tm.begin(T_EXT);
-                if ((r = launch_trace_closest(st, a.sc, cur.rays, w.hits, &ctl->count[b], 0, &ctl->work[wslot++],
+                if ((r = launch_trace_closest(st, a.sc, cur.rays, w.hits, &ctl->count[b], 0, &ctl->work[(size_t)(wslot++) * kWorkWords],
                                               tblock, tgrid, spill, diag, tvar, BATCH ? &tb_ext : nullptr)))
                     return r;
                 tm.end(T_EXT);
@@ -1001,11 +1002,11 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
                 tm.begin(T_SHADOW);
                 if (a.sc.light_first)
                     r = launch_trace_any(st, a.sc, w.nee_rays, w.nee_occl, w.nee_ignore, &ctl->nee_count[b], 0,
-                                         &ctl->work[wslot++], tblock, tgrid, spill, diag, tvar,
+                                         &ctl->work[(size_t)(wslot++) * kWorkWords], tblock, tgrid, spill, diag, tvar,
                                          BATCH ? &tb_nee : nullptr);
                 else
                     r = launch_trace_closest(st, a.sc, w.nee_rays, w.nee_trace_hits, &ctl->nee_count[b], 0,
-                                             &ctl->work[wslot++], tblock, tgrid, spill, diag, tvar,
+                                             &ctl->work[(size_t)(wslot++) * kWorkWords], tblock, tgrid, spill, diag, tvar,
                                              BATCH ? &tb_nee : nullptr);
                 if (r) return r;
                 tm.end(T_SHADOW);

@@ -37,13 +37,18 @@ constexpr int LDS_STACK = 24;
 constexpr int STACK_CAP = 96;
 constexpr uint32_t MAX_STEPS = 1u << 22;
 
-/* Hands out rays to the waves of a persistent launch: each atomic on the
- * launch's work counter gives a wave R * 64 consecutive rays, which it traces
- * 64 at a time. One device-wide counter takes on the order of 10^8 atomics
- * per second, which a launch of cheap rays fetched 64 at a time reaches;
- * R = kFeedRounds cuts the atomics to a quarter (at most a quarter of the
- * launch's rays per wave, so that every wave still fetches about four times
- * or more). Handing out consecutive rays in fetch order keeps the rays in
+/* Hands out rays to the waves of a persistent launch. The rays are split
+ * into kWorkCounters contiguous segments with a work counter each; a wave
+ * starts on segment blockIdx.x % kWorkCounters (the XCD that workgroups are
+ * dispatched to round-robin, so each XCD's L2 sees a contiguous part of the
+ * queue), takes R * 64 consecutive rays per atomic on that segment's counter
+ * and traces them 64 at a time; once its segment is used up it moves on to
+ * the next one, so no XCD idles while another has rays left. One
+ * device-wide counter takes on the order of 10^8 atomics per second, which
+ * a launch of cheap rays fetched 64 at a time reached; eight counters and
+ * R = kFeedRounds (at most a quarter of the launch's rays per wave, so that
+ * every wave still fetches about four times or more) keep each counter far
+ * below that. Handing out consecutive rays in fetch order keeps the rays in
  * flight neighbours in the queue (and mostly in the image), whose shared BVH
  * nodes then stay in L2: dealing the rays out statically in stripes measured
  * 60% slower on the triangle-heavy scene, and sizing each wave's fetch by
@@ -51,18 +56,33 @@ constexpr uint32_t MAX_STEPS = 1u << 22;
  * takes a long run of expensive ones). */
 constexpr uint32_t kFeedRounds = 4;
 struct RayFeed {
-    uint32_t n, chunk;
+    uint32_t n, chunk, seg, left, end;
     LT_D RayFeed(uint32_t count, int block) {
         const uint32_t nwaves = (gridDim.x * (uint32_t)block) >> 6;
         n = count;
         chunk = 64u * max(1u, min(kFeedRounds, count / (nwaves * 64u * 4u)));
+        seg = blockIdx.x % (uint32_t)kWorkCounters;
+        left = (uint32_t)kWorkCounters;
+        end = 0;
     }
-    /* first ray of the wave's next chunk; false when none are left */
+    LT_D uint32_t seg_start(uint32_t k) const { return (uint32_t)(((uint64_t)n * k) / (uint32_t)kWorkCounters); }
+    /* first ray of the wave's next chunk, which ends before `end`; false when
+     * every segment is used up */
     LT_D bool next(uint32_t *work, int lane, uint32_t &base) {
-        uint32_t b = 0;
-        if (lane == 0) b = atomicAdd(work, chunk);
-        base = __shfl(b, 0);
-        return base < n;
+        while (left != 0) {
+            const uint32_t lo = seg_start(seg), hi = seg_start(seg + 1);
+            uint32_t b = 0;
+            if (lane == 0 && lo < hi) b = atomicAdd(work + seg * (uint32_t)kWorkStride, chunk);
+            b = __shfl(b, 0);
+            if (lo < hi && b < hi - lo) {
+                base = lo + b;
+                end = hi;
+                return true;
+            }
+            seg = (seg + 1u) % (uint32_t)kWorkCounters;
+            --left;
+        }
+        return false;
     }
 };
 
@@ -114,7 +134,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, TraceBatch batch, co
         }
         const uint32_t i = base + r + lane;
         r += 64u;
-        if (i >= n) continue;
+        if (i >= feed.end) continue;
 
         const float4 *rp = (const float4 *)(rays + i);
         float4 ra = rp[0], rb = rp[1];
@@ -461,7 +481,7 @@ void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__
     while (feed.next(work, lane, base)) {
         for (uint32_t r = 0; r < feed.chunk; r += 64u) {
             const uint32_t i = base + r + lane;
-            if (i >= n) break;
+            if (i >= feed.end) break;
             if (trav8_begin<ANY, BATCH>(s, sc, batch, rays, ignore_geom, i))
                 while (!trav8_visit<ANY, BLOCK, LDS8>(s, lds_node, lds_t, tid, gstack, overflow)) {
                 }
