@@ -365,7 +365,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(WaveArgs a, uint32_t b, Q
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (int r = 0; r < kShadeRounds; ++r) {
+        for (uint32_t r = 0; r < rounds; ++r) {
             const uint32_t pos = r * 64 + lane;
             const bool valid = pos < nvalid;
             const uint32_t q = valid ? s_order[pos] : 0u;
@@ -384,24 +384,26 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(WaveArgs a, uint32_t b, Q
                 scene = pi.scene;
                 const DScene &sc = BATCH ? a.batch.scenes[scene] : a.sc;
                 const uint32_t pix = y * a.cam.w + x;
-                PathRng rng;
-                rng.init(x, y, a.stream + scene, a.seed, pi.frame, a.ps.rngi[p]);
-                const float4 *rp = (const float4 *)(cur.rays + q);
-                float4 ra = rp[0], rb = rp[1];
                 const Hit h = hits[q];
                 V3 T = v3(a.ps.tx[p], a.ps.ty[p], a.ps.tz[p]);
                 V3 C = v3(a.ps.cx[p], a.ps.cy[p], a.ps.cz[p]);
-                Interaction it;
-                it.iori = a.ps.iori[p];
-                it.lobe = (int)a.ps.lobe[p];
-                int medium = a.ps.medium[p];
                 bool dead = false;
                 uint32_t nee_info = 0;
                 if (h.prim == 0xFFFFFFFFu) {
+                    /* a miss ends the path: only its color and bounce count are
+                     * read again */
                     if (ADJ) C = C - T * v3(sc.bg[0], sc.bg[1], sc.bg[2]);
                     else C = C + T * v3(sc.bg[0], sc.bg[1], sc.bg[2]);
                     dead = true;
                 } else {
+                    PathRng rng;
+                    rng.init(x, y, a.stream + scene, a.seed, pi.frame, a.ps.rngi[p]);
+                    const float4 *rp = (const float4 *)(cur.rays + q);
+                    const float4 ra = rp[0], rb = rp[1];
+                    Interaction it;
+                    it.iori = a.ps.iori[p];
+                    it.lobe = (int)a.ps.lobe[p];
+                    int medium = a.ps.medium[p];
                     V3 origin = v3(ra.x, ra.y, ra.z);
                     V3 dir = normalize(v3(rb.x, rb.y, rb.z));
                     bool surface = true;
@@ -570,16 +572,19 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(WaveArgs a, uint32_t b, Q
                         if (rng.next_float() > pr || !(pr > 0.0f)) dead = true;
                         else T = T * (1.0f / pr);
                     }
+                    /* the rest of the state matters only to a path that goes on */
+                    if (!dead && b + 1 < a.max_depth) {
+                        a.ps.tx[p] = T.x; a.ps.ty[p] = T.y; a.ps.tz[p] = T.z;
+                        a.ps.iori[p] = it.iori;
+                        a.ps.medium[p] = medium;
+                        a.ps.lobe[p] = (uint32_t)it.lobe;
+                        a.ps.rngi[p] = rng.i;
+                    }
                 }
                 nee.info[q] = nee_info;
                 alive = !dead && (b + 1 < a.max_depth);
                 if (!alive) a.ps.bounces[p] = dead ? b : b + 1;
-                a.ps.tx[p] = T.x; a.ps.ty[p] = T.y; a.ps.tz[p] = T.z;
                 a.ps.cx[p] = C.x; a.ps.cy[p] = C.y; a.ps.cz[p] = C.z;
-                a.ps.iori[p] = it.iori;
-                a.ps.medium[p] = medium;
-                a.ps.lobe[p] = (uint32_t)it.lobe;
-                a.ps.rngi[p] = rng.i;
             }
             /* wave-level compaction of the survivors and of the next-event rays */
             const unsigned long long below = lanes_below;
