@@ -60,16 +60,19 @@ struct CameraArgs {
     float cdf[33];
 };
 
-/* Path state, SoA, indexed by path id. */
+/* Path state, indexed by path id, as records: after the first bounce the
+ * live paths of a queue slot range are scattered over the path ids, and a
+ * lane then touches one or two cache lines for its state instead of one per
+ * field. */
 struct PathState {
-    float *tx, *ty, *tz;      /* throughput */
-    float *cx, *cy, *cz;      /* radiance accumulated so far */
-    float *iori;
-    int *medium;
-    uint32_t *lobe;
-    uint32_t *rngi;           /* draws consumed */
-    uint32_t *bounces;        /* final bounce count */
+    float4 *rec;              /* 2 per path: (throughput, IOR), (medium, lobe, draws consumed, -) as bits */
+    float4 *col;              /* (radiance accumulated so far, final bounce count as bits) */
 };
+__device__ __forceinline__ void store_rec(const PathState &ps, uint32_t p, V3 T, float iori, int medium, uint32_t lobe,
+                                          uint32_t rngi) {
+    ps.rec[2 * (size_t)p] = make_float4(T.x, T.y, T.z, iori);
+    ps.rec[2 * (size_t)p + 1] = make_float4(__int_as_float(medium), __uint_as_float(lobe), __uint_as_float(rngi), 0.0f);
+}
 /* One queue of rays (indexed by slot) with their owners (and, in a batch,
  * the scene of each ray). */
 struct Queue {
@@ -98,13 +101,12 @@ struct BatchArgs {
     uint32_t *path_seg;        /* per path: its segment */
     uint32_t *nee_sid;         /* per compacted next-event ray: its scene */
 };
-/* Next-event record, indexed by the shaded slot. */
+/* Next-event record, indexed by the shaded slot: 3 x float4 (throughput at
+ * the shading point, number of lights), (light-sample f, pdf), (BSDF-sample
+ * f, pdf), and the info word apart (read first, for every slot). */
 struct NeeRec {
-    float *tx, *ty, *tz;      /* throughput at the shading point */
-    float *f1x, *f1y, *f1z, *p1;
-    float *f2x, *f2y, *f2z, *p2;
+    float4 *rec;
     uint32_t *info;           /* light index | flag bits 30 (light sample) and 31 (BSDF sample) */
-    float *nl;
 };
 
 /* Path-replay adjoint (Vicini, Speierer & Jakob, "Path Replay
@@ -219,18 +221,13 @@ __global__ void k_generate(WaveArgs a, Queue q, uint32_t n, Ctl *ctl) {
         rp[0] = r0;
         rp[1] = r1;
         q.path[p] = p;
-        a.ps.tx[p] = 1.0f; a.ps.ty[p] = 1.0f; a.ps.tz[p] = 1.0f;
+        store_rec(a.ps, p, v3s(1.0f), 1.0f, -1, (uint32_t)LOBE_NULL, rng.i);
         if (ADJ) {
             const float *rad = a.adj.sample_rad + 3 * ((size_t)a.sample_base * a.npix + p);
-            a.ps.cx[p] = rad[0]; a.ps.cy[p] = rad[1]; a.ps.cz[p] = rad[2];
+            a.ps.col[p] = make_float4(rad[0], rad[1], rad[2], __uint_as_float(0u));
         } else {
-            a.ps.cx[p] = 0.0f; a.ps.cy[p] = 0.0f; a.ps.cz[p] = 0.0f;
+            a.ps.col[p] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
         }
-        a.ps.iori[p] = 1.0f;
-        a.ps.medium[p] = -1;
-        a.ps.lobe[p] = LOBE_NULL;
-        a.ps.rngi[p] = rng.i;
-        a.ps.bounces[p] = 0;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->count[0] = n;
 }
@@ -282,7 +279,7 @@ template <bool BATCH>
 __device__ __forceinline__ uint32_t shade_bin(const WaveArgs &a, const Queue &cur, const Hit *hits, uint32_t q) {
     const uint32_t prim = hits[q].prim;
     if (prim == 0xFFFFFFFFu) return 0u;
-    if (a.ps.medium[cur.path[q]] >= 0) return 1u;
+    if (__float_as_int(a.ps.rec[2 * (size_t)cur.path[q] + 1].x) >= 0) return 1u;
     const DScene &sc = BATCH ? a.batch.scenes[cur.sid[q]] : a.sc;
     const int bsdf = sc.mats[sc.geoms[sc.tri_geom[prim]].material].bsdf;
     return 2u + (uint32_t)min(max(bsdf, 0), kBins - 3);
@@ -385,8 +382,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(WaveArgs a, uint32_t b, Q
                 const DScene &sc = BATCH ? a.batch.scenes[scene] : a.sc;
                 const uint32_t pix = y * a.cam.w + x;
                 const Hit h = hits[q];
-                V3 T = v3(a.ps.tx[p], a.ps.ty[p], a.ps.tz[p]);
-                V3 C = v3(a.ps.cx[p], a.ps.cy[p], a.ps.cz[p]);
+                const float4 s0 = a.ps.rec[2 * (size_t)p];   /* throughput, IOR */
+                V3 T = v3(s0.x, s0.y, s0.z);
+                const float4 col = a.ps.col[p];
+                V3 C = v3(col.x, col.y, col.z);
                 bool dead = false;
                 uint32_t nee_info = 0;
                 if (h.prim == 0xFFFFFFFFu) {
@@ -396,14 +395,15 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(WaveArgs a, uint32_t b, Q
                     else C = C + T * v3(sc.bg[0], sc.bg[1], sc.bg[2]);
                     dead = true;
                 } else {
+                    const float4 s1 = a.ps.rec[2 * (size_t)p + 1];   /* medium, lobe, draws */
                     PathRng rng;
-                    rng.init(x, y, a.stream + scene, a.seed, pi.frame, a.ps.rngi[p]);
+                    rng.init(x, y, a.stream + scene, a.seed, pi.frame, __float_as_uint(s1.z));
                     const float4 *rp = (const float4 *)(cur.rays + q);
                     const float4 ra = rp[0], rb = rp[1];
                     Interaction it;
-                    it.iori = a.ps.iori[p];
-                    it.lobe = (int)a.ps.lobe[p];
-                    int medium = a.ps.medium[p];
+                    it.iori = s0.w;
+                    it.lobe = (int)__float_as_uint(s1.y);
+                    int medium = __float_as_int(s1.x);
                     V3 origin = v3(ra.x, ra.y, ra.z);
                     V3 dir = normalize(v3(rb.x, rb.y, rb.z));
                     bool surface = true;
@@ -544,10 +544,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(WaveArgs a, uint32_t b, Q
                                 }
                             }
                             if (nee1 || nee2) {
-                                nee.tx[q] = T.x; nee.ty[q] = T.y; nee.tz[q] = T.z;
-                                nee.f1x[q] = f1.x; nee.f1y[q] = f1.y; nee.f1z[q] = f1.z; nee.p1[q] = p1;
-                                nee.f2x[q] = f2.x; nee.f2y[q] = f2.y; nee.f2z[q] = f2.z; nee.p2[q] = p2;
-                                nee.nl[q] = (float)nl;
+                                nee.rec[3 * (size_t)q] = make_float4(T.x, T.y, T.z, (float)nl);
+                                nee.rec[3 * (size_t)q + 1] = make_float4(f1.x, f1.y, f1.z, p1);
+                                nee.rec[3 * (size_t)q + 2] = make_float4(f2.x, f2.y, f2.z, p2);
                                 nee_info = li | (nee1 ? 0x40000000u : 0u) | (nee2 ? 0x80000000u : 0u);
                                 ignore_geom = L.geom;
                             }
@@ -573,18 +572,12 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(WaveArgs a, uint32_t b, Q
                         else T = T * (1.0f / pr);
                     }
                     /* the rest of the state matters only to a path that goes on */
-                    if (!dead && b + 1 < a.max_depth) {
-                        a.ps.tx[p] = T.x; a.ps.ty[p] = T.y; a.ps.tz[p] = T.z;
-                        a.ps.iori[p] = it.iori;
-                        a.ps.medium[p] = medium;
-                        a.ps.lobe[p] = (uint32_t)it.lobe;
-                        a.ps.rngi[p] = rng.i;
-                    }
+                    if (!dead && b + 1 < a.max_depth) store_rec(a.ps, p, T, it.iori, medium, (uint32_t)it.lobe, rng.i);
                 }
                 nee.info[q] = nee_info;
                 alive = !dead && (b + 1 < a.max_depth);
-                if (!alive) a.ps.bounces[p] = dead ? b : b + 1;
-                a.ps.cx[p] = C.x; a.ps.cy[p] = C.y; a.ps.cz[p] = C.z;
+                /* the bounce count is final once the path ends */
+                a.ps.col[p] = make_float4(C.x, C.y, C.z, __uint_as_float(alive ? 0u : (dead ? b : b + 1)));
             }
             /* wave-level compaction of the survivors and of the next-event rays */
             const unsigned long long below = lanes_below;
@@ -659,8 +652,9 @@ __global__ __launch_bounds__(256) void k_resolve(WaveArgs a, uint32_t b, Queue c
                 V3 dd = ip - o;
                 float d2 = dot(dd, dd);
                 float lpdf = d2 / (fabsf(dot(normalize(interp_normal(sc, h.prim, h.u, h.v, lg)), -d)) * L.area);
-                V3 f1 = v3(nee.f1x[q], nee.f1y[q], nee.f1z[q]);
-                float p1 = nee.p1[q];
+                const float4 r1 = nee.rec[3 * (size_t)q + 1];
+                V3 f1 = v3(r1.x, r1.y, r1.z);
+                float p1 = r1.w;
                 if (lpdf != 0.0f && !is_black(le) && p1 != 0.0f && !is_black(f1)) {
                     float w = power_heuristic(lpdf, p1);
                     direct = direct + ((f1 * le) * w) / lpdf;
@@ -680,20 +674,24 @@ __global__ __launch_bounds__(256) void k_resolve(WaveArgs a, uint32_t b, Queue c
                 float d2 = dot(dd, dd);
                 float lpdf = d2 / (fabsf(dot(normalize(interp_normal(sc, h.prim, h.u, h.v, lg)), d)) * L.area);
                 if (lpdf != 0.0f) {
-                    float p2 = nee.p2[q];
-                    V3 f2 = v3(nee.f2x[q], nee.f2y[q], nee.f2z[q]);
+                    const float4 r2 = nee.rec[3 * (size_t)q + 2];
+                    float p2 = r2.w;
+                    V3 f2 = v3(r2.x, r2.y, r2.z);
                     float w = power_heuristic(p2, lpdf);
                     direct = direct + ((f2 * le) * w) / p2;
                 }
             }
         }
         const uint32_t p = cur.path[q];
-        V3 T = v3(nee.tx[q], nee.ty[q], nee.tz[q]);
-        V3 add = T * (nee.nl[q] * direct);
+        const float4 r0 = nee.rec[3 * (size_t)q];
+        V3 T = v3(r0.x, r0.y, r0.z);
+        V3 add = T * (r0.w * direct);
         if (ADJ) add = -add;
-        a.ps.cx[p] = a.ps.cx[p] + add.x;
-        a.ps.cy[p] = a.ps.cy[p] + add.y;
-        a.ps.cz[p] = a.ps.cz[p] + add.z;
+        float4 col = a.ps.col[p];
+        col.x = col.x + add.x;
+        col.y = col.y + add.y;
+        col.z = col.z + add.z;
+        a.ps.col[p] = col;
     }
 }
 
@@ -704,16 +702,17 @@ __global__ void k_accumulate(WaveArgs a, uint32_t nsamples, float *sum, uint32_t
         uint32_t bb = bounces[pix], cc = count[pix];
         for (uint32_t s = 0; s < nsamples; ++s) {
             size_t p = (size_t)s * a.npix + pix;
+            const float4 col = a.ps.col[p];
             if (a.adj.sample_rad) {
                 float *rad = a.adj.sample_rad + 3 * ((size_t)a.sample_base * a.npix + p);
-                rad[0] = a.ps.cx[p];
-                rad[1] = a.ps.cy[p];
-                rad[2] = a.ps.cz[p];
+                rad[0] = col.x;
+                rad[1] = col.y;
+                rad[2] = col.z;
             }
-            sx = sx + a.ps.cx[p];
-            sy = sy + a.ps.cy[p];
-            sz = sz + a.ps.cz[p];
-            bb += a.ps.bounces[p];
+            sx = sx + col.x;
+            sy = sy + col.y;
+            sz = sz + col.z;
+            bb += __float_as_uint(col.w);
             cc += 1u;
         }
         sum[3 * (size_t)pix] = sx;
@@ -740,10 +739,11 @@ __global__ void k_accumulate_batch(WaveArgs a) {
         uint32_t bb = f.bounces[pix], cc = f.count[pix];
         for (uint32_t s = 0; s < g.ns; ++s) {
             size_t p = (size_t)g.path_off + (size_t)s * g.npix + pix;
-            sx = sx + a.ps.cx[p];
-            sy = sy + a.ps.cy[p];
-            sz = sz + a.ps.cz[p];
-            bb += a.ps.bounces[p];
+            const float4 col = a.ps.col[p];
+            sx = sx + col.x;
+            sy = sy + col.y;
+            sz = sz + col.z;
+            bb += __float_as_uint(col.w);
             cc += 1u;
         }
         f.sum[3 * (size_t)pix] = sx;
@@ -782,9 +782,9 @@ __global__ void k_reset_ctl(Ctl *ctl, uint32_t nlaunch) {
 }
 
 /* Bytes of workspace per path in flight. */
-constexpr size_t kPathBytes = 4 * 11;                 /* path state */
+constexpr size_t kPathBytes = 3 * 16;                 /* path state records */
 constexpr size_t kQueueBytes = sizeof(Ray) + 4;       /* per queue slot */
-constexpr size_t kNeeBytes = 4 * 14 + 2 * sizeof(Ray) + 4 * sizeof(Hit) + 2 * 4 + 2 * 4 + 2 * 4;
+constexpr size_t kNeeBytes = 3 * 16 + 4 + 2 * sizeof(Ray) + 4 * sizeof(Hit) + 2 * 4 + 2 * 4 + 2 * 4;
 constexpr size_t kBatchBytes = 4 + 2 * 4 + 2 * 4;       /* path_seg, queue sids, next-event sids */
 constexpr size_t kBytesPerPath = kPathBytes + 2 * kQueueBytes + sizeof(Hit) + kNeeBytes + kBatchBytes;
 
@@ -816,21 +816,13 @@ int carve(lt_device *dev, size_t n, Workspace &w) {
         p += (n * each + 255) & ~(size_t)255;
         return (void *)r;
     };
-    w.ps.tx = (float *)take(4); w.ps.ty = (float *)take(4); w.ps.tz = (float *)take(4);
-    w.ps.cx = (float *)take(4); w.ps.cy = (float *)take(4); w.ps.cz = (float *)take(4);
-    w.ps.iori = (float *)take(4);
-    w.ps.medium = (int *)take(4);
-    w.ps.lobe = (uint32_t *)take(4);
-    w.ps.rngi = (uint32_t *)take(4);
-    w.ps.bounces = (uint32_t *)take(4);
+    w.ps.rec = (float4 *)take(2 * sizeof(float4));
+    w.ps.col = (float4 *)take(sizeof(float4));
     w.qa.rays = (Ray *)take(sizeof(Ray)); w.qa.path = (uint32_t *)take(4);
     w.qb.rays = (Ray *)take(sizeof(Ray)); w.qb.path = (uint32_t *)take(4);
     w.hits = (Hit *)take(sizeof(Hit));
-    w.nee.tx = (float *)take(4); w.nee.ty = (float *)take(4); w.nee.tz = (float *)take(4);
-    w.nee.f1x = (float *)take(4); w.nee.f1y = (float *)take(4); w.nee.f1z = (float *)take(4); w.nee.p1 = (float *)take(4);
-    w.nee.f2x = (float *)take(4); w.nee.f2y = (float *)take(4); w.nee.f2z = (float *)take(4); w.nee.p2 = (float *)take(4);
+    w.nee.rec = (float4 *)take(3 * sizeof(float4));
     w.nee.info = (uint32_t *)take(4);
-    w.nee.nl = (float *)take(4);
     w.nee_rays = (Ray *)take(2 * sizeof(Ray));
     w.nee_hits = (Hit *)take(2 * sizeof(Hit));
     w.nee_occl = (uint32_t *)take(2 * 4);
