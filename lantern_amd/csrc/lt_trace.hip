@@ -41,26 +41,26 @@ constexpr uint32_t MAX_STEPS = 1u << 22;
  * into kWorkCounters contiguous segments with a work counter each; a wave
  * starts on segment blockIdx.x % kWorkCounters (the XCD that workgroups are
  * dispatched to round-robin, so each XCD's L2 sees a contiguous part of the
- * queue), takes R * 64 consecutive rays per atomic on that segment's counter
- * and traces them 64 at a time; once its segment is used up it moves on to
- * the next one, so no XCD idles while another has rays left. One
- * device-wide counter takes on the order of 10^8 atomics per second, which
- * a launch of cheap rays fetched 64 at a time reached; eight counters and
- * R = kFeedRounds (at most a quarter of the launch's rays per wave, so that
- * every wave still fetches about four times or more) keep each counter far
- * below that. Handing out consecutive rays in fetch order keeps the rays in
- * flight neighbours in the queue (and mostly in the image), whose shared BVH
- * nodes then stay in L2: dealing the rays out statically in stripes measured
- * 60% slower on the triangle-heavy scene, and sizing each wave's fetch by
- * the time its last rounds took 4% slower (a wave that drew cheap rays then
- * takes a long run of expensive ones). */
-constexpr uint32_t kFeedRounds = 4;
+ * queue), takes a chunk of R * 64 consecutive rays per atomic on that
+ * segment's counter, and once its segment is used up moves on to the next
+ * one, so no XCD idles while another has rays left. One device-wide counter
+ * takes on the order of 10^8 atomics per second, which a launch of cheap rays
+ * fetched 64 at a time reached; eight counters and R > 1 (at most a quarter
+ * of the launch's rays per wave, so that every wave still fetches about four
+ * times or more) keep each counter far below that. R is 16 when the wave
+ * refills idle lanes from its chunk and 4 when it traces the chunk 64 rays
+ * at a time (measured best for each). Handing out consecutive rays in fetch
+ * order keeps the rays in flight neighbours in the queue (and mostly in the
+ * image), whose shared BVH nodes then stay in L2: dealing the rays out
+ * statically in stripes measured 60% slower on the triangle-heavy scene, and
+ * sizing each wave's fetch by the time its last rounds took 4% slower (a wave
+ * that drew cheap rays then takes a long run of expensive ones). */
 struct RayFeed {
     uint32_t n, chunk, seg, left, end;
-    LT_D RayFeed(uint32_t count, int block) {
+    LT_D RayFeed(uint32_t count, int block, uint32_t rounds) {
         const uint32_t nwaves = (gridDim.x * (uint32_t)block) >> 6;
         n = count;
-        chunk = 64u * max(1u, min(kFeedRounds, count / (nwaves * 64u * 4u)));
+        chunk = 64u * max(1u, min(rounds, count / (nwaves * 64u * 4u)));
         seg = blockIdx.x % (uint32_t)kWorkCounters;
         left = (uint32_t)kWorkCounters;
         end = 0;
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(DScene sc, TraceBatch batch, co
     const int lane = tid & 63;
     int *gstack = spill + (size_t)(blockIdx.x * BLOCK + tid) * (STACK_CAP - LDS_STACK);
 
-    RayFeed feed(n, BLOCK);
+    RayFeed feed(n, BLOCK, 4u);
     uint32_t base;
     uint32_t r = feed.chunk;
     while (true) {
@@ -457,14 +457,18 @@ LT_D bool trav8_visit(Trav8 &s, int *lds_node, float *lds_t, int tid, int2 *gsta
 }
 
 /* LDS8: stack entries kept in LDS per lane; WAVES: occupancy the register
- * allocator targets (0: the compiler's choice). Rays go to waves 64 at a
- * time, see next_rays. (Refilling single lanes as soon as enough of them are
- * idle, Aila & Laine 2009, measured 1.5-2x slower here: the lanes of a wave
- * then sit in unrelated parts of the tree, and the node fetches that the
- * wave's lanes used to share become separate cache lines. Ordering each
- * wave's rays by direction octant helped a scene of short random rays and
- * cost the triangle-heavy scene 5%.) */
-template <bool ANY, int BLOCK, int LDS8, int WAVES, bool BATCH>
+ * allocator targets (0: the compiler's choice); REFILL: 0 = a wave traces
+ * its chunk (RayFeed) 64 rays at a time, each round waiting for its slowest
+ * ray; otherwise lanes whose ray is finished take the next untraced ray of
+ * the chunk once at least REFILL lanes are idle (Aila & Laine 2009, with the
+ * refill confined to the wave's chunk of consecutive rays: 17% faster
+ * closest-hit traversal on the triangle-heavy scene). Refilling from the
+ * launch-wide counter instead measured 1.5-2x slower: the lanes of a wave
+ * then sit in unrelated parts of the tree, and the node fetches that they
+ * used to share become separate cache lines. Ordering each wave's rays by
+ * direction octant helped a scene of short random rays and cost the
+ * triangle-heavy scene 5%. */
+template <bool ANY, int BLOCK, int LDS8, int WAVES, int REFILL, bool BATCH>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVES : 1)))
 void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__restrict__ hits,
               uint32_t *__restrict__ occluded, const uint32_t *ignore_geom, const uint32_t *count_ptr,
@@ -475,9 +479,39 @@ void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     int2 *gstack = spill + (size_t)(blockIdx.x * BLOCK + tid) * STACK8_CAP;
-    RayFeed feed(n, BLOCK);
+    RayFeed feed(n, BLOCK, REFILL ? 16u : 4u);
     Trav8 s;
     uint32_t base;
+    if (REFILL) {
+        /* A new chunk is fetched when every lane is idle. */
+        const unsigned long long below = (1ull << lane) - 1ull;
+        uint32_t next = 0, end = 0;   /* wave-uniform: untraced rays [next, end) of the chunk */
+        bool have = false;
+        while (true) {
+            const unsigned long long idle = __ballot(!have);
+            if (idle == ~0ull && next >= end) {
+                if (!feed.next(work, lane, base)) break;
+                next = base;
+                end = min(base + feed.chunk, feed.end);
+            }
+            const uint32_t nidle = (uint32_t)__popcll(idle);
+            if (next < end && (nidle >= (uint32_t)REFILL || idle == ~0ull)) {
+                if (!have) {
+                    const uint32_t k = next + (uint32_t)__popcll(idle & below);
+                    if (k < end) {
+                        have = trav8_begin<ANY, BATCH>(s, sc, batch, rays, ignore_geom, k);
+                        if (!have) trav8_end<ANY>(s, hits, occluded);
+                    }
+                }
+                next = min(end, next + nidle);
+            }
+            if (have && trav8_visit<ANY, BLOCK, LDS8>(s, lds_node, lds_t, tid, gstack, overflow)) {
+                trav8_end<ANY>(s, hits, occluded);
+                have = false;
+            }
+        }
+        return;
+    }
     while (feed.next(work, lane, base)) {
         for (uint32_t r = 0; r < feed.chunk; r += 64u) {
             const uint32_t i = base + r + lane;
@@ -497,12 +531,14 @@ int launch_b(hipStream_t st, const DScene &sc, const TraceBatch &tb, const Ray *
     if (sc.layout == 0) {
         int2 *sp2 = (int2 *)spill;
         const int v = variant;
-#define LT_T8(B, L, W) hipLaunchKernelGGL((k_trace8<ANY, B, L, W, BATCH>), dim3(grid), dim3(B), 0, st, sc, tb, rays, hits, occl, ign, count_ptr, count, work, sp2, overflow)
-#define LT_T8B(B)                          \
-        if (v == 0) LT_T8(B, 16, 0);       \
-        else if (v == 1) LT_T8(B, 8, 0);   \
-        else if (v == 2) LT_T8(B, 8, 6);   \
-        else LT_T8(B, 8, 8);
+#define LT_T8(B, L, W, R) hipLaunchKernelGGL((k_trace8<ANY, B, L, W, R, BATCH>), dim3(grid), dim3(B), 0, st, sc, tb, rays, hits, occl, ign, count_ptr, count, work, sp2, overflow)
+#define LT_T8B(B)                             \
+        if (v == 0) LT_T8(B, 16, 0, 0);       \
+        else if (v == 1) LT_T8(B, 8, 0, 0);   \
+        else if (v == 2) LT_T8(B, 8, 6, 0);   \
+        else if (v == 3) LT_T8(B, 8, 8, 0);   \
+        else if (v == 4) LT_T8(B, 16, 0, 16); \
+        else LT_T8(B, 16, 0, 8);
         switch (block) {
         case 64: LT_T8B(64); break;
         case 128: LT_T8B(128); break;
@@ -545,7 +581,7 @@ int trace_default_block() { return 128; }
  * (binary: LDS_STACK x 4 B, 8-wide: LDS8 x 8 B per lane, of 160 KB) and by
  * 32 waves per CU. */
 int trace_grid(int num_cus, int block, int variant) {
-    const int lds8 = variant == 0 ? 16 : 8;
+    const int lds8 = (variant == 0 || variant >= 4) ? 16 : 8;
     const int lds_per_lane = std::max(LDS_STACK * 4, lds8 * 8);
     const int by_lds = (160 * 1024) / (lds_per_lane * block);
     const int by_waves = std::max(1, 32 / (block / 64));
