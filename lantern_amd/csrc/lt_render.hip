@@ -304,9 +304,11 @@ constexpr int kShadeChunk = kShadeRounds * 64;
  * radiance still to come instead of added, and parameter factors scatter
  * their gradients. */
 template <bool ADJ, bool BATCH>
-__global__ __launch_bounds__(kShadeBlock) void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits,
-                                                       Queue next, NeeRec nee, Ray *nee_rays, Hit *nee_hits,
-                                                       uint32_t *nee_ignore, uint32_t *nee_cidx, Ctl *ctl) {
+/* amdgpu_waves_per_eu(4): at most 128 VGPRs (4 waves per SIMD instead of 3),
+ * which the kernel fits without spilling; measured 3% faster (5 waves spill). */
+__global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4)))
+void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, NeeRec nee, Ray *nee_rays,
+             Hit *nee_hits, uint32_t *nee_ignore, uint32_t *nee_cidx, Ctl *ctl) {
     __shared__ uint32_t s_order_all[(kShadeBlock / 64) * kShadeChunk];
     const uint32_t count = ctl->count[b];
     const int lane = threadIdx.x & 63;
