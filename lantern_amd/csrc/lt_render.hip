@@ -60,25 +60,27 @@ struct CameraArgs {
     float cdf[33];
 };
 
-/* Path state, indexed by path id, as records: after the first bounce the
- * live paths of a queue slot range are scattered over the path ids, and a
- * lane then touches one or two cache lines for its state instead of one per
- * field. */
+/* Final color of each path, indexed by path id: (radiance, bounce count as
+ * bits), written when the path ends. The state of a live path travels with
+ * its queue slot instead (Queue::st), so that the kernels of a bounce read
+ * and write it in queue order rather than gathering it by path id. */
 struct PathState {
-    float4 *rec;              /* 2 per path: (throughput, IOR), (medium, lobe, draws consumed, -) as bits */
-    float4 *col;              /* (radiance accumulated so far, final bounce count as bits) */
+    float4 *col;
 };
-__device__ __forceinline__ void store_rec(const PathState &ps, uint32_t p, V3 T, float iori, int medium, uint32_t lobe,
-                                          uint32_t rngi) {
-    ps.rec[2 * (size_t)p] = make_float4(T.x, T.y, T.z, iori);
-    ps.rec[2 * (size_t)p + 1] = make_float4(__int_as_float(medium), __uint_as_float(lobe), __uint_as_float(rngi), 0.0f);
-}
+/* medium (>= -1) and lobe (< 16) of a live path in one word */
+__device__ __forceinline__ uint32_t pack_ml(int medium, uint32_t lobe) { return ((uint32_t)(medium + 1) << 4) | lobe; }
+__device__ __forceinline__ int ml_medium(uint32_t w) { return (int)(w >> 4) - 1; }
+__device__ __forceinline__ uint32_t ml_lobe(uint32_t w) { return w & 15u; }
 /* One queue of rays (indexed by slot) with their owners (and, in a batch,
- * the scene of each ray). */
+ * the scene of each ray) and the state of the owner path: st[2 * slot] =
+ * (throughput, IOR), st[2 * slot + 1] = (radiance so far, draws consumed as
+ * bits), ml[slot] = medium and lobe (pack_ml). */
 struct Queue {
     Ray *rays;
     uint32_t *path;
     uint32_t *sid;
+    float4 *st;
+    uint32_t *ml;
 };
 
 /* A batch wave is a list of segments: consecutive samples [s0, s0 + ns) of
@@ -221,13 +223,14 @@ __global__ void k_generate(WaveArgs a, Queue q, uint32_t n, Ctl *ctl) {
         rp[0] = r0;
         rp[1] = r1;
         q.path[p] = p;
-        store_rec(a.ps, p, v3s(1.0f), 1.0f, -1, (uint32_t)LOBE_NULL, rng.i);
+        q.st[2 * (size_t)p] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
         if (ADJ) {
             const float *rad = a.adj.sample_rad + 3 * ((size_t)a.sample_base * a.npix + p);
-            a.ps.col[p] = make_float4(rad[0], rad[1], rad[2], __uint_as_float(0u));
+            q.st[2 * (size_t)p + 1] = make_float4(rad[0], rad[1], rad[2], __uint_as_float(rng.i));
         } else {
-            a.ps.col[p] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+            q.st[2 * (size_t)p + 1] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(rng.i));
         }
+        q.ml[p] = pack_ml(-1, (uint32_t)LOBE_NULL);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->count[0] = n;
 }
@@ -279,7 +282,7 @@ template <bool BATCH>
 __device__ __forceinline__ uint32_t shade_bin(const WaveArgs &a, const Queue &cur, const Hit *hits, uint32_t q) {
     const uint32_t prim = hits[q].prim;
     if (prim == 0xFFFFFFFFu) return 0u;
-    if (__float_as_int(a.ps.rec[2 * (size_t)cur.path[q] + 1].x) >= 0) return 1u;
+    if (ml_medium(cur.ml[q]) >= 0) return 1u;
     const DScene &sc = BATCH ? a.batch.scenes[cur.sid[q]] : a.sc;
     const int bsdf = sc.mats[sc.geoms[sc.tri_geom[prim]].material].bsdf;
     return 2u + (uint32_t)min(max(bsdf, 0), kBins - 3);
@@ -308,7 +311,7 @@ template <bool ADJ, bool BATCH>
  * which the kernel fits without spilling; measured 3% faster (5 waves spill). */
 __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4)))
 void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, NeeRec nee, Ray *nee_rays,
-             Hit *nee_hits, uint32_t *nee_ignore, uint32_t *nee_cidx, Ctl *ctl) {
+             Hit *nee_hits, uint32_t *nee_ignore, uint32_t *nee_cidx, uint32_t *nee_dest, Ctl *ctl) {
     __shared__ uint32_t s_order_all[(kShadeBlock / 64) * kShadeChunk];
     const uint32_t count = ctl->count[b];
     const int lane = threadIdx.x & 63;
@@ -376,6 +379,9 @@ void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, Nee
             V3 P = v3s(0.0f), wi1 = v3s(0.0f), wi2 = v3s(0.0f);
             uint32_t ignore_geom = 0xFFFFFFFFu;
             uint32_t scene = 0;
+            V3 C = v3s(0.0f);                        /* radiance so far */
+            float4 nst0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);   /* next state of a live path */
+            uint32_t nrng = 0u, nml = 0u;
             if (valid) {
                 p = cur.path[q];
                 const PathInfo pi = path_info<BATCH>(a, p, BATCH ? a.batch.path_seg[p] : 0u);
@@ -384,10 +390,10 @@ void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, Nee
                 const DScene &sc = BATCH ? a.batch.scenes[scene] : a.sc;
                 const uint32_t pix = y * a.cam.w + x;
                 const Hit h = hits[q];
-                const float4 s0 = a.ps.rec[2 * (size_t)p];   /* throughput, IOR */
+                const float4 s0 = cur.st[2 * (size_t)q];       /* throughput, IOR */
+                const float4 s1 = cur.st[2 * (size_t)q + 1];   /* radiance, draws consumed */
                 V3 T = v3(s0.x, s0.y, s0.z);
-                const float4 col = a.ps.col[p];
-                V3 C = v3(col.x, col.y, col.z);
+                C = v3(s1.x, s1.y, s1.z);
                 bool dead = false;
                 uint32_t nee_info = 0;
                 if (h.prim == 0xFFFFFFFFu) {
@@ -397,15 +403,15 @@ void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, Nee
                     else C = C + T * v3(sc.bg[0], sc.bg[1], sc.bg[2]);
                     dead = true;
                 } else {
-                    const float4 s1 = a.ps.rec[2 * (size_t)p + 1];   /* medium, lobe, draws */
+                    const uint32_t ml = cur.ml[q];
                     PathRng rng;
-                    rng.init(x, y, a.stream + scene, a.seed, pi.frame, __float_as_uint(s1.z));
+                    rng.init(x, y, a.stream + scene, a.seed, pi.frame, __float_as_uint(s1.w));
                     const float4 *rp = (const float4 *)(cur.rays + q);
                     const float4 ra = rp[0], rb = rp[1];
                     Interaction it;
                     it.iori = s0.w;
-                    it.lobe = (int)__float_as_uint(s1.y);
-                    int medium = __float_as_int(s1.x);
+                    it.lobe = (int)ml_lobe(ml);
+                    int medium = ml_medium(ml);
                     V3 origin = v3(ra.x, ra.y, ra.z);
                     V3 dir = normalize(v3(rb.x, rb.y, rb.z));
                     bool surface = true;
@@ -574,12 +580,16 @@ void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, Nee
                         else T = T * (1.0f / pr);
                     }
                     /* the rest of the state matters only to a path that goes on */
-                    if (!dead && b + 1 < a.max_depth) store_rec(a.ps, p, T, it.iori, medium, (uint32_t)it.lobe, rng.i);
+                    if (!dead && b + 1 < a.max_depth) {
+                        nst0 = make_float4(T.x, T.y, T.z, it.iori);
+                        nrng = rng.i;
+                        nml = pack_ml(medium, (uint32_t)it.lobe);
+                    }
                 }
                 nee.info[q] = nee_info;
                 alive = !dead && (b + 1 < a.max_depth);
-                /* the bounce count is final once the path ends */
-                a.ps.col[p] = make_float4(C.x, C.y, C.z, __uint_as_float(alive ? 0u : (dead ? b : b + 1)));
+                /* an ending path writes its color and final bounce count */
+                if (!alive) a.ps.col[p] = make_float4(C.x, C.y, C.z, __uint_as_float(dead ? b : b + 1));
             }
             /* wave-level compaction of the survivors and of the next-event rays */
             const unsigned long long below = lanes_below;
@@ -600,7 +610,13 @@ void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, Nee
                 next.path[s] = p;
                 if (BATCH) next.sid[s] = scene;
                 store_ray(next.rays + s, new_o, new_d, 0.001f, inf);
+                next.st[2 * (size_t)s] = nst0;
+                next.st[2 * (size_t)s + 1] = make_float4(C.x, C.y, C.z, __uint_as_float(nrng));
+                next.ml[s] = nml;
             }
+            /* where k_resolve adds this slot's direct light: the path's next
+             * slot, or (bit 31) its final color */
+            if (nee1 || nee2) nee_dest[q] = alive ? slot0 + (uint32_t)__popcll(mask & below) : (0x80000000u | p);
             if (nee1 || nee2) {
                 uint32_t c = nslot0 + (uint32_t)(__popcll(nmask1 & below) + __popcll(nmask2 & below));
                 if (nee1) {
@@ -628,9 +644,10 @@ void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, Nee
 /* The casts of AreaLight::SampleLi / PdfLi resolved (area_light.cpp:58-71,
  * 100-109) and the MIS sums of EstimateDirect (integrator.cpp:287-314). */
 template <bool ADJ, bool BATCH>
-__global__ __launch_bounds__(256) void k_resolve(WaveArgs a, uint32_t b, Queue cur, NeeRec nee, const Ray *nee_rays,
-                                                 const Hit *nee_hits, const Hit *nee_trace_hits,
-                                                 const uint32_t *nee_occl, const uint32_t *nee_cidx, Ctl *ctl) {
+__global__ __launch_bounds__(256) void k_resolve(WaveArgs a, uint32_t b, Queue cur, Queue next, NeeRec nee,
+                                                 const Ray *nee_rays, const Hit *nee_hits,
+                                                 const Hit *nee_trace_hits, const uint32_t *nee_occl,
+                                                 const uint32_t *nee_cidx, const uint32_t *nee_dest, Ctl *ctl) {
     const uint32_t count = ctl->count[b];
     
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
@@ -684,16 +701,17 @@ __global__ __launch_bounds__(256) void k_resolve(WaveArgs a, uint32_t b, Queue c
                 }
             }
         }
-        const uint32_t p = cur.path[q];
         const float4 r0 = nee.rec[3 * (size_t)q];
         V3 T = v3(r0.x, r0.y, r0.z);
         V3 add = T * (r0.w * direct);
         if (ADJ) add = -add;
-        float4 col = a.ps.col[p];
+        const uint32_t dest = nee_dest[q];
+        float4 *cp = (dest & 0x80000000u) ? a.ps.col + (dest & 0x7FFFFFFFu) : next.st + 2 * (size_t)dest + 1;
+        float4 col = *cp;
         col.x = col.x + add.x;
         col.y = col.y + add.y;
         col.z = col.z + add.z;
-        a.ps.col[p] = col;
+        *cp = col;
     }
 }
 
@@ -784,9 +802,9 @@ __global__ void k_reset_ctl(Ctl *ctl, uint32_t nlaunch) {
 }
 
 /* Bytes of workspace per path in flight. */
-constexpr size_t kPathBytes = 3 * 16;                 /* path state records */
-constexpr size_t kQueueBytes = sizeof(Ray) + 4;       /* per queue slot */
-constexpr size_t kNeeBytes = 3 * 16 + 4 + 2 * sizeof(Ray) + 4 * sizeof(Hit) + 2 * 4 + 2 * 4 + 2 * 4;
+constexpr size_t kPathBytes = 16;                     /* final color */
+constexpr size_t kQueueBytes = sizeof(Ray) + 4 + 2 * 16 + 4;   /* per queue slot: ray, owner, state */
+constexpr size_t kNeeBytes = 3 * 16 + 4 + 2 * sizeof(Ray) + 4 * sizeof(Hit) + 2 * 4 + 2 * 4 + 2 * 4 + 4;
 constexpr size_t kBatchBytes = 4 + 2 * 4 + 2 * 4;       /* path_seg, queue sids, next-event sids */
 constexpr size_t kBytesPerPath = kPathBytes + 2 * kQueueBytes + sizeof(Hit) + kNeeBytes + kBatchBytes;
 
@@ -801,6 +819,7 @@ struct Workspace {
     uint32_t *nee_ignore;    /* per compacted next-event ray: the light's geomID */
     Hit *nee_trace_hits;     /* per compacted next-event ray (closest-hit mode) */
     uint32_t *nee_cidx;      /* per (slot, ray): index of its compacted ray */
+    uint32_t *nee_dest;      /* per slot: where its direct light goes (next slot, or bit 31 | path id) */
     uint32_t *path_seg;      /* batch: per path, its segment */
     uint32_t *nee_sid;       /* batch: per compacted next-event ray, its scene */
 };
@@ -818,10 +837,11 @@ int carve(lt_device *dev, size_t n, Workspace &w) {
         p += (n * each + 255) & ~(size_t)255;
         return (void *)r;
     };
-    w.ps.rec = (float4 *)take(2 * sizeof(float4));
     w.ps.col = (float4 *)take(sizeof(float4));
     w.qa.rays = (Ray *)take(sizeof(Ray)); w.qa.path = (uint32_t *)take(4);
     w.qb.rays = (Ray *)take(sizeof(Ray)); w.qb.path = (uint32_t *)take(4);
+    w.qa.st = (float4 *)take(2 * sizeof(float4)); w.qa.ml = (uint32_t *)take(4);
+    w.qb.st = (float4 *)take(2 * sizeof(float4)); w.qb.ml = (uint32_t *)take(4);
     w.hits = (Hit *)take(sizeof(Hit));
     w.nee.rec = (float4 *)take(3 * sizeof(float4));
     w.nee.info = (uint32_t *)take(4);
@@ -831,6 +851,7 @@ int carve(lt_device *dev, size_t n, Workspace &w) {
     w.nee_ignore = (uint32_t *)take(2 * 4);
     w.nee_trace_hits = (Hit *)take(2 * sizeof(Hit));
     w.nee_cidx = (uint32_t *)take(2 * 4);
+    w.nee_dest = (uint32_t *)take(4);
     w.qa.sid = (uint32_t *)take(4);
     w.qb.sid = (uint32_t *)take(4);
     w.path_seg = (uint32_t *)take(4);
@@ -993,7 +1014,7 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
                 tm.begin(T_SHADE);
                 hipLaunchKernelGGL((k_shade<ADJ, BATCH>), dim3(sgrid), dim3(kShadeBlock), 0, st, a, b, cur,
                                    (const Hit *)w.hits, nxt, w.nee, w.nee_rays, w.nee_hits, w.nee_ignore, w.nee_cidx,
-                                   ctl);
+                                   w.nee_dest, ctl);
                 tm.end(T_SHADE);
                 LT_HIP(hipGetLastError());
                 /* next-event rays: occlusion up to the light hit (small lights) or
@@ -1011,9 +1032,10 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
                 tm.end(T_SHADOW);
                 ++sh_launches;
                 tm.begin(T_RESOLVE);
-                hipLaunchKernelGGL((k_resolve<ADJ, BATCH>), dim3(sgrid), dim3(256), 0, st, a, b, cur, w.nee,
+                hipLaunchKernelGGL((k_resolve<ADJ, BATCH>), dim3(sgrid), dim3(256), 0, st, a, b, cur, nxt, w.nee,
                                    (const Ray *)w.nee_rays, (const Hit *)w.nee_hits, (const Hit *)w.nee_trace_hits,
-                                   (const uint32_t *)w.nee_occl, (const uint32_t *)w.nee_cidx, ctl);
+                                   (const uint32_t *)w.nee_occl, (const uint32_t *)w.nee_cidx,
+                                   (const uint32_t *)w.nee_dest, ctl);
                 tm.end(T_RESOLVE);
                 LT_HIP(hipGetLastError());
                 std::swap(cur, nxt);
