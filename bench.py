@@ -31,7 +31,7 @@ RAY_BYTES = 48
 # commands and raw figures are in profiles/k_trace8_triangle_heavy.md.
 # Keyed by (workload, layout, width, height, spp).
 TRAFFIC = {
-    ("triangle_heavy", 0, 1280, 720, 64): round((559090.4375 * 2.0 + 193402.275 * 1.0) * 1024),
+    ("triangle_heavy", 0, 1280, 720, 64): round((637495.03125 * 2.0 + 279850.478125 * 1.0) * 1024),
 }
 
 WORKLOADS = {
