@@ -265,6 +265,11 @@ struct Trav8 {
     const float4 *__restrict__ wn;
     const float4 *__restrict__ wt;
     int sp, node;
+    /* after a node's box tests: its hit leaf triangles still to test (bit k =
+     * triangle tbase + k) and its nearest hit internal child (not pushed) */
+    uint32_t tmask, tbase;
+    int near_node;
+    float near_t;
 };
 
 /* Loads ray i and sets up its traversal; returns false if the ray needs no
@@ -302,6 +307,7 @@ LT_D bool trav8_begin(Trav8 &s, const DScene &sc, const TraceBatch &batch, const
     s.sp = 0;
     s.node = 0;
     s.steps = 0;
+    s.tmask = 0u;
     return (s.r.tfar >= 0.0f) && ntris != 0;
 }
 
@@ -319,11 +325,14 @@ LT_D void trav8_end(const Trav8 &s, Hit *__restrict__ hits, uint32_t *__restrict
     }
 }
 
-/* One node visit: the eight child boxes, then the triangles of the hit leaf
- * children, then the next node (the nearest hit internal child, or a pop).
- * Returns true when the ray is finished. */
+/* A node visit in three steps. trav8_boxes tests the eight child boxes of
+ * s.node, pushes every hit internal child but the nearest and records the
+ * hit leaves' triangles in s.tmask; trav8_tri tests one of those triangles;
+ * trav8_next, once they are all tested, moves to the nearest child (unless
+ * it lies beyond the shrunk tfar) or pops. Each returns true when the ray is
+ * finished. */
 template <bool ANY, int BLOCK, int LDS8>
-LT_D bool trav8_visit(Trav8 &s, int *lds_node, float *lds_t, int tid, int2 *gstack, uint32_t *overflow) {
+LT_D bool trav8_boxes(Trav8 &s, int *lds_node, float *lds_t, int tid, int2 *gstack, uint32_t *overflow) {
     /* at most 7 pushes per node: stop before the stack could overflow */
     if (++s.steps > MAX_STEPS || s.sp > STACK8_CAP - 7) {
         s.best_prim = 0xFFFFFFFFu;
@@ -407,33 +416,45 @@ LT_D bool trav8_visit(Trav8 &s, int *lds_node, float *lds_t, int tid, int2 *gsta
             }
         }
     }
-    /* Leaf triangles after the box tests, one per lane per iteration: lanes
-     * that hit leaves in different slots test them together, rather than one
-     * slot at a time with the other lanes idle. */
-    while (tmask) {
-        const uint32_t k = (uint32_t)__builtin_ctz(tmask);
-        tmask &= tmask - 1u;
-        const float4 *tp = s.wt + 3 * (size_t)(tbase + k);
-        const float4 a = tp[0], b = tp[1], c = tp[2];
-        float T, U, V, den;
-        if ((!ANY || __float_as_uint(c.w) != s.ignore) &&
-            moeller_raw(r.o, r.d, r.tnear, r.tfar, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), T, U, V,
-                        den, (__float_as_uint(b.w) & 2u) != 0u)) {
-            s.best_prim = __float_as_uint(a.w);
-            if (ANY) return true;
-            s.r.tfar = T / den;
-            s.best_t = s.r.tfar;
-            s.best_U = U;
-            s.best_V = V;
-            s.best_den = den;
-        }
+    s.sp = sp;
+    s.tmask = tmask;
+    s.tbase = tbase;
+    s.near_node = near_node;
+    s.near_t = near_t;
+    return false;
+}
+
+/* Tests triangle s.tbase + lowest bit of s.tmask (which must be non-zero). */
+template <bool ANY>
+LT_D bool trav8_tri(Trav8 &s) {
+    const uint32_t k = (uint32_t)__builtin_ctz(s.tmask);
+    s.tmask &= s.tmask - 1u;
+    const float4 *tp = s.wt + 3 * (size_t)(s.tbase + k);
+    const float4 a = tp[0], b = tp[1], c = tp[2];
+    const TravRay &r = s.r;
+    float T, U, V, den;
+    if ((!ANY || __float_as_uint(c.w) != s.ignore) &&
+        moeller_raw(r.o, r.d, r.tnear, r.tfar, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), T, U, V, den,
+                    (__float_as_uint(b.w) & 2u) != 0u)) {
+        s.best_prim = __float_as_uint(a.w);
+        if (ANY) return true;
+        s.r.tfar = T / den;
+        s.best_t = s.r.tfar;
+        s.best_U = U;
+        s.best_V = V;
+        s.best_den = den;
     }
+    return false;
+}
+
+template <int BLOCK, int LDS8>
+LT_D bool trav8_next(Trav8 &s, int *lds_node, float *lds_t, int tid, int2 *gstack) {
     const float limit = s.r.tfar * 1.0000005f;
-    if (near_node >= 0 && near_t <= limit) {
-        s.node = near_node;
-        s.sp = sp;
+    if (s.near_node >= 0 && s.near_t <= limit) {
+        s.node = s.near_node;
         return false;
     }
+    int sp = s.sp;
     while (sp > 0) {
         --sp;
         int nd;
@@ -456,6 +477,18 @@ LT_D bool trav8_visit(Trav8 &s, int *lds_node, float *lds_t, int tid, int2 *gsta
     return true;
 }
 
+/* One whole node visit (box tests, then every hit leaf triangle, then the
+ * next node). The triangles are tested after the box tests, one per lane per
+ * iteration: lanes that hit leaves in different slots test them together,
+ * rather than one slot at a time with the other lanes idle. */
+template <bool ANY, int BLOCK, int LDS8>
+LT_D bool trav8_visit(Trav8 &s, int *lds_node, float *lds_t, int tid, int2 *gstack, uint32_t *overflow) {
+    if (trav8_boxes<ANY, BLOCK, LDS8>(s, lds_node, lds_t, tid, gstack, overflow)) return true;
+    while (s.tmask)
+        if (trav8_tri<ANY>(s)) return true;
+    return trav8_next<BLOCK, LDS8>(s, lds_node, lds_t, tid, gstack);
+}
+
 /* LDS8: stack entries kept in LDS per lane; WAVES: occupancy the register
  * allocator targets (0: the compiler's choice); REFILL: 0 = a wave traces
  * its chunk (RayFeed) 64 rays at a time, each round waiting for its slowest
@@ -467,7 +500,9 @@ LT_D bool trav8_visit(Trav8 &s, int *lds_node, float *lds_t, int tid, int2 *gsta
  * then sit in unrelated parts of the tree, and the node fetches that they
  * used to share become separate cache lines. Ordering each wave's rays by
  * direction octant helped a scene of short random rays and cost the
- * triangle-heavy scene 5%. */
+ * triangle-heavy scene 5%; deferring triangle tests until half the lanes
+ * have some (speculative traversal, so that box and triangle steps run with
+ * fuller waves) cost it 14%, as tfar then shrinks later. */
 template <bool ANY, int BLOCK, int LDS8, int WAVES, int REFILL, bool BATCH>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVES : 1)))
 void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__restrict__ hits,
