@@ -8,20 +8,20 @@
  * (kernels/geometry/intersector_epilog.h:241-305).
  *
  * Design for CDNA4:
- *  - one ray per lane, 64-lane waves fetch 64 rays at a time from an atomic
- *    work counter (persistent threads, Aila & Laine 2009), so waves that drew
- *    short rays go back for more instead of idling;
- *  - rays (32 B) and hits (16 B) are read/written as float4 pairs, so a wave
- *    touches 2 KB of contiguous HBM per batch;
+ *  - one ray per lane in persistent blocks; waves take chunks of consecutive
+ *    rays from per-XCD work counters (RayFeed) and, in the default kernel,
+ *    refill idle lanes from their chunk (Aila & Laine 2009), so long rays
+ *    hold few idle lanes and the rays in flight stay neighbours;
+ *  - rays (32 B) and hits (16 B) are read/written as float4 pairs;
  *  - the traversal stack lives in LDS, laid out [depth][lane] so the 64 lanes
  *    of a push hit 64 consecutive dwords (no bank conflicts); entries beyond
- *    LDS_STACK spill to a per-thread slice of global memory (the LBVH is at
- *    most 64 levels deep, so the total capacity STACK_CAP cannot overflow);
+ *    the LDS part spill to a per-thread slice of global memory;
  *  - a malformed hierarchy cannot hang or overrun: traversal steps, stack
  *    depth and leaf walks are bounded, and a ray that hits a bound reports
- *    t = -1 with no primitive;
- *  - box tests are (lo - o) * (1/d) with the far distance scaled by
- *    (1 + 2^-21), so rounding never culls a box that contains a hit; the
+ *    t = -1 with no primitive (and is counted in the diagnostics word);
+ *  - box tests are conservative (binary layout: (lo - o) * (1/d) with the far
+ *    distance scaled by 1 + 2^-21; 8-wide layout: per-axis rounding bounds,
+ *    see trav8_boxes), so rounding never culls a box that contains a hit; the
  *    closest hit is decided only by the exact triangle test, which makes the
  *    result independent of the BVH except for exact t ties.
  */

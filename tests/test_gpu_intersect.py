@@ -203,3 +203,18 @@ def test_quad_meshes_match_brute_force(gpu_device, layout):
     rays = _camera_rays(d, 20000, 8)
     _compare(gs, os_, rays)
     gs.close()
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_ray_counts_across_work_segments(gpu_device, layout):
+    """The rays of a launch are split into 8 segments with a work counter each
+    and handed out in chunks; counts below 8, at chunk and wave boundaries and
+    in between must each trace every ray exactly once."""
+    d = scenes.nine_sphere(64, 64, 1)
+    gs = lantern_amd.Scene(gpu_device, d, layout=layout)
+    os_ = OracleScene(d)
+    rays_all = _random_rays(70000, 31)
+    for n in (1, 2, 7, 8, 9, 63, 64, 65, 1023, 1025, 8191, 65537, 70000):
+        _compare(gs, os_, rays_all[:n], brute=False)
+    os_.close()
+    gs.close()
