@@ -65,6 +65,7 @@ def main():
     ap.add_argument("--layout", type=int, default=0, help="BVH layout: 0 8-wide compressed, 1 binary")
     ap.add_argument("--builder", type=int, default=0, help="BVH builder: 0 PLOC, 1 linear BVH")
     ap.add_argument("--max-leaf", type=int, default=4, help="BVH leaf size limit (1-4)")
+    ap.add_argument("--wide-collapse", type=int, default=0, help="8-wide collapse: 0 SAH-optimal, 1 greedy by area")
     ap.add_argument("--variant", type=int, default=0, help="traversal kernel variant (0 automatic, 1..4)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -108,7 +109,8 @@ def main():
     params["trace_variant"] = args.variant
     params["use_graph"] = 1 if args.graph == 1 else 0
     with lantern_amd.Device(0) as dev:
-        scene = lantern_amd.Scene(dev, desc, max_leaf_size=args.max_leaf, layout=args.layout, builder=args.builder)
+        scene = lantern_amd.Scene(dev, desc, max_leaf_size=args.max_leaf, layout=args.layout, builder=args.builder,
+                                  wide_collapse=args.wide_collapse)
         info = scene.info()
         film = lantern_amd.Film(dev, args.width, args.height)
         for _ in range(args.warmup):

@@ -51,13 +51,13 @@ class Device:
 class Scene:
     """lt_scene built from a SceneDesc (Scene::LoadSceneFromJSON + rtcCommitScene)."""
 
-    def __init__(self, device, desc=None, max_leaf_size=4, layout=0, builder=0):
+    def __init__(self, device, desc=None, max_leaf_size=4, layout=0, builder=0, wide_collapse=0):
         self.dev = device
         self.h = C.c_void_p()
         check(lib().lt_scene_create(device.h, C.byref(self.h)), "lt_scene_create")
         if desc is not None:
             self.load(desc)
-            self.commit(max_leaf_size, layout, builder)
+            self.commit(max_leaf_size, layout, builder, wide_collapse)
 
     def load(self, d):
         L = lib()
@@ -103,11 +103,12 @@ class Scene:
                       "lt_scene_set_emission")
         check(L.lt_scene_set_background(self.h, (C.c_float * 3)(*d.background)), "lt_scene_set_background")
 
-    def commit(self, max_leaf_size=4, layout=0, builder=0):
+    def commit(self, max_leaf_size=4, layout=0, builder=0, wide_collapse=0):
         bp = _capi.lt_build_params()
         bp.max_leaf_size = max_leaf_size
         bp.layout = layout
         bp.builder = builder
+        bp.wide_collapse = wide_collapse
         check(lib().lt_scene_commit(self.h, C.byref(bp)), "lt_scene_commit")
 
     def info(self):

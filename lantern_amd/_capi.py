@@ -33,7 +33,7 @@ class lt_medium(C.Structure):
 
 class lt_build_params(C.Structure):
     _fields_ = [("max_leaf_size", C.c_uint32), ("layout", C.c_uint32), ("builder", C.c_uint32),
-                ("reserved", C.c_uint32 * 5)]
+                ("wide_collapse", C.c_uint32), ("reserved", C.c_uint32 * 4)]
 
 
 class lt_scene_info(C.Structure):

@@ -524,6 +524,8 @@ int lt_scene_commit(lt_scene *s, const lt_build_params *params) {
     LT_CHECK(layout <= 1, "lt_scene_commit: layout must be 0 (wide) or 1 (binary)");
     uint32_t builder = params ? params->builder : 0;
     LT_CHECK(builder <= 1, "lt_scene_commit: builder must be 0 (PLOC) or 1 (LBVH)");
+    uint32_t wide_collapse = params ? params->wide_collapse : 0;
+    LT_CHECK(wide_collapse <= 1, "lt_scene_commit: wide_collapse must be 0 (SAH-optimal) or 1 (greedy)");
     s->layout = layout;
     s->build = params ? *params : lt_build_params{};
     s->build.max_leaf_size = max_leaf;
@@ -563,8 +565,8 @@ int lt_scene_commit(lt_scene *s, const lt_build_params *params) {
     auto t0 = std::chrono::steady_clock::now();
     uint32_t ntris = (uint32_t)s->tri_geom.size();
     if ((rc = build_bvh(st, s->d_pos.as<float>(), s->d_vidx.as<uint32_t>(), s->d_tri_geom.as<uint32_t>(),
-                        s->d_tri_flags.as<uint32_t>(), ntris, max_leaf,
-                        builder, s->d_nodes, s->d_tris, s->d_wnodes, s->d_wtris, s->bvh)))
+                        s->d_tri_flags.as<uint32_t>(), ntris, max_leaf, builder, wide_collapse, s->d_nodes,
+                        s->d_tris, s->d_wnodes, s->d_wtris, s->bvh)))
         return rc;
     LT_HIP(hipStreamSynchronize(st));
     s->build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -154,9 +154,23 @@ struct BuildArrays {
     float4 *bhi;      /* xyz hi, w = surface area */
     int *done;        /* pass number (1-based) in which the node completed */
     int *collapse;
+    /* SAH-optimal 8-wide collapse (null: greedy): per internal node n and
+     * k = 1..8, dp[8n + k-1] = the least cost of representing n's subtree as
+     * at most k subtrees (each a wide node or a leaf), and dpk[8n + k-1] how:
+     * for k >= 2, 0 = as at most k-1, d > 0 = d for the left child and k-d
+     * for the right; dpk[8n] = the split of 8 slots when n is a wide node */
+    float *dp;
+    uint8_t *dpk;
     int n;
     int max_leaf;
 };
+
+/* Cost model of the wide collapse (Ylitie, Karras & Laine, HPG 2017):
+ * surface area x (kWideNodeCost per wide node, kWideTriCost per leaf
+ * triangle); a node visit tests eight boxes, a triangle test costs about a
+ * quarter of that in the traversal kernel. */
+constexpr float kWideNodeCost = 1.0f;
+constexpr float kWideTriCost = 0.3f;
 
 LT_D float area3(const float lo[3], const float hi[3]) {
     float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
@@ -204,6 +218,52 @@ __global__ void k_refit_pass(BuildArrays a, int pass) {
     a.blo[i] = make_float4(lo[0], lo[1], lo[2], col ? leaf : split);
     a.bhi[i] = make_float4(hi[0], hi[1], hi[2], ar);
     a.collapse[i] = col;
+    if (a.dp && !col) {
+        /* the children's costs as 1..8 subtrees (a leafy child is one leaf) */
+        float cl[8], cr[8];
+        const int cc[2] = {c.x, c.y};
+        const float ac[2] = {a0, a1};
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            float *t = side ? cr : cl;
+            const int ch = cc[side];
+            if (ch < 0 || a.collapse[ch]) {
+                const int2 rr = ch < 0 ? make_int2(~ch, ~ch) : a.range[ch];
+                const float leaf_cost = ac[side] * kWideTriCost * (float)(rr.y - rr.x + 1);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) t[k] = leaf_cost;
+            } else {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) t[k] = a.dp[8 * (size_t)ch + k];
+            }
+        }
+        float dist[9];
+        uint8_t arg[9];
+#pragma unroll
+        for (int j = 2; j <= 8; ++j) {
+            float best = __builtin_huge_valf();
+            uint8_t bk = 1;
+            for (int k = 1; k < j; ++k) {
+                const float v = cl[k - 1] + cr[j - k - 1];
+                if (v < best) {
+                    best = v;
+                    bk = (uint8_t)k;
+                }
+            }
+            dist[j] = best;
+            arg[j] = bk;
+        }
+        float prev = ar * kWideNodeCost + dist[8];
+        a.dp[8 * (size_t)i] = prev;
+        a.dpk[8 * (size_t)i] = arg[8];
+#pragma unroll
+        for (int j = 2; j <= 8; ++j) {
+            const bool split = dist[j] < prev;
+            prev = split ? dist[j] : prev;
+            a.dp[8 * (size_t)i + j - 1] = prev;
+            a.dpk[8 * (size_t)i + j - 1] = split ? arg[j] : (uint8_t)0;
+        }
+    }
     a.done[i] = pass + 1;
 }
 
@@ -372,8 +432,46 @@ __global__ void k_wide_pass(WideArgs w) {
         leafy[j] = is_leafy(a, ids[j]);
         area[j] = ids[j] >= 0 ? a.bhi[ids[j]].w : 0.0f;
     }
+    if (a.dp) {
+        /* the SAH-optimal cut: the children follow the recorded decisions,
+         * starting from b's split of eight slots */
+        int stk_node[16], stk_slots[16];
+        int top = 0;
+        const int d8 = (int)a.dpk[8 * (size_t)b];
+        stk_node[top] = c0.y;
+        stk_slots[top++] = 8 - d8;
+        stk_node[top] = c0.x;
+        stk_slots[top++] = d8;
+        n = 0;
+        while (top > 0 && n < 8) {
+            --top;
+            const int c = stk_node[top];
+            int k = stk_slots[top];
+            if (is_leafy(a, c) || k <= 1) {
+                ids[n] = c;
+                leafy[n] = is_leafy(a, c);
+                area[n] = c >= 0 ? a.bhi[c].w : 0.0f;
+                ++n;
+                continue;
+            }
+            while (k > 1 && a.dpk[8 * (size_t)c + k - 1] == 0) --k;
+            if (k <= 1) {
+                ids[n] = c;
+                leafy[n] = false;
+                area[n] = a.bhi[c].w;
+                ++n;
+                continue;
+            }
+            const int d = (int)a.dpk[8 * (size_t)c + k - 1];
+            const int2 cc = a.child[c];
+            stk_node[top] = cc.y;
+            stk_slots[top++] = k - d;
+            stk_node[top] = cc.x;
+            stk_slots[top++] = d;
+        }
+    }
     /* greedy expansion: open the internal child with the largest surface area */
-    while (n < 8) {
+    while (!a.dp && n < 8) {
         int best = -1;
         float ba = -1.0f;
         for (int j = 0; j < n; ++j)
@@ -719,8 +817,8 @@ static int ploc_build(hipStream_t st, const float *d_pos, const uint32_t *d_vidx
 
 int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const uint32_t *d_tri_geom,
               const uint32_t *d_tri_flags, uint32_t n,
-              uint32_t max_leaf, uint32_t builder, DevBuf &nodes, DevBuf &tris, DevBuf &wnodes, DevBuf &wtris,
-              BvhStats &stats) {
+              uint32_t max_leaf, uint32_t builder, uint32_t wide_collapse, DevBuf &nodes, DevBuf &tris,
+              DevBuf &wnodes, DevBuf &wtris, BvhStats &stats) {
     stats = BvhStats();
     if (n == 0) return LT_OK;
     if (max_leaf > 4) return set_error(LT_E_INVALID, "BVH build: max_leaf_size must be <= 4 for the wide layout");
@@ -774,7 +872,7 @@ int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const 
         return LT_OK;
     }
 
-    DevBuf child, range, pint, pleaf, blo, bhi, done, collapse, emit, dstats, dfs_keys;
+    DevBuf child, range, pint, pleaf, blo, bhi, done, collapse, emit, dstats, dfs_keys, dp, dpk;
     size_t ni = n - 1;
     if (builder == 0) {
         int iters = 0;
@@ -811,6 +909,13 @@ int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const 
     a.bhi = (float4 *)bhi.p;
     a.done = (int *)done.p;
     a.collapse = (int *)collapse.p;
+    a.dp = nullptr;
+    a.dpk = nullptr;
+    if (wide_collapse == 0) {
+        if ((rc = dp.alloc(sizeof(float) * 8 * ni)) || (rc = dpk.alloc(8 * ni))) return rc;
+        a.dp = (float *)dp.p;
+        a.dpk = (uint8_t *)dpk.p;
+    }
     a.n = (int)n;
     a.max_leaf = (int)max_leaf;
     /* Bottom-up passes until the root completes (one tree level per pass). */

@@ -59,8 +59,8 @@ struct BvhStats {
  * collapse (wnodes, wtris). */
 int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const uint32_t *d_tri_geom,
               const uint32_t *d_tri_flags, uint32_t n,
-              uint32_t max_leaf, uint32_t builder, DevBuf &nodes, DevBuf &tris, DevBuf &wnodes, DevBuf &wtris,
-              BvhStats &stats);
+              uint32_t max_leaf, uint32_t builder, uint32_t wide_collapse, DevBuf &nodes, DevBuf &tris,
+              DevBuf &wnodes, DevBuf &wtris, BvhStats &stats);
 
 /* Traversal launches (lt_trace.hip). count_ptr (device) overrides count when
  * non-null; work is a device counter that must be zero at launch. */

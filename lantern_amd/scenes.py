@@ -298,9 +298,11 @@ def fbm_grid(res, seed, octaves=5, freq=3.0):
 def volume_box(width=512, height=512, spp=256, seed=DEFAULT_SEED, res=128, max_depth=32, sigma_scale=5.0):
     """Heterogeneous smoke in a glass box: a 128^3 grid of clamp(fBm_5, 0, 1)
     densities (sigma_t = 5 density, single-scatter albedo 0.9, Henyey-Greenstein
-    g = 0.3) filling a 2x2x2 CreateBox of IOR 1.5 standing on a Lambertian
-    floor, one 2x2 quad light (radiance 20) at y = 5, dim environment 0.2.
-    Free flights by delta tracking."""
+    g = 0.3) filling a 2x2x2 CreateBox of IOR 1.5 standing 0.01 above a
+    Lambertian floor (a box resting on the floor would make its bottom face
+    and the floor coplanar, where the nearest hit depends on the order the
+    two triangles are tested in), one 2x2 quad light (radiance 20) at y = 5,
+    dim environment 0.2. Free flights by delta tracking."""
     d = SceneDesc()
     dens = fbm_grid(res, seed)
     gid = len(d.grids)
@@ -314,6 +316,7 @@ def volume_box(width=512, height=512, spp=256, seed=DEFAULT_SEED, res=128, max_d
     bp2, bn2 = transform(bp, bn, translate(0.0, 1.0, 0.0))
     d.add_mesh(bp2, bi, bn2, buv, glass)
     gp, gn, guv, gi = grid(20.0, 20.0, 2, 2)
+    gp, gn = transform(gp, gn, translate(0.0, -0.01, 0.0))
     d.add_mesh(gp, gi, gn, guv, floor)
     lp, ln, luv, li, power = quad_light(2.0, 5.0, 20.0)
     d.add_mesh(lp, li, ln, luv, light_mat, emission=((1.0, 1.0, 1.0), power))
@@ -323,7 +326,7 @@ def volume_box(width=512, height=512, spp=256, seed=DEFAULT_SEED, res=128, max_d
     d.params = dict(spp=spp, max_depth=max_depth, rr_start=4, seed=seed)
     d.meta = dict(workload="volume_box", seed=seed, grid_res=res, fbm_octaves=5, fbm_freq=3.0,
                   sigma_t_scale=sigma_scale, single_scatter_albedo=0.9, hg_g=0.3, box_ior=1.5, light_height=5.0,
-                  light_le=20.0, background=0.2, floor_albedo=0.8, camera_phi=1.2, camera_theta=0.6,
+                  light_le=20.0, background=0.2, floor_albedo=0.8, floor_y=-0.01, camera_phi=1.2, camera_theta=0.6,
                   camera_radius=7.0)
     return d
 

@@ -218,3 +218,30 @@ def test_ray_counts_across_work_segments(gpu_device, layout):
         _compare(gs, os_, rays_all[:n], brute=False)
     os_.close()
     gs.close()
+
+
+@pytest.mark.parametrize("wide_collapse", [pytest.param(0, id="sah"), pytest.param(1, id="greedy")])
+def test_wide_collapse_modes_match_oracle(gpu_device, wide_collapse):
+    """Both ways of cutting the binary tree into 8-wide nodes (the SAH-optimal
+    cut and the greedy largest-area expansion) keep every triangle reachable:
+    hits match the oracle on random and camera rays of the triangle-heavy
+    scene and of tiny scenes."""
+    d = scenes.triangle_heavy(96, 54, 1)
+    gs = lantern_amd.Scene(gpu_device, d, wide_collapse=wide_collapse)
+    assert gs.info()["wide_nodes"] > 1
+    os_ = OracleScene(d)
+    rays = np.concatenate([_random_rays(20000, 41, -3, 3), _camera_rays(d, 20000, 42)])
+    _compare(gs, os_, rays, brute=False)
+    os_.close()
+    gs.close()
+    for ntris in (2, 5, 9, 17, 33, 65):
+        rng = np.random.default_rng(ntris)
+        tris = rng.uniform(-1, 1, (ntris, 3, 3)).astype(np.float32)
+        desc = SceneDesc()
+        m = desc.add_material(bsdf=0, albedo=(0.5, 0.5, 0.5))
+        desc.add_mesh(tris.reshape(-1, 3), np.arange(3 * ntris, dtype=np.uint32).reshape(-1, 3), material=m)
+        gs = lantern_amd.Scene(gpu_device, desc, wide_collapse=wide_collapse)
+        os_ = OracleScene(desc)
+        _compare(gs, os_, _random_rays(3000, 50 + ntris, -2, 2))
+        os_.close()
+        gs.close()
