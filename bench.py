@@ -31,7 +31,7 @@ RAY_BYTES = 48
 # commands and raw figures are in profiles/k_trace8_triangle_heavy.md.
 # Keyed by (workload, layout, width, height, spp).
 TRAFFIC = {
-    ("triangle_heavy", 0, 1280, 720, 64): round((637495.03125 * 2.0 + 279850.478125 * 1.0) * 1024),
+    ("triangle_heavy", 0, 1280, 720, 64): round((279189.946875 * 2.0 + 136748.0953125 * 1.0) * 1024),
 }
 
 WORKLOADS = {
@@ -66,7 +66,8 @@ def main():
     ap.add_argument("--builder", type=int, default=0, help="BVH builder: 0 PLOC, 1 linear BVH")
     ap.add_argument("--max-leaf", type=int, default=4, help="BVH leaf size limit (1-4)")
     ap.add_argument("--wide-collapse", type=int, default=0, help="8-wide collapse: 0 SAH-optimal, 1 greedy by area")
-    ap.add_argument("--variant", type=int, default=0, help="traversal kernel variant (0 automatic, 1..4)")
+    ap.add_argument("--variant", type=int, default=0, help="traversal kernel variant (0 automatic, 1..6)")
+    ap.add_argument("--streams", type=int, default=0, help="render streams (0 automatic, 1 or 2)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rendering iteration timing")
@@ -107,6 +108,7 @@ def main():
     params["wave_paths"] = args.wave_paths
     params["block_size"] = args.block
     params["trace_variant"] = args.variant
+    params["streams"] = args.streams
     params["use_graph"] = 1 if args.graph == 1 else 0
     with lantern_amd.Device(0) as dev:
         scene = lantern_amd.Scene(dev, desc, max_leaf_size=args.max_leaf, layout=args.layout, builder=args.builder,
@@ -179,7 +181,8 @@ def main():
                      "kernel": ("k_trace8" if info["layout"] == 0 else "k_trace") + " closest hit (extension rays)",
                      "bytes_per_ray": RAY_BYTES, "launches_per_step": ext_launches // args.steps,
                      "rays_per_launch": round(closest / max(ext_launches, 1)),
-                     "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4)},
+                     "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4),
+                     "streams": per_step[-1]["streams"]},
         "closest_rays_per_step": closest // args.steps,
         "shadow_rays_per_step": shadow // args.steps,
         "bvh": {"layout": "wide8" if info["layout"] == 0 else "binary",
@@ -271,7 +274,8 @@ def bench_batch(args):
     descs = scenes.batch_scenes(64, spp=args.spp or 32)
     gen_s = time.time() - t0
     params = dict(descs[0].params)
-    params.update(wave_paths=args.wave_paths, block_size=args.block, trace_variant=args.variant)
+    params.update(wave_paths=args.wave_paths, block_size=args.block, trace_variant=args.variant,
+                  streams=args.streams)
     with lantern_amd.Device(0) as dev:
         tb = time.time()
         ss = [lantern_amd.Scene(dev, d, layout=args.layout, builder=args.builder) for d in descs]
@@ -340,7 +344,8 @@ def bench_batch(args):
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "kernel": "k_trace8 closest hit, batched (extension rays); per-kernel times from one eager step "
                                "after the timed (graph) steps", "bytes_per_ray": RAY_BYTES,
-                     "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4)},
+                     "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4),
+                     "streams": per_step[-1]["streams"]},
         "scene_gen_s": round(gen_s, 2), "bvh_build_s": round(build_s, 2),
         "autotune": tune, "graph": bool(args.graph),
         "cpu_baseline": None,

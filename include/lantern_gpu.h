@@ -233,7 +233,10 @@ typedef struct lt_render_params {
     int32_t use_graph;     /* 1: capture the bounce loop in a hipGraph and replay it */
     uint32_t trace_variant;/* 0 = automatic; 1 + variant of the 8-wide traversal kernel
                               (LDS stack depth / occupancy target, see the autotuner) */
-    uint32_t reserved[6];
+    uint32_t streams;      /* 0 = automatic; 1: one stream; 2: two waves side by side on two
+                              streams (traversal of one overlaps shading of the other; a
+                              second workspace of the same size). Films do not depend on it. */
+    uint32_t reserved[5];
 } lt_render_params;
 
 typedef struct lt_render_stats {
@@ -251,7 +254,7 @@ typedef struct lt_render_stats {
     uint32_t extend_launches;
     uint32_t shadow_launches;
     uint32_t waves;
-    uint32_t reserved;
+    uint32_t streams;        /* streams the waves ran on (1 or 2) */
 } lt_render_stats;
 
 /* spp x Integrator::RenderFrame (integrator.cpp:28-43) into film; the film

@@ -223,6 +223,7 @@ def params_struct(p):
     r.block_size = p.get("block_size", 0)
     r.use_graph = p.get("use_graph", 0)
     r.trace_variant = p.get("trace_variant", 0)
+    r.streams = p.get("streams", 0)
     return r
 
 

@@ -62,7 +62,7 @@ class lt_render_params(C.Structure):
     _fields_ = [("spp", C.c_uint32), ("max_depth", C.c_uint32), ("rr_start", C.c_uint32), ("seed", C.c_uint32),
                 ("frame0", C.c_uint32), ("stream", C.c_uint32), ("wave_paths", C.c_uint32),
                 ("block_size", C.c_uint32), ("use_graph", C.c_int32), ("trace_variant", C.c_uint32),
-                ("reserved", C.c_uint32 * 6)]
+                ("streams", C.c_uint32), ("reserved", C.c_uint32 * 5)]
 
 
 class lt_tune_result(C.Structure):
@@ -76,7 +76,7 @@ class lt_render_stats(C.Structure):
                 ("path_segments", C.c_uint64), ("total_ms", C.c_double), ("extend_ms", C.c_double),
                 ("shade_ms", C.c_double), ("shadow_ms", C.c_double), ("resolve_ms", C.c_double),
                 ("generate_ms", C.c_double), ("accumulate_ms", C.c_double), ("extend_launches", C.c_uint32),
-                ("shadow_launches", C.c_uint32), ("waves", C.c_uint32), ("reserved", C.c_uint32)]
+                ("shadow_launches", C.c_uint32), ("waves", C.c_uint32), ("streams", C.c_uint32)]
 
 
 _vp = C.c_void_p

@@ -240,15 +240,28 @@ int lt_device_create(int ordinal, lt_device **out) {
     if (!d) return set_error(LT_E_NOMEM, "out of host memory");
     d->ordinal = ordinal;
     d->num_cus = prop.multiProcessorCount;
+    /* both streams at the default priority */
     hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete d;
         return hip_error(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
     }
+    e = hipStreamCreateWithFlags(&d->lane[1].stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(d->stream);
+        delete d;
+        return hip_error(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+    }
     int rc = d->trace_work.alloc(kWorkWords * sizeof(uint32_t));
+    if (!rc && hipHostMalloc((void **)&d->pinned, 256, hipHostMallocDefault) != hipSuccess) {
+        d->pinned = nullptr;
+        rc = set_error(LT_E_NOMEM, "hipHostMalloc of the live-count words failed");
+    }
     if (!rc) rc = d->diag.alloc(64);
     if (!rc && hipMemset(d->diag.p, 0, 64) != hipSuccess) rc = set_error(LT_E_HIP, "hipMemset of the diagnostics word failed");
     if (rc) {
+        if (d->pinned) (void)hipHostFree(d->pinned);
+        (void)hipStreamDestroy(d->lane[1].stream);
         (void)hipStreamDestroy(d->stream);
         delete d;
         return rc;
@@ -261,12 +274,18 @@ int lt_device_destroy(lt_device *d) {
     if (!d) return LT_OK;
     (void)hipSetDevice(d->ordinal);
     (void)hipStreamSynchronize(d->stream);
+    (void)hipStreamSynchronize(d->lane[1].stream);
     if (d->graph_exec) (void)hipGraphExecDestroy((hipGraphExec_t)d->graph_exec);
     d->ws.release();
     d->ctl.release();
     d->trace_work.release();
     d->spill.release();
     d->diag.release();
+    d->lane[1].ws.release();
+    d->lane[1].ctl.release();
+    d->lane[1].spill.release();
+    if (d->pinned) (void)hipHostFree(d->pinned);
+    (void)hipStreamDestroy(d->lane[1].stream);
     (void)hipStreamDestroy(d->stream);
     delete d;
     return LT_OK;

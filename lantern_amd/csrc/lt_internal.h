@@ -140,11 +140,19 @@ struct lt_device {
     lt::DevBuf spill;        /* traversal stack overflow area */
     lt::DevBuf diag;         /* [0]: rays that hit a traversal bound (stack/steps) */
     lt::DevBuf batch_tables; /* scene, camera, film and segment tables of a batch render */
+    /* the second stream of a render that runs two waves side by side, with
+     * its own workspace, control block and traversal spill area (lane[0] is
+     * unused: the first lane is stream/ws/ctl/spill above) */
+    struct Lane {
+        hipStream_t stream = nullptr;
+        lt::DevBuf ws, ctl, spill;
+        size_t ws_paths = 0;
+    } lane[2];
     /* the last captured launch sequence, replayed while its arguments repeat */
     void *graph_exec = nullptr;
     std::string graph_key;
     uint32_t graph_captures = 0;
-    uint32_t *pinned = nullptr;
+    uint32_t *pinned = nullptr;   /* 64 words of pinned host memory: live counts read back during a render */
 };
 
 struct lt_scene {

@@ -824,14 +824,14 @@ struct Workspace {
     uint32_t *nee_sid;       /* batch: per compacted next-event ray, its scene */
 };
 
-int carve(lt_device *dev, size_t n, Workspace &w) {
+int carve(lt::DevBuf &ws, size_t &ws_paths, size_t n, Workspace &w) {
     size_t bytes = n * kBytesPerPath + 64 * 256;
-    if (dev->ws_paths < n) {
-        int rc = dev->ws.alloc(bytes);
+    if (ws_paths < n) {
+        int rc = ws.alloc(bytes);
         if (rc) return rc;
-        dev->ws_paths = n;
+        ws_paths = n;
     }
-    char *p = (char *)dev->ws.p;
+    char *p = (char *)ws.p;
     auto take = [&](size_t each) {
         char *r = p;
         p += (n * each + 255) & ~(size_t)255;
@@ -861,9 +861,8 @@ int carve(lt_device *dev, size_t n, Workspace &w) {
 
 struct EventTimer {
     bool on;
-    hipStream_t st;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[6];
-    explicit EventTimer(bool enable, hipStream_t s) : on(enable), st(s) {}
+    explicit EventTimer(bool enable) : on(enable) {}
     ~EventTimer() {
         for (auto &v : ev)
             for (auto &e : v) {
@@ -871,7 +870,7 @@ struct EventTimer {
                 (void)hipEventDestroy(e.second);
             }
     }
-    int begin(int k) {
+    int begin(int k, hipStream_t st) {
         if (!on) return 0;
         hipEvent_t a, b;
         LT_HIP(hipEventCreate(&a));
@@ -880,11 +879,12 @@ struct EventTimer {
         LT_HIP(hipEventRecord(a, st));
         return 0;
     }
-    int end(int k) {
+    int end(int k, hipStream_t st) {
         if (!on) return 0;
         LT_HIP(hipEventRecord(ev[k].back().second, st));
         return 0;
     }
+    /* sum of the kernels' own durations (with two lanes they overlap) */
     double total(int k) {
         double t = 0;
         for (auto &e : ev[k]) {
@@ -950,114 +950,187 @@ struct WavePlan {
 
 /* The wavefront launch sequence over all planned waves; see the file comment.
  * ADJ: the adjoint replay (no film). BATCH: several scenes per wave. */
+/* One stream of the launch sequence with its own workspace, control block
+ * and traversal spill area. */
+struct Lane {
+    hipStream_t st;
+    Workspace w;
+    Ctl *ctl;
+    int *spill;
+};
+
+/* The wavefront launch sequence over all planned waves; see the file comment.
+ * ADJ: the adjoint replay (no film). BATCH: several scenes per wave. With two
+ * lanes, waves 2g and 2g + 1 run side by side on the two streams (the
+ * traversal of one wave overlaps the latency-bound shading of the other);
+ * film accumulation stays in wave order, so the result does not depend on
+ * the number of lanes. */
 template <bool ADJ, bool BATCH>
 int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, const Seg *d_segs,
-              const lt_render_params &prm, lt_film *film, lt_render_stats *stats, Workspace &w) {
-    hipStream_t st = dev->stream;
+              const lt_render_params &prm, lt_film *film, lt_render_stats *stats, Lane *lanes, int nlanes) {
     int rc;
-    if ((rc = dev->ctl.reserve(sizeof(Ctl)))) return rc;
-    Ctl *ctl = (Ctl *)dev->ctl.p;
     const int tblock = prm.block_size ? (int)prm.block_size : trace_default_block();
     const int tvar = prm.trace_variant ? (int)prm.trace_variant - 1 : kDefaultTraceVariant;
     if (tvar < 0 || tvar >= kTraceVariants) return set_error(LT_E_INVALID, "unknown traversal variant");
     const int tgrid = trace_grid(dev->num_cus, tblock, tvar);
-    if ((rc = dev->spill.reserve(trace_spill_bytes(tgrid, tblock)))) return rc;
-    int *spill = (int *)dev->spill.p;
+    for (int k = 0; k < nlanes; ++k) {
+        lt::DevBuf &cb = k == 0 ? dev->ctl : dev->lane[k].ctl;
+        lt::DevBuf &sb = k == 0 ? dev->spill : dev->lane[k].spill;
+        if ((rc = cb.reserve(sizeof(Ctl))) || (rc = sb.reserve(trace_spill_bytes(tgrid, tblock)))) return rc;
+        lanes[k].ctl = (Ctl *)cb.p;
+        lanes[k].spill = (int *)sb.p;
+    }
     uint32_t *diag = dev->diag.as<uint32_t>();
     const bool graph = prm.use_graph != 0;
     const int sgrid = std::max(1, dev->num_cus) * 8;
     /* per-kernel event timing only outside graphs */
-    EventTimer tm(stats != nullptr && !graph, st);
+    EventTimer tm(stats != nullptr && !graph);
     uint32_t ext_launches = 0, sh_launches = 0;
     uint64_t samples = 0;
+    /* per lane: the end of its last film accumulation, and of its last
+     * live-count copy */
+    hipEvent_t acc_ev[2] = {nullptr, nullptr}, chk_ev[2] = {nullptr, nullptr};
+    struct EvGuard {
+        hipEvent_t *e;
+        ~EvGuard() {
+            for (int k = 0; k < 2; ++k)
+                if (e[k]) (void)hipEventDestroy(e[k]);
+        }
+    } acc_guard{acc_ev}, chk_guard{chk_ev};
+    for (int k = 0; k < nlanes; ++k) {
+        LT_HIP(hipEventCreateWithFlags(&acc_ev[k], hipEventDisableTiming));
+        LT_HIP(hipEventCreateWithFlags(&chk_ev[k], hipEventDisableTiming));
+    }
 
-    /* The launch sequence of all waves. With early_exit, a wave stops after
-     * every 8th bounce once no path is alive (a host round trip); a captured
-     * graph runs all max_depth bounces, whose kernels return at once when
-     * their live count is zero. */
+    /* The launch sequence of all waves. With early_exit, a group of waves
+     * stops after every 8th bounce once no path is alive (a host round trip);
+     * a captured graph runs all max_depth bounces, whose kernels return at
+     * once when their live count is zero. */
     auto enqueue = [&](bool early_exit) -> int {
         int r;
-        hipLaunchKernelGGL(k_reset_tally, dim3(1), dim3(64), 0, st, ctl);
-        for (const WavePlan &wp : plan) {
-            const uint32_t n = wp.n;
-            a.sample_base = wp.sample_base;
-            if (BATCH) {
-                a.batch.segs = d_segs + wp.seg0;
-                a.batch.nsegs = wp.nsegs;
-                a.batch.npix_slots = wp.npix_slots;
+        for (int k = 0; k < nlanes; ++k) hipLaunchKernelGGL(k_reset_tally, dim3(1), dim3(64), 0, lanes[k].st, lanes[k].ctl);
+        int last_acc = -1;   /* lane of the last film accumulation */
+        for (size_t g = 0; g < plan.size(); g += (size_t)nlanes) {
+            const int nw = (int)std::min<size_t>((size_t)nlanes, plan.size() - g);
+            WaveArgs wa[2];
+            Queue cur[2], nxt[2];
+            uint32_t wslot[2] = {0, 0};
+            for (int k = 0; k < nw; ++k) {
+                const WavePlan &wp = plan[g + k];
+                Lane &L = lanes[k];
+                WaveArgs &A = wa[k];
+                A = a;
+                A.ps = L.w.ps;
+                A.sample_base = wp.sample_base;
+                if (BATCH) {
+                    A.batch.segs = d_segs + wp.seg0;
+                    A.batch.nsegs = wp.nsegs;
+                    A.batch.npix_slots = wp.npix_slots;
+                    A.batch.path_seg = L.w.path_seg;
+                    A.batch.nee_sid = L.w.nee_sid;
+                }
+                samples += wp.n;
+                hipLaunchKernelGGL(k_reset_ctl, dim3(64), dim3(256), 0, L.st, L.ctl, 2u * prm.max_depth);
+                tm.begin(T_GEN, L.st);
+                hipLaunchKernelGGL((k_generate<ADJ, BATCH>), dim3(std::min<uint32_t>((wp.n + 255) / 256, 65535)),
+                                   dim3(256), 0, L.st, A, L.w.qa, wp.n, L.ctl);
+                tm.end(T_GEN, L.st);
+                LT_HIP(hipGetLastError());
+                cur[k] = L.w.qa;
+                nxt[k] = L.w.qb;
             }
-            samples += n;
-            hipLaunchKernelGGL(k_reset_ctl, dim3(64), dim3(256), 0, st, ctl, 2u * prm.max_depth);
-            tm.begin(T_GEN);
-            hipLaunchKernelGGL((k_generate<ADJ, BATCH>), dim3(std::min<uint32_t>((n + 255) / 256, 65535)), dim3(256),
-                               0, st, a, w.qa, n, ctl);
-            tm.end(T_GEN);
-            LT_HIP(hipGetLastError());
-            Queue cur = w.qa, nxt = w.qb;
-            uint32_t wslot = 0;
+            bool pending = false;
             for (uint32_t b = 0; b < prm.max_depth; ++b) {
                 if (early_exit && b > 0 && (b % 8) == 0) {
-                    uint32_t live = 0;
-                    LT_HIP(hipMemcpyAsync(&live, &ctl->count[b], sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-                    LT_HIP(hipStreamSynchronize(st));
-                    if (live == 0) break;
+                    /* the live counts copied 8 bounces ago: waiting for them
+                     * leaves 8 bounces queued, so the device never idles */
+                    if (pending) {
+                        uint32_t live = 0;
+                        for (int k = 0; k < nw; ++k) {
+                            LT_HIP(hipEventSynchronize(chk_ev[k]));
+                            live += dev->pinned[k];
+                        }
+                        if (live == 0) break;
+                    }
+                    for (int k = 0; k < nw; ++k) {
+                        LT_HIP(hipMemcpyAsync(&dev->pinned[k], &lanes[k].ctl->count[b], sizeof(uint32_t),
+                                              hipMemcpyDeviceToHost, lanes[k].st));
+                        LT_HIP(hipEventRecord(chk_ev[k], lanes[k].st));
+                    }
+                    pending = true;
                 }
-                if (wslot + 2 > 4096) return set_error(LT_E_STATE, "too many traversal launches in one wave");
-                TraceBatch tb_ext = {a.batch.scenes, cur.sid};
-                TraceBatch tb_nee = {a.batch.scenes, a.batch.nee_sid};
-                tm.begin(T_EXT);
-                if ((r = launch_trace_closest(st, a.sc, cur.rays, w.hits, &ctl->count[b], 0, &ctl->work[(size_t)(wslot++) * kWorkWords],
-                                              tblock, tgrid, spill, diag, tvar, BATCH ? &tb_ext : nullptr)))
-                    return r;
-                tm.end(T_EXT);
-                ++ext_launches;
-                tm.begin(T_SHADE);
-                hipLaunchKernelGGL((k_shade<ADJ, BATCH>), dim3(sgrid), dim3(kShadeBlock), 0, st, a, b, cur,
-                                   (const Hit *)w.hits, nxt, w.nee, w.nee_rays, w.nee_hits, w.nee_ignore, w.nee_cidx,
-                                   w.nee_dest, ctl);
-                tm.end(T_SHADE);
-                LT_HIP(hipGetLastError());
-                /* next-event rays: occlusion up to the light hit (small lights) or
-                 * the reference's closest-hit query */
-                tm.begin(T_SHADOW);
-                if (a.sc.light_first)
-                    r = launch_trace_any(st, a.sc, w.nee_rays, w.nee_occl, w.nee_ignore, &ctl->nee_count[b], 0,
-                                         &ctl->work[(size_t)(wslot++) * kWorkWords], tblock, tgrid, spill, diag, tvar,
-                                         BATCH ? &tb_nee : nullptr);
-                else
-                    r = launch_trace_closest(st, a.sc, w.nee_rays, w.nee_trace_hits, &ctl->nee_count[b], 0,
-                                             &ctl->work[(size_t)(wslot++) * kWorkWords], tblock, tgrid, spill, diag, tvar,
-                                             BATCH ? &tb_nee : nullptr);
-                if (r) return r;
-                tm.end(T_SHADOW);
-                ++sh_launches;
-                tm.begin(T_RESOLVE);
-                hipLaunchKernelGGL((k_resolve<ADJ, BATCH>), dim3(sgrid), dim3(256), 0, st, a, b, cur, nxt, w.nee,
-                                   (const Ray *)w.nee_rays, (const Hit *)w.nee_hits, (const Hit *)w.nee_trace_hits,
-                                   (const uint32_t *)w.nee_occl, (const uint32_t *)w.nee_cidx,
-                                   (const uint32_t *)w.nee_dest, ctl);
-                tm.end(T_RESOLVE);
-                LT_HIP(hipGetLastError());
-                std::swap(cur, nxt);
+                for (int k = 0; k < nw; ++k) {
+                    Lane &L = lanes[k];
+                    const WaveArgs &A = wa[k];
+                    Ctl *ctl = L.ctl;
+                    if (wslot[k] + 2 > 4096) return set_error(LT_E_STATE, "too many traversal launches in one wave");
+                    TraceBatch tb_ext = {A.batch.scenes, cur[k].sid};
+                    TraceBatch tb_nee = {A.batch.scenes, A.batch.nee_sid};
+                    const Workspace &w = L.w;
+                    tm.begin(T_EXT, L.st);
+                    if ((r = launch_trace_closest(L.st, A.sc, cur[k].rays, w.hits, &ctl->count[b], 0,
+                                                  &ctl->work[(size_t)(wslot[k]++) * kWorkWords], tblock, tgrid,
+                                                  L.spill, diag, tvar, BATCH ? &tb_ext : nullptr)))
+                        return r;
+                    tm.end(T_EXT, L.st);
+                    ++ext_launches;
+                    tm.begin(T_SHADE, L.st);
+                    hipLaunchKernelGGL((k_shade<ADJ, BATCH>), dim3(sgrid), dim3(kShadeBlock), 0, L.st, A, b, cur[k],
+                                       (const Hit *)w.hits, nxt[k], w.nee, w.nee_rays, w.nee_hits, w.nee_ignore,
+                                       w.nee_cidx, w.nee_dest, ctl);
+                    tm.end(T_SHADE, L.st);
+                    LT_HIP(hipGetLastError());
+                    /* next-event rays: occlusion up to the light hit (small lights) or
+                     * the reference's closest-hit query */
+                    tm.begin(T_SHADOW, L.st);
+                    if (A.sc.light_first)
+                        r = launch_trace_any(L.st, A.sc, w.nee_rays, w.nee_occl, w.nee_ignore, &ctl->nee_count[b], 0,
+                                             &ctl->work[(size_t)(wslot[k]++) * kWorkWords], tblock, tgrid, L.spill,
+                                             diag, tvar, BATCH ? &tb_nee : nullptr);
+                    else
+                        r = launch_trace_closest(L.st, A.sc, w.nee_rays, w.nee_trace_hits, &ctl->nee_count[b], 0,
+                                                 &ctl->work[(size_t)(wslot[k]++) * kWorkWords], tblock, tgrid,
+                                                 L.spill, diag, tvar, BATCH ? &tb_nee : nullptr);
+                    if (r) return r;
+                    tm.end(T_SHADOW, L.st);
+                    ++sh_launches;
+                    tm.begin(T_RESOLVE, L.st);
+                    hipLaunchKernelGGL((k_resolve<ADJ, BATCH>), dim3(sgrid), dim3(256), 0, L.st, A, b, cur[k], nxt[k],
+                                       w.nee, (const Ray *)w.nee_rays, (const Hit *)w.nee_hits,
+                                       (const Hit *)w.nee_trace_hits, (const uint32_t *)w.nee_occl,
+                                       (const uint32_t *)w.nee_cidx, (const uint32_t *)w.nee_dest, ctl);
+                    tm.end(T_RESOLVE, L.st);
+                    LT_HIP(hipGetLastError());
+                    std::swap(cur[k], nxt[k]);
+                }
             }
-            if (!ADJ) {
-                tm.begin(T_ACC);
-                if (BATCH)
-                    hipLaunchKernelGGL(k_accumulate_batch,
-                                       dim3(std::min<uint32_t>((wp.npix_slots + 255) / 256, 65535)), dim3(256), 0,
-                                       st, a);
-                else
-                    hipLaunchKernelGGL(k_accumulate, dim3(std::min<uint32_t>((a.npix + 255) / 256, 65535)),
-                                       dim3(256), 0, st, a, wp.ns, film->sum.as<float>(),
-                                       film->bounces.as<uint32_t>(), film->count.as<uint32_t>());
-                tm.end(T_ACC);
-                LT_HIP(hipGetLastError());
+            for (int k = 0; k < nw; ++k) {
+                Lane &L = lanes[k];
+                const WavePlan &wp = plan[g + k];
+                if (!ADJ) {
+                    /* film accumulation in wave order, across the lanes */
+                    if (last_acc >= 0 && last_acc != k) LT_HIP(hipStreamWaitEvent(L.st, acc_ev[last_acc], 0));
+                    tm.begin(T_ACC, L.st);
+                    if (BATCH)
+                        hipLaunchKernelGGL(k_accumulate_batch,
+                                           dim3(std::min<uint32_t>((wp.npix_slots + 255) / 256, 65535)), dim3(256),
+                                           0, L.st, wa[k]);
+                    else
+                        hipLaunchKernelGGL(k_accumulate, dim3(std::min<uint32_t>((a.npix + 255) / 256, 65535)),
+                                           dim3(256), 0, L.st, wa[k], wp.ns, film->sum.as<float>(),
+                                           film->bounces.as<uint32_t>(), film->count.as<uint32_t>());
+                    tm.end(T_ACC, L.st);
+                    LT_HIP(hipGetLastError());
+                    if (nlanes > 1) LT_HIP(hipEventRecord(acc_ev[k], L.st));
+                    last_acc = k;
+                }
+                hipLaunchKernelGGL(k_tally, dim3(1), dim3(64), 0, L.st, L.ctl, prm.max_depth);
             }
-            hipLaunchKernelGGL(k_tally, dim3(1), dim3(64), 0, st, ctl, prm.max_depth);
         }
         return LT_OK;
     };
 
+    hipStream_t st = lanes[0].st;
     hipEvent_t e0, e1;
     LT_HIP(hipEventCreate(&e0));
     LT_HIP(hipEventCreate(&e1));
@@ -1065,8 +1138,15 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
         /* replay the cached graph when every kernel argument is the same */
         std::string key((const char *)&a, sizeof a);
         for (const WavePlan &wp : plan) key.append((const char *)&wp, sizeof wp);
-        const void *ptrs[4] = {d_segs, film ? film->sum.p : nullptr, spill, ctl};
-        const int ints[5] = {tblock, tvar, tgrid, (int)prm.max_depth, (int)BATCH * 2 + (int)ADJ};
+        const void *ptrs[8] = {d_segs, film ? film->sum.p : nullptr, lanes[0].spill, lanes[0].ctl, nullptr,
+                               nullptr, nullptr, nullptr};
+        if (nlanes > 1) {
+            ptrs[4] = lanes[1].spill;
+            ptrs[5] = lanes[1].ctl;
+            ptrs[6] = lanes[1].w.ps.col;
+            ptrs[7] = lanes[1].w.qa.rays;
+        }
+        const int ints[6] = {tblock, tvar, tgrid, (int)prm.max_depth, (int)BATCH * 2 + (int)ADJ, nlanes};
         key.append((const char *)ptrs, sizeof ptrs);
         key.append((const char *)ints, sizeof ints);
         if (!dev->graph_exec || dev->graph_key != key) {
@@ -1074,8 +1154,33 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
                 (void)hipGraphExecDestroy((hipGraphExec_t)dev->graph_exec);
                 dev->graph_exec = nullptr;
             }
+            /* the second lane joins the capture through an event of the
+             * first (fork) and is joined back before the capture ends */
+            hipEvent_t fj[2] = {nullptr, nullptr};
+            struct FjGuard {
+                hipEvent_t *e;
+                ~FjGuard() {
+                    for (int k = 0; k < 2; ++k)
+                        if (e[k]) (void)hipEventDestroy(e[k]);
+                }
+            } fj_guard{fj};
+            LT_HIP(hipEventCreateWithFlags(&fj[0], hipEventDisableTiming));
+            LT_HIP(hipEventCreateWithFlags(&fj[1], hipEventDisableTiming));
             LT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            rc = enqueue(false);
+            rc = LT_OK;
+            bool forked = false;
+            if (nlanes > 1) {
+                hipError_t fe = hipEventRecord(fj[0], st);
+                if (fe == hipSuccess) fe = hipStreamWaitEvent(lanes[1].st, fj[0], 0);
+                if (fe != hipSuccess) rc = hip_error(fe, "fork of the second lane", __FILE__, __LINE__);
+                forked = fe == hipSuccess;
+            }
+            if (!rc) rc = enqueue(false);
+            if (forked) {
+                hipError_t je = hipEventRecord(fj[1], lanes[1].st);
+                if (je == hipSuccess) je = hipStreamWaitEvent(st, fj[1], 0);
+                if (je != hipSuccess && !rc) rc = hip_error(je, "join of the second lane", __FILE__, __LINE__);
+            }
             hipGraph_t g = nullptr;
             hipError_t ce = hipStreamEndCapture(st, &g);
             if (rc) {
@@ -1099,7 +1204,13 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
         LT_HIP(hipEventRecord(e1, st));
     } else {
         LT_HIP(hipEventRecord(e0, st));
+        for (int k = 1; k < nlanes; ++k) LT_HIP(hipStreamWaitEvent(lanes[k].st, e0, 0));
         if ((rc = enqueue(true))) return rc;
+        /* join the other lanes into the first before the end event */
+        for (int k = 1; k < nlanes; ++k) {
+            LT_HIP(hipEventRecord(acc_ev[k], lanes[k].st));
+            LT_HIP(hipStreamWaitEvent(st, acc_ev[k], 0));
+        }
         LT_HIP(hipEventRecord(e1, st));
     }
     LT_HIP(hipEventSynchronize(e1));
@@ -1110,7 +1221,12 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
     if ((rc = check_traversal_bounds(dev))) return rc;
     if (stats) {
         unsigned long long tally[2] = {0, 0};
-        LT_HIP(hipMemcpy(tally, ctl->tally, sizeof tally, hipMemcpyDeviceToHost));
+        for (int k = 0; k < nlanes; ++k) {
+            unsigned long long t[2];
+            LT_HIP(hipMemcpy(t, lanes[k].ctl->tally, sizeof t, hipMemcpyDeviceToHost));
+            tally[0] += t[0];
+            tally[1] += t[1];
+        }
         std::memset(stats, 0, sizeof *stats);
         stats->samples = samples;
         stats->closest_rays = tally[0];
@@ -1126,6 +1242,7 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
         stats->extend_launches = ext_launches;
         stats->shadow_launches = sh_launches;
         stats->waves = (uint32_t)plan.size();
+        stats->streams = (uint32_t)nlanes;
     }
     return LT_OK;
 }
@@ -1134,16 +1251,41 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
  * workspace: fewer, fuller launches and shorter tails than small waves), or
  * what half of the device memory not yet in use by other allocations holds,
  * if that is less. */
-size_t wave_budget(lt_device *dev, const lt_render_params &prm) {
+size_t wave_budget(lt_device *dev, const lt_render_params &prm, int nlanes) {
     if (prm.wave_paths) return prm.wave_paths;
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return (size_t)16 << 20;
-    const size_t usable = (free_b + dev->ws_paths * kBytesPerPath) / 2;
-    return std::max<size_t>((size_t)1 << 20, std::min<size_t>(kDefaultWavePaths, usable / kBytesPerPath));
+    const size_t usable = (free_b + (dev->ws_paths + dev->lane[1].ws_paths) * kBytesPerPath) / 2;
+    const size_t n = std::min<size_t>(kDefaultWavePaths, usable / kBytesPerPath) / (size_t)nlanes;
+    return std::max<size_t>((size_t)1 << 20, n);
+}
+
+/* Streams a render runs on (lt_render_params.streams): 0 picks two when the
+ * render holds at least kTwoLanePaths paths that split into two waves of
+ * whole samples. */
+constexpr size_t kTwoLanePaths = (size_t)16 << 20;
+int lane_count(const lt_render_params &prm, size_t paths, uint32_t units) {
+    if (units < 2) return 1;
+    if (prm.streams == 1 || prm.streams == 2) return (int)prm.streams;
+    return paths >= kTwoLanePaths ? 2 : 1;
+}
+
+/* Workspace, control block and spill area of each lane; lane 0 uses the
+ * device's own, lane 1 the second set. */
+int make_lanes(lt_device *dev, size_t nmax, int nlanes, Lane *lanes) {
+    int rc;
+    lanes[0].st = dev->stream;
+    if ((rc = carve(dev->ws, dev->ws_paths, nmax, lanes[0].w))) return rc;
+    if (nlanes > 1) {
+        lanes[1].st = dev->lane[1].stream;
+        if ((rc = carve(dev->lane[1].ws, dev->lane[1].ws_paths, nmax, lanes[1].w))) return rc;
+    }
+    return LT_OK;
 }
 
 int check_params(const lt_render_params &prm) {
     if (prm.max_depth == 0 || prm.max_depth > 2000) return set_error(LT_E_INVALID, "max_depth must be in 1..2000");
+    if (prm.streams > 2) return set_error(LT_E_INVALID, "streams must be 0, 1 or 2");
     return LT_OK;
 }
 
@@ -1158,11 +1300,20 @@ int render_impl(lt_scene *scene, const lt_camera &cam, const lt_render_params &p
     if (npix == 0 || prm.spp == 0) return LT_OK;
     int rc = check_params(prm);
     if (rc) return rc;
-    uint32_t spw = (uint32_t)std::max<size_t>(1, std::min<size_t>(prm.spp, wave_budget(dev, prm) / npix));
+    const int nlanes = lane_count(prm, (size_t)prm.spp * npix, prm.spp);
+    const size_t budget = wave_budget(dev, prm, nlanes);
+    uint32_t spw = (uint32_t)std::max<size_t>(1, std::min<size_t>(prm.spp, budget / npix));
+    if (nlanes > 1) {
+        /* an even number of waves of equal size, one per lane in turn */
+        uint32_t nw = std::max<uint32_t>(2, (prm.spp + spw - 1) / spw);
+        nw += nw & 1;
+        spw = (prm.spp + nw - 1) / nw;
+    }
     size_t nmax = (size_t)spw * npix;
     if (nmax > 0xFFFFFFFFull / 2) return set_error(LT_E_INVALID, "wave too large");
-    Workspace w;
-    if ((rc = carve(dev, nmax, w))) return rc;
+    Lane lanes[2];
+    if ((rc = make_lanes(dev, nmax, nlanes, lanes))) return rc;
+    Workspace &w = lanes[0].w;
     std::vector<WavePlan> plan;
     for (uint32_t s0 = 0; s0 < prm.spp; s0 += spw) {
         WavePlan wp = {};
@@ -1186,8 +1337,8 @@ int render_impl(lt_scene *scene, const lt_camera &cam, const lt_render_params &p
     a.stream = prm.stream;
     a.max_depth = prm.max_depth;
     a.rr_start = prm.rr_start;
-    if (replay) return run_waves<true, false>(dev, a, plan, nullptr, prm, film, stats, w);
-    return run_waves<false, false>(dev, a, plan, nullptr, prm, film, stats, w);
+    if (replay) return run_waves<true, false>(dev, a, plan, nullptr, prm, film, stats, lanes, nlanes);
+    return run_waves<false, false>(dev, a, plan, nullptr, prm, film, stats, lanes, nlanes);
 }
 
 /* Several scenes in one launch sequence: their (scene, sample) ranges are
@@ -1225,7 +1376,12 @@ int render_batch(lt_scene *const *scenes, const lt_camera *cams, lt_film *const 
     for (DScene &d : tbl) d.light_first = light_first ? 1u : 0u;
 
     /* plan: pack whole samples of each scene into waves */
-    const size_t budget = wave_budget(scenes[0]->dev, prm);
+    size_t total = 0;
+    for (uint32_t i = 0; i < nscenes; ++i) total += (size_t)cams[i].width * cams[i].height * prm.spp;
+    const int nlanes = lane_count(prm, total, 2);
+    size_t budget = wave_budget(dev, prm, nlanes);
+    /* two lanes: at least two waves */
+    if (nlanes > 1) budget = std::min(budget, (total + 1) / 2);
     std::vector<Seg> segs;
     std::vector<WavePlan> plan;
     WavePlan cur = {};
@@ -1270,8 +1426,9 @@ int render_batch(lt_scene *const *scenes, const lt_camera *cams, lt_film *const 
     close_wave();
     if (plan.empty()) return LT_OK;
     if (nmax > 0xFFFFFFFFull / 2) return set_error(LT_E_INVALID, "wave too large");
-    Workspace w;
-    if ((rc = carve(dev, nmax, w))) return rc;
+    Lane lanes[2];
+    if ((rc = make_lanes(dev, nmax, nlanes, lanes))) return rc;
+    Workspace &w = lanes[0].w;
     const size_t tbytes = nscenes * (sizeof(DScene) + sizeof(CameraArgs) + sizeof(FilmPtrs)) + segs.size() * sizeof(Seg)
                           + 1024;
     if ((rc = dev->batch_tables.reserve(tbytes))) return rc;
@@ -1298,7 +1455,7 @@ int render_batch(lt_scene *const *scenes, const lt_camera *cams, lt_film *const 
     a.stream = prm.stream;
     a.max_depth = prm.max_depth;
     a.rr_start = prm.rr_start;
-    return run_waves<false, true>(dev, a, plan, d_segs, prm, nullptr, stats, w);
+    return run_waves<false, true>(dev, a, plan, d_segs, prm, nullptr, stats, lanes, plan.size() > 1 ? nlanes : 1);
 }
 
 int render_scene(lt_scene *scene, const lt_camera &cam, const lt_render_params &prm, lt_film *film,
