@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--wide-collapse", type=int, default=0, help="8-wide collapse: 0 SAH-optimal, 1 greedy by area")
     ap.add_argument("--variant", type=int, default=0, help="traversal kernel variant (0 automatic, 1..6)")
     ap.add_argument("--streams", type=int, default=0, help="render streams (0 automatic, 1 or 2)")
+    ap.add_argument("--trace-share", type=int, default=0,
+                    help="two streams: percent of the full traversal grid (0 automatic)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rendering iteration timing")
@@ -109,6 +111,7 @@ def main():
     params["block_size"] = args.block
     params["trace_variant"] = args.variant
     params["streams"] = args.streams
+    params["trace_share"] = args.trace_share
     params["use_graph"] = 1 if args.graph == 1 else 0
     with lantern_amd.Device(0) as dev:
         scene = lantern_amd.Scene(dev, desc, max_leaf_size=args.max_leaf, layout=args.layout, builder=args.builder,
@@ -275,7 +278,7 @@ def bench_batch(args):
     gen_s = time.time() - t0
     params = dict(descs[0].params)
     params.update(wave_paths=args.wave_paths, block_size=args.block, trace_variant=args.variant,
-                  streams=args.streams)
+                  streams=args.streams, trace_share=args.trace_share)
     with lantern_amd.Device(0) as dev:
         tb = time.time()
         ss = [lantern_amd.Scene(dev, d, layout=args.layout, builder=args.builder) for d in descs]

@@ -236,7 +236,10 @@ typedef struct lt_render_params {
     uint32_t streams;      /* 0 = automatic; 1: one stream; 2: two waves side by side on two
                               streams (traversal of one overlaps shading of the other; a
                               second workspace of the same size). Films do not depend on it. */
-    uint32_t reserved[5];
+    uint32_t trace_share;  /* with two streams: percent (1-100) of the full traversal grid, the
+                              rest of each CU left to the other stream's shading; 0 = automatic
+                              (50 when a scene holds a grid medium, else 100) */
+    uint32_t reserved[4];
 } lt_render_params;
 
 typedef struct lt_render_stats {

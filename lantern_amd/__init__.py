@@ -224,6 +224,7 @@ def params_struct(p):
     r.use_graph = p.get("use_graph", 0)
     r.trace_variant = p.get("trace_variant", 0)
     r.streams = p.get("streams", 0)
+    r.trace_share = p.get("trace_share", 0)
     return r
 
 

@@ -121,19 +121,22 @@ def test_quad_scene_matches_oracle(gpu_device):
 def test_two_streams_give_identical_images(gpu_device):
     """Two waves side by side on two streams accumulate into the film in
     wave order, so the film equals the one-stream render's bit for bit, for
-    an even and an odd number of waves and for the batch entry point."""
+    an even and an odd number of waves, a reduced traversal grid and the
+    batch entry point."""
     d = scenes.triangle_heavy(128, 72, 6)
     gs = lantern_amd.Scene(gpu_device, d)
     film = lantern_amd.Film(gpu_device, 128, 72)
     for wave_paths in (0, 128 * 72 * 2, 128 * 72 * 3):
         outs = []
-        for streams in (1, 2):
+        for streams, share in ((1, 0), (2, 0), (2, 35)):
             film.reset()
-            st = lantern_amd.render(gs, d.camera, dict(d.params, streams=streams, wave_paths=wave_paths), film)
+            st = lantern_amd.render(gs, d.camera, dict(d.params, streams=streams, wave_paths=wave_paths,
+                                                       trace_share=share), film)
             assert st["streams"] == streams
             outs.append(film.download())
-        for a, b in zip(*outs):
-            assert np.array_equal(a, b), f"wave_paths {wave_paths}"
+        for other in outs[1:]:
+            for a, b in zip(outs[0], other):
+                assert np.array_equal(a, b), f"wave_paths {wave_paths}"
     film.close()
     descs = [scenes.triangle_heavy(64, 48, 3), scenes.nine_sphere(48, 32, 3), scenes.quad_scene(32, 32, 3)]
     ss = [lantern_amd.Scene(gpu_device, x) for x in descs]
