@@ -313,11 +313,13 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
 void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, NeeRec nee, Ray *nee_rays,
              Hit *nee_hits, uint32_t *nee_ignore, uint32_t *nee_cidx, uint32_t *nee_dest, Ctl *ctl) {
     __shared__ uint32_t s_order_all[(kShadeBlock / 64) * kShadeChunk];
+    __shared__ uint8_t s_bin_all[(kShadeBlock / 64) * kShadeChunk];
     const uint32_t count = ctl->count[b];
     const int lane = threadIdx.x & 63;
     const float inf = __builtin_huge_valf();
     const unsigned long long lanes_below = (1ull << lane) - 1ull;
     uint32_t *s_order = s_order_all + (threadIdx.x >> 6) * kShadeChunk;
+    uint8_t *s_bin = s_bin_all + (threadIdx.x >> 6) * kShadeChunk;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     const uint32_t rounds = max(1u, min((uint32_t)kShadeRounds, count / (nwaves * 64u)));
@@ -362,6 +364,7 @@ void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, Nee
                 for (int k = 1; k < kBins; ++k)
                     if (key[r] == k) st0 = bin_start[k];
                 s_order[st0 + rank[r]] = cbase + r * 64 + lane;
+                s_bin[st0 + rank[r]] = (uint8_t)key[r];
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -371,6 +374,8 @@ void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, Nee
             const uint32_t pos = r * 64 + lane;
             const bool valid = pos < nvalid;
             const uint32_t q = valid ? s_order[pos] : 0u;
+            /* bin 0: a miss, known before any load */
+            const bool hit = valid && s_bin[pos] != 0u;
             bool alive = false;
             bool nee1 = false, nee2 = false;
             float tfar1 = 0.0f, tfar2 = 0.0f;
@@ -389,25 +394,32 @@ void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, Nee
                 scene = pi.scene;
                 const DScene &sc = BATCH ? a.batch.scenes[scene] : a.sc;
                 const uint32_t pix = y * a.cam.w + x;
-                const Hit h = hits[q];
                 const float4 s0 = cur.st[2 * (size_t)q];       /* throughput, IOR */
                 const float4 s1 = cur.st[2 * (size_t)q + 1];   /* radiance, draws consumed */
+                /* a hit's slot data, loaded together with the state */
+                Hit h;
+                uint32_t ml = 0u;
+                float4 ra = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rb = ra;
+                if (hit) {
+                    h = hits[q];
+                    ml = cur.ml[q];
+                    const float4 *rp = (const float4 *)(cur.rays + q);
+                    ra = rp[0];
+                    rb = rp[1];
+                }
                 V3 T = v3(s0.x, s0.y, s0.z);
                 C = v3(s1.x, s1.y, s1.z);
                 bool dead = false;
                 uint32_t nee_info = 0;
-                if (h.prim == 0xFFFFFFFFu) {
+                if (!hit) {
                     /* a miss ends the path: only its color and bounce count are
                      * read again */
                     if (ADJ) C = C - T * v3(sc.bg[0], sc.bg[1], sc.bg[2]);
                     else C = C + T * v3(sc.bg[0], sc.bg[1], sc.bg[2]);
                     dead = true;
                 } else {
-                    const uint32_t ml = cur.ml[q];
                     PathRng rng;
                     rng.init(x, y, a.stream + scene, a.seed, pi.frame, __float_as_uint(s1.w));
-                    const float4 *rp = (const float4 *)(cur.rays + q);
-                    const float4 ra = rp[0], rb = rp[1];
                     Interaction it;
                     it.iori = s0.w;
                     it.lobe = (int)ml_lobe(ml);
