@@ -288,6 +288,33 @@ __device__ __forceinline__ uint32_t shade_bin(const WaveArgs &a, const Queue &cu
     return 2u + (uint32_t)min(max(bsdf, 0), kBins - 3);
 }
 
+/* The direct-light term of one next-event ray from o along d that reached
+ * light L at h: AreaLight::SampleLi / PdfLi after their casts
+ * (area_light.cpp:58-71, 100-109) and its part of the MIS sum of
+ * EstimateDirect (integrator.cpp:287-314). LIGHT: the light-sampled ray
+ * (f, p: BSDF value and pdf along it), else the BSDF-sampled one. False when
+ * the term is skipped. */
+template <bool LIGHT>
+__device__ __forceinline__ bool nee_term(const DScene &sc, const DLight &L, const DGeom &lg, const Hit &h, V3 o, V3 d,
+                                         V3 f, float p, V3 &out) {
+    const V3 le = v3(L.le[0], L.le[1], L.le[2]);
+    V3 ip = o + normalize(d) * h.t;
+    V3 dd = ip - o;
+    float d2 = dot(dd, dd);
+    const V3 n = normalize(interp_normal(sc, h.prim, h.u, h.v, lg));
+    float lpdf = d2 / (fabsf(dot(n, LIGHT ? -d : d)) * L.area);
+    if (LIGHT) {
+        if (!(lpdf != 0.0f && !is_black(le) && p != 0.0f && !is_black(f))) return false;
+        float w = power_heuristic(lpdf, p);
+        out = ((f * le) * w) / lpdf;
+    } else {
+        if (!(lpdf != 0.0f)) return false;
+        float w = power_heuristic(p, lpdf);
+        out = ((f * le) * w) / p;
+    }
+    return true;
+}
+
 /* k_shade works through the live slots in chunks of R * 64 consecutive
  * slots per wave. A wave first sorts its chunk by shading bin (a counting
  * sort with ballots, into its own part of LDS), then shades the sorted chunk
@@ -311,7 +338,7 @@ template <bool ADJ, bool BATCH>
  * which the kernel fits without spilling; measured 3% faster (5 waves spill). */
 __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4)))
 void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, NeeRec nee, Ray *nee_rays,
-             Hit *nee_hits, uint32_t *nee_ignore, uint32_t *nee_cidx, uint32_t *nee_dest, Ctl *ctl) {
+             uint32_t *nee_ignore, uint32_t *nee_cidx, uint32_t *nee_dest, Ctl *ctl) {
     __shared__ uint32_t s_order_all[(kShadeBlock / 64) * kShadeChunk];
     __shared__ uint8_t s_bin_all[(kShadeBlock / 64) * kShadeChunk];
     const uint32_t count = ctl->count[b];
@@ -541,8 +568,13 @@ void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, Nee
                                         Hit lh;
                                         nee1 = light_hit(sc, L, ic.P, ldir, lh);
                                         if (nee1) {
-                                            nee_hits[2 * (size_t)q] = lh;
                                             tfar1 = lh.t;
+                                            /* the term is known here; k_resolve adds it if
+                                             * the occlusion query finds the way free */
+                                            V3 t = v3s(0.0f);
+                                            const bool on = nee_term<true>(sc, L, sc.geoms[L.geom], lh, ic.P, ldir, f1, p1, t);
+                                            f1 = t;
+                                            p1 = on ? 1.0f : 0.0f;
                                         }
                                     }
                                 }
@@ -558,8 +590,11 @@ void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, Nee
                                     Hit lh;
                                     nee2 = light_hit(sc, L, ic.P, wi2, lh);
                                     if (nee2) {
-                                        nee_hits[2 * (size_t)q + 1] = lh;
                                         tfar2 = lh.t;
+                                        V3 t = v3s(0.0f);
+                                        const bool on = nee_term<false>(sc, L, sc.geoms[L.geom], lh, ic.P, wi2, f2, p2, t);
+                                        f2 = t;
+                                        p2 = on ? 1.0f : 0.0f;
                                     }
                                 }
                             }
@@ -654,64 +689,41 @@ void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, Nee
 }
 
 /* The casts of AreaLight::SampleLi / PdfLi resolved (area_light.cpp:58-71,
- * 100-109) and the MIS sums of EstimateDirect (integrator.cpp:287-314). */
+ * 100-109) and the MIS sums of EstimateDirect (integrator.cpp:287-314). For
+ * light-first scenes k_shade has already computed each term (record: term,
+ * 1 if it counts); only the occlusion result is new here. Otherwise the
+ * record holds the BSDF value and pdf, and the term comes from the
+ * closest hit of the next-event ray. */
 template <bool ADJ, bool BATCH>
 __global__ __launch_bounds__(256) void k_resolve(WaveArgs a, uint32_t b, Queue cur, Queue next, NeeRec nee,
-                                                 const Ray *nee_rays, const Hit *nee_hits,
-                                                 const Hit *nee_trace_hits, const uint32_t *nee_occl,
-                                                 const uint32_t *nee_cidx, const uint32_t *nee_dest, Ctl *ctl) {
+                                                 const Ray *nee_rays, const Hit *nee_trace_hits,
+                                                 const uint32_t *nee_occl, const uint32_t *nee_cidx,
+                                                 const uint32_t *nee_dest, Ctl *ctl) {
     const uint32_t count = ctl->count[b];
-    
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
         uint32_t info = nee.info[q];
         if (!(info & 0xC0000000u)) continue;
         const DScene &sc = BATCH ? a.batch.scenes[cur.sid[q]] : a.sc;
-        uint32_t li = info & 0x3FFFFFFFu;
-        const DLight L = sc.lights[li];
-        const DGeom lg = sc.geoms[L.geom];
-        V3 le = v3(L.le[0], L.le[1], L.le[2]);
         V3 direct = v3s(0.0f);
-        if (info & 0x40000000u) {
-            const uint32_t c = nee_cidx[2 * (size_t)q];
-            const Hit h = sc.light_first ? nee_hits[2 * (size_t)q] : nee_trace_hits[c];
-            bool reached = sc.light_first ? nee_occl[c] == 0u
-                                          : (h.prim != 0xFFFFFFFFu && sc.tri_geom[h.prim] == L.geom);
-            if (reached) {
-                const Ray r = nee_rays[c];
-                V3 o = v3(r.ox, r.oy, r.oz), d = v3(r.dx, r.dy, r.dz);
-                V3 ip = o + normalize(d) * h.t;
-                V3 dd = ip - o;
-                float d2 = dot(dd, dd);
-                float lpdf = d2 / (fabsf(dot(normalize(interp_normal(sc, h.prim, h.u, h.v, lg)), -d)) * L.area);
-                const float4 r1 = nee.rec[3 * (size_t)q + 1];
-                V3 f1 = v3(r1.x, r1.y, r1.z);
-                float p1 = r1.w;
-                if (lpdf != 0.0f && !is_black(le) && p1 != 0.0f && !is_black(f1)) {
-                    float w = power_heuristic(lpdf, p1);
-                    direct = direct + ((f1 * le) * w) / lpdf;
-                }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (!(info & (0x40000000u << k))) continue;
+            const uint32_t c = nee_cidx[2 * (size_t)q + k];
+            const float4 rk = nee.rec[3 * (size_t)q + 1 + k];
+            if (sc.light_first) {
+                if (nee_occl[c] == 0u && rk.w != 0.0f) direct = direct + v3(rk.x, rk.y, rk.z);
+                continue;
             }
-        }
-        if (info & 0x80000000u) {
-            const uint32_t c = nee_cidx[2 * (size_t)q + 1];
-            const Hit h = sc.light_first ? nee_hits[2 * (size_t)q + 1] : nee_trace_hits[c];
-            bool reached = sc.light_first ? nee_occl[c] == 0u
-                                          : (h.prim != 0xFFFFFFFFu && sc.tri_geom[h.prim] == L.geom);
-            if (reached) {
-                const Ray r = nee_rays[c];
-                V3 o = v3(r.ox, r.oy, r.oz), d = v3(r.dx, r.dy, r.dz);
-                V3 ip = o + normalize(d) * h.t;
-                V3 dd = ip - o;
-                float d2 = dot(dd, dd);
-                float lpdf = d2 / (fabsf(dot(normalize(interp_normal(sc, h.prim, h.u, h.v, lg)), d)) * L.area);
-                if (lpdf != 0.0f) {
-                    const float4 r2 = nee.rec[3 * (size_t)q + 2];
-                    float p2 = r2.w;
-                    V3 f2 = v3(r2.x, r2.y, r2.z);
-                    float w = power_heuristic(p2, lpdf);
-                    direct = direct + ((f2 * le) * w) / p2;
-                }
-            }
+            const DLight L = sc.lights[info & 0x3FFFFFFFu];
+            const Hit h = nee_trace_hits[c];
+            if (h.prim == 0xFFFFFFFFu || sc.tri_geom[h.prim] != L.geom) continue;
+            const Ray r = nee_rays[c];
+            V3 t;
+            const bool on = k == 0 ? nee_term<true>(sc, L, sc.geoms[L.geom], h, v3(r.ox, r.oy, r.oz),
+                                                    v3(r.dx, r.dy, r.dz), v3(rk.x, rk.y, rk.z), rk.w, t)
+                                   : nee_term<false>(sc, L, sc.geoms[L.geom], h, v3(r.ox, r.oy, r.oz),
+                                                     v3(r.dx, r.dy, r.dz), v3(rk.x, rk.y, rk.z), rk.w, t);
+            if (on) direct = direct + t;
         }
         const float4 r0 = nee.rec[3 * (size_t)q];
         V3 T = v3(r0.x, r0.y, r0.z);
@@ -816,7 +828,7 @@ __global__ void k_reset_ctl(Ctl *ctl, uint32_t nlaunch) {
 /* Bytes of workspace per path in flight. */
 constexpr size_t kPathBytes = 16;                     /* final color */
 constexpr size_t kQueueBytes = sizeof(Ray) + 4 + 2 * 16 + 4;   /* per queue slot: ray, owner, state */
-constexpr size_t kNeeBytes = 3 * 16 + 4 + 2 * sizeof(Ray) + 4 * sizeof(Hit) + 2 * 4 + 2 * 4 + 2 * 4 + 4;
+constexpr size_t kNeeBytes = 3 * 16 + 4 + 2 * sizeof(Ray) + 2 * sizeof(Hit) + 2 * 4 + 2 * 4 + 2 * 4 + 4;
 constexpr size_t kBatchBytes = 4 + 2 * 4 + 2 * 4;       /* path_seg, queue sids, next-event sids */
 constexpr size_t kBytesPerPath = kPathBytes + 2 * kQueueBytes + sizeof(Hit) + kNeeBytes + kBatchBytes;
 
@@ -826,7 +838,6 @@ struct Workspace {
     Hit *hits;
     NeeRec nee;
     Ray *nee_rays;
-    Hit *nee_hits;
     uint32_t *nee_occl;      /* per compacted next-event ray */
     uint32_t *nee_ignore;    /* per compacted next-event ray: the light's geomID */
     Hit *nee_trace_hits;     /* per compacted next-event ray (closest-hit mode) */
@@ -858,7 +869,6 @@ int carve(lt::DevBuf &ws, size_t &ws_paths, size_t n, Workspace &w) {
     w.nee.rec = (float4 *)take(3 * sizeof(float4));
     w.nee.info = (uint32_t *)take(4);
     w.nee_rays = (Ray *)take(2 * sizeof(Ray));
-    w.nee_hits = (Hit *)take(2 * sizeof(Hit));
     w.nee_occl = (uint32_t *)take(2 * 4);
     w.nee_ignore = (uint32_t *)take(2 * 4);
     w.nee_trace_hits = (Hit *)take(2 * sizeof(Hit));
@@ -1092,7 +1102,7 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
                     ++ext_launches;
                     tm.begin(T_SHADE, L.st);
                     hipLaunchKernelGGL((k_shade<ADJ, BATCH>), dim3(sgrid), dim3(kShadeBlock), 0, L.st, A, b, cur[k],
-                                       (const Hit *)w.hits, nxt[k], w.nee, w.nee_rays, w.nee_hits, w.nee_ignore,
+                                       (const Hit *)w.hits, nxt[k], w.nee, w.nee_rays, w.nee_ignore,
                                        w.nee_cidx, w.nee_dest, ctl);
                     tm.end(T_SHADE, L.st);
                     LT_HIP(hipGetLastError());
@@ -1112,7 +1122,7 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
                     ++sh_launches;
                     tm.begin(T_RESOLVE, L.st);
                     hipLaunchKernelGGL((k_resolve<ADJ, BATCH>), dim3(sgrid), dim3(256), 0, L.st, A, b, cur[k], nxt[k],
-                                       w.nee, (const Ray *)w.nee_rays, (const Hit *)w.nee_hits,
+                                       w.nee, (const Ray *)w.nee_rays,
                                        (const Hit *)w.nee_trace_hits, (const uint32_t *)w.nee_occl,
                                        (const uint32_t *)w.nee_cidx, (const uint32_t *)w.nee_dest, ctl);
                     tm.end(T_RESOLVE, L.st);
