@@ -185,16 +185,39 @@ __device__ __forceinline__ float filter_sample(const CameraArgs &c, float x) {
     if (c.filter == 2) return x - 0.5f;
     bool neg = x < 0.5f;
     x = neg ? x * 2.0f : (x - 0.5f) * 2.0f;
-    uint32_t index = 32;
-    for (uint32_t i = 1; i < 33; ++i) {
-        if (x < c.cdf[i]) {
-            index = i - 1;
-            break;
-        }
+    /* the first i in 1..32 with x < cdf[i], as the reference's linear scan
+     * finds it: cdf rises up to cdf[31] and cdf[32] = 1 > x, so "x < cdf[i]"
+     * turns true once and stays true, and a binary search (5 steps instead
+     * of up to 32 divergent ones) lands on the same i */
+    uint32_t lo = 1u, hi = 32u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (x < c.cdf[mid]) hi = mid;
+        else lo = mid + 1u;
     }
+    const uint32_t index = lo - 1u;
     float pdf = c.cdf[index + 1] - c.cdf[index];
     float u = c.bin_size * ((float)index + ((x - c.cdf[index]) / pdf));
     return neg ? -u : u;
+}
+
+/* Random numbers a camera path has drawn when its first extension ray
+ * leaves (the two filter samples of k_generate). */
+constexpr uint32_t kCameraDraws = 2u;
+
+/* State of camera slot p at bounce 0 (slot p holds path p): throughput 1,
+ * IOR 1, radiance 0 (the adjoint replay: the recorded sample radiance),
+ * kCameraDraws draws consumed, no medium, the null lobe. */
+template <bool ADJ>
+__device__ __forceinline__ void initial_state(const WaveArgs &a, uint32_t p, float4 &s0, float4 &s1, uint32_t &ml) {
+    s0 = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    if (ADJ) {
+        const float *rad = a.adj.sample_rad + 3 * ((size_t)a.sample_base * a.npix + p);
+        s1 = make_float4(rad[0], rad[1], rad[2], __uint_as_float(kCameraDraws));
+    } else {
+        s1 = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kCameraDraws));
+    }
+    ml = pack_ml(-1, (uint32_t)LOBE_NULL);
 }
 
 template <bool ADJ, bool BATCH>
@@ -222,15 +245,8 @@ __global__ void k_generate(WaveArgs a, Queue q, uint32_t n, Ctl *ctl) {
         float4 *rp = (float4 *)(q.rays + p);
         rp[0] = r0;
         rp[1] = r1;
-        q.path[p] = p;
-        q.st[2 * (size_t)p] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-        if (ADJ) {
-            const float *rad = a.adj.sample_rad + 3 * ((size_t)a.sample_base * a.npix + p);
-            q.st[2 * (size_t)p + 1] = make_float4(rad[0], rad[1], rad[2], __uint_as_float(rng.i));
-        } else {
-            q.st[2 * (size_t)p + 1] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(rng.i));
-        }
-        q.ml[p] = pack_ml(-1, (uint32_t)LOBE_NULL);
+        /* the owner, state and medium/lobe of a camera slot are known:
+         * k_shade synthesizes them at bounce 0 (initial_state) */
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->count[0] = n;
 }
@@ -279,10 +295,12 @@ __device__ __forceinline__ V3 adj_weight(const WaveArgs &a, uint32_t pix, const 
  * type of the surface hit. */
 constexpr int kBins = 6;
 template <bool BATCH>
-__device__ __forceinline__ uint32_t shade_bin(const WaveArgs &a, const Queue &cur, const Hit *hits, uint32_t q) {
+__device__ __forceinline__ uint32_t shade_bin(const WaveArgs &a, const Queue &cur, const Hit *hits, uint32_t q,
+                                              uint32_t b) {
     const uint32_t prim = hits[q].prim;
     if (prim == 0xFFFFFFFFu) return 0u;
-    if (ml_medium(cur.ml[q]) >= 0) return 1u;
+    /* camera slots (bounce 0) start outside every medium */
+    if (b != 0 && ml_medium(cur.ml[q]) >= 0) return 1u;
     const DScene &sc = BATCH ? a.batch.scenes[cur.sid[q]] : a.sc;
     const int bsdf = sc.mats[sc.geoms[sc.tri_geom[prim]].material].bsdf;
     return 2u + (uint32_t)min(max(bsdf, 0), kBins - 3);
@@ -362,7 +380,7 @@ void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, Nee
 #pragma unroll
         for (int r = 0; r < kShadeRounds; ++r) {
             const uint32_t pos = r * 64 + lane;
-            key[r] = pos < nvalid ? (int)shade_bin<BATCH>(a, cur, hits, cbase + pos) : -1;
+            key[r] = pos < nvalid ? (int)shade_bin<BATCH>(a, cur, hits, cbase + pos, b) : -1;
             rank[r] = 0u;
             unsigned long long rem = __ballot(key[r] >= 0);
             while (rem) {
@@ -415,21 +433,26 @@ void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, Nee
             float4 nst0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);   /* next state of a live path */
             uint32_t nrng = 0u, nml = 0u;
             if (valid) {
-                p = cur.path[q];
+                p = b == 0 ? q : cur.path[q];
                 const PathInfo pi = path_info<BATCH>(a, p, BATCH ? a.batch.path_seg[p] : 0u);
                 const uint32_t x = pi.x, y = pi.y;
                 scene = pi.scene;
                 const DScene &sc = BATCH ? a.batch.scenes[scene] : a.sc;
                 const uint32_t pix = y * a.cam.w + x;
-                const float4 s0 = cur.st[2 * (size_t)q];       /* throughput, IOR */
-                const float4 s1 = cur.st[2 * (size_t)q + 1];   /* radiance, draws consumed */
+                float4 s0, s1;   /* (throughput, IOR), (radiance, draws consumed) */
+                uint32_t ml = 0u;
+                if (b == 0) {
+                    initial_state<ADJ>(a, q, s0, s1, ml);
+                } else {
+                    s0 = cur.st[2 * (size_t)q];
+                    s1 = cur.st[2 * (size_t)q + 1];
+                }
                 /* a hit's slot data, loaded together with the state */
                 Hit h;
-                uint32_t ml = 0u;
                 float4 ra = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rb = ra;
                 if (hit) {
                     h = hits[q];
-                    ml = cur.ml[q];
+                    if (b != 0) ml = cur.ml[q];
                     const float4 *rp = (const float4 *)(cur.rays + q);
                     ra = rp[0];
                     rb = rp[1];
