@@ -581,6 +581,16 @@ int lt_scene_commit(lt_scene *s, const lt_build_params *params) {
         (rc = s->d_lights.upload(s->lights.data(), s->lights.size() * sizeof(DLight), st)) ||
         (rc = s->d_light_tris.upload(s->light_tris.data(), s->light_tris.size() * 4, st)))
         return rc;
+    {
+        /* the shading bin of each triangle, from its material */
+        std::vector<uint8_t> bins(s->tri_geom.size());
+        for (size_t t = 0; t < bins.size(); ++t) {
+            const int32_t mi = s->geoms[s->tri_geom[t]].material;
+            bins[t] = (uint8_t)surface_bin(mi >= 0 && (size_t)mi < s->mats.size() ? s->mats[mi].bsdf : 0);
+        }
+        if ((rc = s->d_tri_bin.upload(bins.data(), bins.size(), st))) return rc;
+        LT_HIP(hipStreamSynchronize(st));
+    }
     auto t0 = std::chrono::steady_clock::now();
     uint32_t ntris = (uint32_t)s->tri_geom.size();
     if ((rc = build_bvh(st, s->d_pos.as<float>(), s->d_vidx.as<uint32_t>(), s->d_tri_geom.as<uint32_t>(),
@@ -594,6 +604,7 @@ int lt_scene_commit(lt_scene *s, const lt_build_params *params) {
     d.nodes = s->d_nodes.as<float4>();
     d.tris = s->d_tris.as<float4>();
     d.tri_geom = s->d_tri_geom.as<uint32_t>();
+    d.tri_bin = s->d_tri_bin.as<uint8_t>();
     d.tri_vidx = s->d_vidx.as<uint32_t>();
     d.normals = s->d_nrm.as<float>();
     d.uvs = s->d_uv.as<float>();

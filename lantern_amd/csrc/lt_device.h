@@ -301,7 +301,15 @@ struct DScene {
     uint32_t nlights;
     float bg[3];
     uint32_t light_first;      /* 1: every light has <= kLightFirstMaxTris triangles */
+    const uint8_t *tri_bin;    /* per global primitive: surface_bin of its material's BSDF */
 };
+
+/* Shading bins of k_shade's per-wave sort: 0 a miss, 1 inside a medium, and
+ * from 2 on a surface hit by BSDF type (types beyond the last share it). */
+constexpr int kShadeBins = 6;
+LT_HD uint32_t surface_bin(int bsdf) {
+    return 2u + (uint32_t)(bsdf < 0 ? 0 : bsdf > kShadeBins - 3 ? kShadeBins - 3 : bsdf);
+}
 
 /* Lights with at most this many triangles are intersected directly in the
  * shade kernel, which turns their next-event rays into occlusion queries. */

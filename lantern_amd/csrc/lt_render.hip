@@ -291,9 +291,9 @@ __device__ __forceinline__ V3 adj_weight(const WaveArgs &a, uint32_t pix, const 
     return v3(g[0] * C.x, g[1] * C.y, g[2] * C.z) * a.adj.inv_spp;
 }
 
-/* Shading bin of slot q: 0 a miss, 1 a path inside a medium, 2 + the BSDF
- * type of the surface hit. */
-constexpr int kBins = 6;
+/* Shading bin of slot q: 0 a miss, 1 a path inside a medium, else the
+ * surface bin of the triangle hit (DScene::tri_bin). */
+constexpr int kBins = kShadeBins;
 template <bool BATCH>
 __device__ __forceinline__ uint32_t shade_bin(const WaveArgs &a, const Queue &cur, const Hit *hits, uint32_t q,
                                               uint32_t b) {
@@ -302,8 +302,7 @@ __device__ __forceinline__ uint32_t shade_bin(const WaveArgs &a, const Queue &cu
     /* camera slots (bounce 0) start outside every medium */
     if (b != 0 && ml_medium(cur.ml[q]) >= 0) return 1u;
     const DScene &sc = BATCH ? a.batch.scenes[cur.sid[q]] : a.sc;
-    const int bsdf = sc.mats[sc.geoms[sc.tri_geom[prim]].material].bsdf;
-    return 2u + (uint32_t)min(max(bsdf, 0), kBins - 3);
+    return sc.tri_bin[prim];
 }
 
 /* The direct-light term of one next-event ray from o along d that reached
