@@ -233,9 +233,10 @@ typedef struct lt_render_params {
     int32_t use_graph;     /* 1: capture the bounce loop in a hipGraph and replay it */
     uint32_t trace_variant;/* 0 = automatic; 1 + variant of the 8-wide traversal kernel
                               (LDS stack depth / occupancy target, see the autotuner) */
-    uint32_t streams;      /* 0 = automatic; 1: one stream; 2: two waves side by side on two
-                              streams (traversal of one overlaps shading of the other; a
-                              second workspace of the same size). Films do not depend on it. */
+    uint32_t streams;      /* 0 = automatic (3 from 128M paths on, 2 from 16M, else 1);
+                              1-3: that many waves side by side on as many streams (traversal
+                              of one overlaps shading of another; a workspace per stream).
+                              Films do not depend on it. */
     uint32_t trace_share;  /* with two streams: percent (1-100) of the full traversal grid, the
                               rest of each CU left to the other stream's shading; 0 = automatic
                               (50 when a scene holds a grid medium, else 100) */

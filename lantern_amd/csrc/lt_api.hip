@@ -246,11 +246,14 @@ int lt_device_create(int ordinal, lt_device **out) {
         delete d;
         return hip_error(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
     }
-    e = hipStreamCreateWithFlags(&d->lane[1].stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        (void)hipStreamDestroy(d->stream);
-        delete d;
-        return hip_error(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+    for (int k = 1; k < kMaxLanes; ++k) {
+        e = hipStreamCreateWithFlags(&d->lane[k].stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            for (int j = 1; j < k; ++j) (void)hipStreamDestroy(d->lane[j].stream);
+            (void)hipStreamDestroy(d->stream);
+            delete d;
+            return hip_error(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+        }
     }
     int rc = d->trace_work.alloc(kWorkWords * sizeof(uint32_t));
     if (!rc && hipHostMalloc((void **)&d->pinned, 256, hipHostMallocDefault) != hipSuccess) {
@@ -261,7 +264,7 @@ int lt_device_create(int ordinal, lt_device **out) {
     if (!rc && hipMemset(d->diag.p, 0, 64) != hipSuccess) rc = set_error(LT_E_HIP, "hipMemset of the diagnostics word failed");
     if (rc) {
         if (d->pinned) (void)hipHostFree(d->pinned);
-        (void)hipStreamDestroy(d->lane[1].stream);
+        for (int k = 1; k < kMaxLanes; ++k) (void)hipStreamDestroy(d->lane[k].stream);
         (void)hipStreamDestroy(d->stream);
         delete d;
         return rc;
@@ -274,18 +277,20 @@ int lt_device_destroy(lt_device *d) {
     if (!d) return LT_OK;
     (void)hipSetDevice(d->ordinal);
     (void)hipStreamSynchronize(d->stream);
-    (void)hipStreamSynchronize(d->lane[1].stream);
+    for (int k = 1; k < kMaxLanes; ++k) (void)hipStreamSynchronize(d->lane[k].stream);
     if (d->graph_exec) (void)hipGraphExecDestroy((hipGraphExec_t)d->graph_exec);
     d->ws.release();
     d->ctl.release();
     d->trace_work.release();
     d->spill.release();
     d->diag.release();
-    d->lane[1].ws.release();
-    d->lane[1].ctl.release();
-    d->lane[1].spill.release();
+    for (int k = 1; k < kMaxLanes; ++k) {
+        d->lane[k].ws.release();
+        d->lane[k].ctl.release();
+        d->lane[k].spill.release();
+        (void)hipStreamDestroy(d->lane[k].stream);
+    }
     if (d->pinned) (void)hipHostFree(d->pinned);
-    (void)hipStreamDestroy(d->lane[1].stream);
     (void)hipStreamDestroy(d->stream);
     delete d;
     return LT_OK;

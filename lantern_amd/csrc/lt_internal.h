@@ -83,6 +83,8 @@ constexpr int kDefaultTraceVariant = 4;
 constexpr int kWorkCounters = 8;
 constexpr int kWorkStride = 32;
 constexpr int kWorkWords = kWorkCounters * kWorkStride;
+/* Streams a render may run its waves on (lt_render_params.streams). */
+constexpr int kMaxLanes = 3;
 /* Default paths in flight per render wave (lt_render_params.wave_paths 0). */
 constexpr size_t kDefaultWavePaths = (size_t)64 << 20;
 int launch_trace_closest(hipStream_t st, const DScene &sc, const Ray *rays, Hit *hits, const uint32_t *count_ptr,
@@ -140,14 +142,14 @@ struct lt_device {
     lt::DevBuf spill;        /* traversal stack overflow area */
     lt::DevBuf diag;         /* [0]: rays that hit a traversal bound (stack/steps) */
     lt::DevBuf batch_tables; /* scene, camera, film and segment tables of a batch render */
-    /* the second stream of a render that runs two waves side by side, with
-     * its own workspace, control block and traversal spill area (lane[0] is
-     * unused: the first lane is stream/ws/ctl/spill above) */
+    /* the further streams of a render that runs waves side by side, each
+     * with its own workspace, control block and traversal spill area (lane[0]
+     * is unused: the first lane is stream/ws/ctl/spill above) */
     struct Lane {
         hipStream_t stream = nullptr;
         lt::DevBuf ws, ctl, spill;
         size_t ws_paths = 0;
-    } lane[2];
+    } lane[lt::kMaxLanes];
     /* the last captured launch sequence, replayed while its arguments repeat */
     void *graph_exec = nullptr;
     std::string graph_key;

@@ -118,17 +118,17 @@ def test_quad_scene_matches_oracle(gpu_device):
     _check(g, o)
 
 
-def test_two_streams_give_identical_images(gpu_device):
-    """Two waves side by side on two streams accumulate into the film in
-    wave order, so the film equals the one-stream render's bit for bit, for
-    an even and an odd number of waves, a reduced traversal grid and the
-    batch entry point."""
+def test_streams_give_identical_images(gpu_device):
+    """Waves side by side on two or three streams accumulate into the film
+    in wave order, so the film equals the one-stream render's bit for bit,
+    for an even and an odd number of waves, a reduced traversal grid, the
+    batch entry point and graphs captured over several streams."""
     d = scenes.triangle_heavy(128, 72, 6)
     gs = lantern_amd.Scene(gpu_device, d)
     film = lantern_amd.Film(gpu_device, 128, 72)
     for wave_paths in (0, 128 * 72 * 2, 128 * 72 * 3):
         outs = []
-        for streams, share in ((1, 0), (2, 0), (2, 35)):
+        for streams, share in ((1, 0), (2, 0), (2, 35), (3, 0)):
             film.reset()
             st = lantern_amd.render(gs, d.camera, dict(d.params, streams=streams, wave_paths=wave_paths,
                                                        trace_share=share), film)
@@ -145,7 +145,7 @@ def test_two_streams_give_identical_images(gpu_device):
     films = [lantern_amd.Film(gpu_device, c["width"], c["height"]) for c in cams]
     # eager on one and two streams, then a graph captured over two streams
     # and replayed three times (films reset by eager work in between)
-    for streams, graph in ((1, 0), (2, 0), (2, 1), (2, 1), (2, 1)):
+    for streams, graph in ((1, 0), (2, 0), (3, 0), (2, 1), (2, 1), (2, 1), (3, 1)):
         for f in films:
             f.reset()
         st = lantern_amd.render_batch(ss, cams, dict(descs[0].params, spp=3, streams=streams, wave_paths=5000,
