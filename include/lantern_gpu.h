@@ -88,7 +88,8 @@ typedef struct lt_medium {
 } lt_medium;
 
 typedef struct lt_build_params {
-    uint32_t max_leaf_size;   /* 1..4, default 4 (Embree's sahBlockSize/minLeaf is 4) */
+    uint32_t max_leaf_size;   /* 1..4; 0 = automatic: 1 for the 8-wide layout (scenes of more
+                                 than 64 triangles), else 4 (Embree's sahBlockSize/minLeaf) */
     uint32_t layout;          /* 0: 8-wide compressed nodes (default), 1: binary nodes */
     uint32_t builder;         /* 0: PLOC clustering (default), 1: linear BVH (Karras) */
     uint32_t wide_collapse;   /* binary tree -> 8-wide nodes: 0 SAH-optimal cut (default), 1 greedy by area */

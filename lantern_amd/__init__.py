@@ -51,7 +51,7 @@ class Device:
 class Scene:
     """lt_scene built from a SceneDesc (Scene::LoadSceneFromJSON + rtcCommitScene)."""
 
-    def __init__(self, device, desc=None, max_leaf_size=4, layout=0, builder=0, wide_collapse=0):
+    def __init__(self, device, desc=None, max_leaf_size=0, layout=0, builder=0, wide_collapse=0):
         self.dev = device
         self.h = C.c_void_p()
         check(lib().lt_scene_create(device.h, C.byref(self.h)), "lt_scene_create")
@@ -103,7 +103,7 @@ class Scene:
                       "lt_scene_set_emission")
         check(L.lt_scene_set_background(self.h, (C.c_float * 3)(*d.background)), "lt_scene_set_background")
 
-    def commit(self, max_leaf_size=4, layout=0, builder=0, wide_collapse=0):
+    def commit(self, max_leaf_size=0, layout=0, builder=0, wide_collapse=0):
         bp = _capi.lt_build_params()
         bp.max_leaf_size = max_leaf_size
         bp.layout = layout

@@ -1179,9 +1179,12 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
     };
 
     hipStream_t st = lanes[0].st;
-    hipEvent_t e0, e1;
-    LT_HIP(hipEventCreate(&e0));
-    LT_HIP(hipEventCreate(&e1));
+    /* start and end of the whole sequence (timed) */
+    hipEvent_t ends[kMaxLanes] = {};
+    EvGuard ends_guard{ends};
+    LT_HIP(hipEventCreate(&ends[0]));
+    LT_HIP(hipEventCreate(&ends[1]));
+    const hipEvent_t e0 = ends[0], e1 = ends[1];
     if (graph) {
         /* replay the cached graph when every kernel argument is the same */
         std::string key((const char *)&a, sizeof a);
@@ -1247,7 +1250,12 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
     } else {
         LT_HIP(hipEventRecord(e0, st));
         for (int k = 1; k < nlanes; ++k) LT_HIP(hipStreamWaitEvent(lanes[k].st, e0, 0));
-        if ((rc = enqueue(true))) return rc;
+        if ((rc = enqueue(true))) {
+            /* what was enqueued still runs: let every lane finish before the
+             * buffers it uses can be reused */
+            for (int k = 0; k < nlanes; ++k) (void)hipStreamSynchronize(lanes[k].st);
+            return rc;
+        }
         /* join the other lanes into the first before the end event */
         for (int k = 1; k < nlanes; ++k) {
             LT_HIP(hipEventRecord(acc_ev[k], lanes[k].st));
@@ -1258,8 +1266,6 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
     LT_HIP(hipEventSynchronize(e1));
     float total = 0;
     LT_HIP(hipEventElapsedTime(&total, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if ((rc = check_traversal_bounds(dev))) return rc;
     if (stats) {
         unsigned long long tally[2] = {0, 0};
