@@ -156,6 +156,44 @@ __device__ __forceinline__ uint32_t seg_of(const BatchArgs &b, uint32_t p) {
 }
 
 /* Pixel, frame and scene of path p (seg: its segment in a batch). */
+/* Order of the w x h pixels of one sample among the path ids of a batch
+ * segment: 8 x 8 tiles, tile after tile along each band of 8 rows (a band's
+ * last, narrower tile row by row; the last, lower band row by row), so that
+ * the 64 lanes of a wave trace the camera rays of one tile. tile_index is
+ * the inverse of tile_pixel. A path's result does not depend on its id (its
+ * random numbers are keyed by pixel and frame), so neither does the film.
+ * Measured: the 64-scene batch (images of 64 to 512 pixels a side) 4 %
+ * faster than with rows; a single 1280 x 720 render 1-5 % slower, so
+ * single-scene renders keep rows. */
+__device__ __forceinline__ void tile_pixel(uint32_t i, uint32_t w, uint32_t h, uint32_t &x, uint32_t &y) {
+    const uint32_t band = i / (8u * w), fb = h >> 3;
+    if (band < fb) {
+        const uint32_t r = i - band * 8u * w, t = r >> 6, ft = w >> 3;
+        if (t < ft) {
+            x = (t << 3) + (r & 7u);
+            y = (band << 3) + ((r >> 3) & 7u);
+        } else {
+            const uint32_t wt = w - (ft << 3), r2 = r - (ft << 6);
+            const uint32_t ly = r2 / wt;
+            x = (ft << 3) + (r2 - ly * wt);
+            y = (band << 3) + ly;
+        }
+    } else {
+        const uint32_t r = i - fb * 8u * w;
+        const uint32_t ly = r / w;
+        x = r - ly * w;
+        y = (fb << 3) + ly;
+    }
+}
+__device__ __forceinline__ uint32_t tile_index(uint32_t x, uint32_t y, uint32_t w, uint32_t h) {
+    const uint32_t band = y >> 3, fb = h >> 3;
+    if (band >= fb) return fb * 8u * w + (y - (fb << 3)) * w + x;
+    const uint32_t t = x >> 3, ft = w >> 3;
+    if (t < ft) return band * 8u * w + (t << 6) + ((y & 7u) << 3) + (x & 7u);
+    const uint32_t wt = w - (ft << 3);
+    return band * 8u * w + (ft << 6) + (y & 7u) * wt + (x - (ft << 3));
+}
+
 template <bool BATCH>
 __device__ __forceinline__ PathInfo path_info(const WaveArgs &a, uint32_t p, uint32_t seg) {
     PathInfo r;
@@ -163,9 +201,8 @@ __device__ __forceinline__ PathInfo path_info(const WaveArgs &a, uint32_t p, uin
         const Seg g = a.batch.segs[seg];
         uint32_t local = p - g.path_off;
         uint32_t s = local / g.npix;
-        uint32_t pix = local - s * g.npix;
-        r.y = pix / g.w;
-        r.x = pix - r.y * g.w;
+        uint32_t i = local - s * g.npix;
+        tile_pixel(i, g.w, g.npix / g.w, r.x, r.y);
         r.frame = a.frame0 + g.s0 + s;
         r.scene = g.scene;
     } else {
@@ -803,8 +840,10 @@ __global__ void k_accumulate_batch(WaveArgs a) {
         const FilmPtrs f = a.batch.films[g.scene];
         float sx = f.sum[3 * (size_t)pix], sy = f.sum[3 * (size_t)pix + 1], sz = f.sum[3 * (size_t)pix + 2];
         uint32_t bb = f.bounces[pix], cc = f.count[pix];
+        const uint32_t y = pix / g.w;
+        const uint32_t i = tile_index(pix - y * g.w, y, g.w, g.npix / g.w);
         for (uint32_t s = 0; s < g.ns; ++s) {
-            size_t p = (size_t)g.path_off + (size_t)s * g.npix + pix;
+            size_t p = (size_t)g.path_off + (size_t)s * g.npix + i;
             const float4 col = a.ps.col[p];
             sx = sx + col.x;
             sy = sy + col.y;
