@@ -234,8 +234,7 @@ typedef struct lt_render_params {
     int32_t use_graph;     /* 1: capture the bounce loop in a hipGraph and replay it */
     uint32_t trace_variant;/* 0 = automatic; 1 + variant of the 8-wide traversal kernel
                               (LDS stack depth / occupancy target, see the autotuner) */
-    uint32_t streams;      /* 0 = automatic (3 from 128M paths on, 2 from 16M or, for scenes
-                              with a grid medium, from 2M, else 1);
+    uint32_t streams;      /* 0 = automatic (3 from 128M paths on, 2 from 2M, else 1);
                               1-3: that many waves side by side on as many streams (traversal
                               of one overlaps shading of another; a workspace per stream).
                               Films do not depend on it. */

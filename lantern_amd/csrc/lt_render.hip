@@ -1351,19 +1351,15 @@ size_t wave_budget(lt_device *dev, const lt_render_params &prm, int nlanes) {
 
 /* Streams a render runs on (lt_render_params.streams), at most one per unit
  * of work that a wave can hold (a sample of a scene). 0 picks two for
- * renders of at least kTwoLanePaths paths (kTwoLanePathsMedia when a scene
- * holds a grid medium, whose shading is long) and three from
- * kThreeLanePaths on. Measured: a 4M-path surface render loses on two
- * streams (857 -> 774 Msamples/s) while a 4M-path iteration of the
- * differentiable smoke scene gains (7.5 -> 6.6 ms); 59M and 67M paths gain
- * on two (+12 %, +11 %) and lose a little on three against two; the
- * 200M-path batch gains 4 % on three against two. */
-constexpr size_t kTwoLanePaths = (size_t)16 << 20;
-constexpr size_t kTwoLanePathsMedia = (size_t)2 << 20;
+ * renders of at least kTwoLanePaths paths and three from kThreeLanePaths
+ * on. Measured: 4M-path renders gain on two streams (nine-sphere 998 ->
+ * 1044 Msamples/s, the differentiable smoke iteration 7.5 -> 6.6 ms); 59M
+ * and 67M paths gain on two (+12 %, +11 %) and lose a little on three
+ * against two; the 200M-path batch gains 4 % on three against two. */
+constexpr size_t kTwoLanePaths = (size_t)2 << 20;
 constexpr size_t kThreeLanePaths = (size_t)128 << 20;
-int lane_count(const lt_render_params &prm, size_t paths, uint32_t units, bool grid_media) {
-    const size_t two = grid_media ? kTwoLanePathsMedia : kTwoLanePaths;
-    const uint32_t n = prm.streams ? prm.streams : paths >= kThreeLanePaths ? 3u : paths >= two ? 2u : 1u;
+int lane_count(const lt_render_params &prm, size_t paths, uint32_t units) {
+    const uint32_t n = prm.streams ? prm.streams : paths >= kThreeLanePaths ? 3u : paths >= kTwoLanePaths ? 2u : 1u;
     return (int)std::max<uint32_t>(1u, std::min<uint32_t>(n, units));
 }
 
@@ -1408,7 +1404,7 @@ int render_impl(lt_scene *scene, const lt_camera &cam, const lt_render_params &p
     if (npix == 0 || prm.spp == 0) return LT_OK;
     int rc = check_params(prm);
     if (rc) return rc;
-    const int nlanes = lane_count(prm, (size_t)prm.spp * npix, prm.spp, !scene->grids.empty());
+    const int nlanes = lane_count(prm, (size_t)prm.spp * npix, prm.spp);
     const size_t budget = wave_budget(dev, prm, nlanes);
     uint32_t spw = (uint32_t)std::max<size_t>(1, std::min<size_t>(prm.spp, budget / npix));
     if (nlanes > 1) {
@@ -1489,7 +1485,7 @@ int render_batch(lt_scene *const *scenes, const lt_camera *cams, lt_film *const 
     for (uint32_t i = 0; i < nscenes; ++i) total += (size_t)cams[i].width * cams[i].height * prm.spp;
     bool any_grid = false;
     for (uint32_t i = 0; i < nscenes; ++i) any_grid = any_grid || !scenes[i]->grids.empty();
-    const int nlanes = lane_count(prm, total, kMaxLanes, any_grid);
+    const int nlanes = lane_count(prm, total, kMaxLanes);
     size_t budget = wave_budget(dev, prm, nlanes);
     /* several lanes: at least one wave each */
     if (nlanes > 1) budget = std::min(budget, (total + nlanes - 1) / nlanes);
