@@ -13,10 +13,16 @@
  *                 random decisions of SampleOneLight/EstimateDirect,
  *                 the continuation BSDF sample and Russian roulette integrator.cpp:119-242, 252-315
  *                 -> compacted queue of surviving paths (ballot + one atomic per wave)
- *                 -> two next-event rays per shaded path (light sample, BSDF sample)
- *     extend      closest hit of the next-event rays               area_light.cpp:58, 100
+ *                 -> two next-event rays per shaded path (light sample, BSDF sample);
+ *                    for small lights their hit and term are known here already
+ *     extend      occlusion (small lights) or closest hit of the
+ *                 next-event rays                                  area_light.cpp:58, 100
  *     resolve     light pdfs at the hits, MIS weights, color += T * N * direct
  *   accumulate    film += path colors in sample order               integrator.cpp:245-249
+ *
+ * The paths of a render run as waves of up to wave_paths paths; two or
+ * three waves at a time run side by side on as many streams ("lanes", each
+ * with its own workspace), and the films are accumulated in wave order.
  *
  * Every random decision of a path is drawn in the order RenderPixel draws it,
  * from the path's own counter-based stream (Philox keyed by seed and frame,
