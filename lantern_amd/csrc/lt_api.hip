@@ -618,6 +618,7 @@ int lt_scene_commit(lt_scene *s, const lt_build_params *params) {
     d.tris = s->d_tris.as<float4>();
     d.tri_geom = s->d_tri_geom.as<uint32_t>();
     d.tri_bin = s->d_tri_bin.as<uint8_t>();
+    d.slot_tris = (layout == 0 && max_leaf == 1) ? 1u : 0u;
     d.tri_vidx = s->d_vidx.as<uint32_t>();
     d.normals = s->d_nrm.as<float>();
     d.uvs = s->d_uv.as<float>();

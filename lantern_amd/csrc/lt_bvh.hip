@@ -489,6 +489,34 @@ __global__ void k_wide_pass(WideArgs w) {
         area[n] = cc.y >= 0 ? a.bhi[cc.y].w : 0.0f;
         ++n;
     }
+    /* leaf children first, in a stable partition: the order among the leaves
+     * and among the internal children is kept, so a ray tests the same
+     * triangles and visits the same nodes in the same order as without it.
+     * With single-triangle leaves, leaf slot j then holds triangle
+     * tri_base + j (the traversal's slot-indexed triangle mask). */
+    {
+        int ids2[8];
+        float area2[8];
+        bool leafy2[8];
+        int m = 0;
+        for (int j = 0; j < n; ++j)
+            if (leafy[j]) {
+                ids2[m] = ids[j];
+                area2[m] = area[j];
+                leafy2[m++] = true;
+            }
+        for (int j = 0; j < n; ++j)
+            if (!leafy[j]) {
+                ids2[m] = ids[j];
+                area2[m] = area[j];
+                leafy2[m++] = false;
+            }
+        for (int j = 0; j < n; ++j) {
+            ids[j] = ids2[j];
+            area[j] = area2[j];
+            leafy[j] = leafy2[j];
+        }
+    }
     int nint = 0, ntri = 0;
     for (int j = 0; j < n; ++j) {
         if (leafy[j]) {
@@ -853,7 +881,11 @@ int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const 
     uint32_t nnodes = n > 1 ? n - 1 : 1;
     if ((rc = nodes.alloc(sizeof(float4) * 4 * (size_t)nnodes))) return rc;
     LT_HIP(hipMemsetAsync(nodes.p, 0, sizeof(float4) * 4 * (size_t)nnodes, st));
-    if ((rc = wtris.alloc(sizeof(float4) * 3 * (size_t)n))) return rc;
+    /* 8 zero (degenerate, never hit) triangles past the end: a node's slot
+     * index added to its triangle base stays inside the array even for a
+     * slot that no triangle fills (the slot-indexed traversal mask) */
+    if ((rc = wtris.alloc(sizeof(float4) * 3 * ((size_t)n + 8)))) return rc;
+    LT_HIP(hipMemsetAsync((float4 *)wtris.p + 3 * (size_t)n, 0, sizeof(float4) * 3 * 8, st));
     if ((rc = wnodes.alloc(sizeof(float4) * 5 * (size_t)std::max<uint32_t>(1u, n - 1)))) return rc;
     if (n == 1) {
         hipLaunchKernelGGL(k_write_tris, dim3(1), dim3(B), 0, st, d_pos, d_vidx, d_tri_geom, d_tri_flags, sk, n,

@@ -302,6 +302,8 @@ struct DScene {
     float bg[3];
     uint32_t light_first;      /* 1: every light has <= kLightFirstMaxTris triangles */
     const uint8_t *tri_bin;    /* per global primitive: surface_bin of its material's BSDF */
+    uint32_t slot_tris;        /* 8-wide layout with single-triangle leaves, leaf children first:
+                                  leaf slot j of a node holds triangle tri_base + j */
 };
 
 /* Shading bins of k_shade's per-wave sort: 0 a miss, 1 inside a medium, and

@@ -1483,8 +1483,14 @@ int render_batch(lt_scene *const *scenes, const lt_camera *cams, lt_film *const 
         std::memcpy(&cam_tbl[i], &cs, sizeof(CameraArgs));
         film_tbl[i] = {films[i]->sum.as<float>(), films[i]->bounces.as<uint32_t>(), films[i]->count.as<uint32_t>()};
     }
-    /* the next-event query kind is per launch: all scenes use light-first or none */
-    for (DScene &d : tbl) d.light_first = light_first ? 1u : 0u;
+    /* the next-event query kind is per launch: all scenes use light-first or
+     * none; likewise the slot-indexed triangle mask of the traversal */
+    bool slot_tris = true;
+    for (const DScene &d : tbl) slot_tris = slot_tris && d.slot_tris;
+    for (DScene &d : tbl) {
+        d.light_first = light_first ? 1u : 0u;
+        d.slot_tris = slot_tris ? 1u : 0u;
+    }
 
     /* plan: pack whole samples of each scene into waves */
     size_t total = 0;

@@ -347,7 +347,7 @@ LT_D void trav8_end(const Trav8 &s, Hit *__restrict__ hits, uint32_t *__restrict
  * trav8_next, once they are all tested, moves to the nearest child (unless
  * it lies beyond the shrunk tfar) or pops. Each returns true when the ray is
  * finished. */
-template <bool ANY, int BLOCK, int LDS8>
+template <bool ANY, int BLOCK, int LDS8, bool SLOT>
 LT_D bool trav8_boxes(Trav8 &s, int *lds_node, float *lds_t, int tid, int2 *gstack, uint32_t *overflow) {
     /* at most 7 pushes per node: stop before the stack could overflow */
     if (++s.steps > MAX_STEPS || s.sp > STACK8_CAP - 7) {
@@ -413,7 +413,14 @@ LT_D bool trav8_boxes(Trav8 &s, int *lds_node, float *lds_t, int tid, int2 *gsta
         const bool hit = tn0 <= __builtin_fmaf(fabsf(tf0), kSlack, tf0);
         /* lower bound of the entry distance, for ordering and culling on pop */
         const float tn = __builtin_fmaf(fabsf(tn0), -kSlack, tn0);
-        tmask |= (hit && !isint) ? ((1u << (meta >> 5)) - 1u) << (meta & 31u) : 0u;
+        /* SLOT: leaf slot j holds triangle tbase + j (single-triangle leaves,
+         * leaf slots first), so the mask is the slot's hit bit. An unused
+         * slot's inverted box is not entered except where a node is too
+         * small against the ray's rounding bounds; its bit then names a real
+         * triangle of another node or one of the zero triangles past the
+         * end, and a test of a real triangle only ever finds real hits. */
+        if (SLOT) tmask |= hit ? (1u << j) : 0u;
+        else tmask |= (hit && !isint) ? ((1u << (meta >> 5)) - 1u) << (meta & 31u) : 0u;
         if (hit && isint) {
             const int child = (int)(cbase + meta);
             const bool closer = tn < near_t;
@@ -433,7 +440,7 @@ LT_D bool trav8_boxes(Trav8 &s, int *lds_node, float *lds_t, int tid, int2 *gsta
         }
     }
     s.sp = sp;
-    s.tmask = tmask;
+    s.tmask = SLOT ? tmask & ~imask : tmask;
     s.tbase = tbase;
     s.near_node = near_node;
     s.near_t = near_t;
@@ -497,9 +504,9 @@ LT_D bool trav8_next(Trav8 &s, int *lds_node, float *lds_t, int tid, int2 *gstac
  * next node). The triangles are tested after the box tests, one per lane per
  * iteration: lanes that hit leaves in different slots test them together,
  * rather than one slot at a time with the other lanes idle. */
-template <bool ANY, int BLOCK, int LDS8>
+template <bool ANY, int BLOCK, int LDS8, bool SLOT>
 LT_D bool trav8_visit(Trav8 &s, int *lds_node, float *lds_t, int tid, int2 *gstack, uint32_t *overflow) {
-    if (trav8_boxes<ANY, BLOCK, LDS8>(s, lds_node, lds_t, tid, gstack, overflow)) return true;
+    if (trav8_boxes<ANY, BLOCK, LDS8, SLOT>(s, lds_node, lds_t, tid, gstack, overflow)) return true;
     while (s.tmask)
         if (trav8_tri<ANY>(s)) return true;
     return trav8_next<BLOCK, LDS8>(s, lds_node, lds_t, tid, gstack);
@@ -519,7 +526,7 @@ LT_D bool trav8_visit(Trav8 &s, int *lds_node, float *lds_t, int tid, int2 *gsta
  * triangle-heavy scene 5%; deferring triangle tests until half the lanes
  * have some (speculative traversal, so that box and triangle steps run with
  * fuller waves) cost it 14%, as tfar then shrinks later. */
-template <bool ANY, int BLOCK, int LDS8, int WAVES, int REFILL, bool BATCH>
+template <bool ANY, int BLOCK, int LDS8, int WAVES, int REFILL, bool BATCH, bool SLOT>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVES : 1)))
 void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__restrict__ hits,
               uint32_t *__restrict__ occluded, const uint32_t *ignore_geom, const uint32_t *count_ptr,
@@ -557,7 +564,7 @@ void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__
                 }
                 next = min(end, next + nidle);
             }
-            if (have && trav8_visit<ANY, BLOCK, LDS8>(s, lds_node, lds_t, tid, gstack, overflow)) {
+            if (have && trav8_visit<ANY, BLOCK, LDS8, SLOT>(s, lds_node, lds_t, tid, gstack, overflow)) {
                 trav8_end<ANY>(s, hits, occluded);
                 have = false;
             }
@@ -569,7 +576,7 @@ void k_trace8(DScene sc, TraceBatch batch, const Ray *__restrict__ rays, Hit *__
             const uint32_t i = base + r + lane;
             if (i >= feed.end) break;
             if (trav8_begin<ANY, BATCH>(s, sc, batch, rays, ignore_geom, i))
-                while (!trav8_visit<ANY, BLOCK, LDS8>(s, lds_node, lds_t, tid, gstack, overflow)) {
+                while (!trav8_visit<ANY, BLOCK, LDS8, SLOT>(s, lds_node, lds_t, tid, gstack, overflow)) {
                 }
             trav8_end<ANY>(s, hits, occluded);
         }
@@ -583,14 +590,18 @@ int launch_b(hipStream_t st, const DScene &sc, const TraceBatch &tb, const Ray *
     if (sc.layout == 0) {
         int2 *sp2 = (int2 *)spill;
         const int v = variant;
-#define LT_T8(B, L, W, R) hipLaunchKernelGGL((k_trace8<ANY, B, L, W, R, BATCH>), dim3(grid), dim3(B), 0, st, sc, tb, rays, hits, occl, ign, count_ptr, count, work, sp2, overflow)
-#define LT_T8B(B)                             \
-        if (v == 0) LT_T8(B, 16, 0, 0);       \
-        else if (v == 1) LT_T8(B, 8, 0, 0);   \
-        else if (v == 2) LT_T8(B, 8, 6, 0);   \
-        else if (v == 3) LT_T8(B, 8, 8, 0);   \
-        else if (v == 4) LT_T8(B, 16, 0, 16); \
-        else LT_T8(B, 16, 0, 8);
+#define LT_T8(B, L, W, R, S) hipLaunchKernelGGL((k_trace8<ANY, B, L, W, R, BATCH, S>), dim3(grid), dim3(B), 0, st, sc, tb, rays, hits, occl, ign, count_ptr, count, work, sp2, overflow)
+        /* the slot-indexed triangle mask only for the default variant; the
+         * others read the general leaf ranges (which also hold for it) */
+        const bool slot = sc.slot_tris != 0;
+#define LT_T8B(B)                                              \
+        if (v == 0) LT_T8(B, 16, 0, 0, false);                 \
+        else if (v == 1) LT_T8(B, 8, 0, 0, false);             \
+        else if (v == 2) LT_T8(B, 8, 6, 0, false);             \
+        else if (v == 3) LT_T8(B, 8, 8, 0, false);             \
+        else if (v == 4 && slot) LT_T8(B, 16, 0, 16, true);    \
+        else if (v == 4) LT_T8(B, 16, 0, 16, false);           \
+        else LT_T8(B, 16, 0, 8, false);
         switch (block) {
         case 64: LT_T8B(64); break;
         case 128: LT_T8B(128); break;
