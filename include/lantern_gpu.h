@@ -276,7 +276,7 @@ int lt_render(lt_scene *scene, const lt_camera *camera, const lt_render_params *
 int lt_render_batch(lt_scene *const *scenes, const lt_camera *cameras, lt_film *const *films, uint32_t n,
                     const lt_render_params *params, lt_render_stats *stats);
 /* Autotuner (no reference counterpart): times short probe renders of the
- * batch over traversal block sizes {64, 128, 256} and kernel variants 1-4
+ * batch over traversal block sizes {64, 128, 256} and kernel variants 1-7
  * (LDS stack depth 16 or 8, register/occupancy targets); with try_layouts
  * also both node widths (8-wide and binary; the scenes are re-committed and
  * left in the fastest). Films are reset after. The wave size is not probed

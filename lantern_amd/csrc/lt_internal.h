@@ -67,16 +67,16 @@ int build_bvh(hipStream_t st, const float *d_pos, const uint32_t *d_vidx, const 
 /* Traversal kernel variants of the 8-wide layout (LDS stack depth,
  * occupancy target, lane refill threshold): 0 = (16, compiler, -),
  * 1 = (8, compiler, -), 2 = (8, 6 waves, -), 3 = (8, 8 waves, -),
- * 4 = (16, compiler, 16 idle lanes), 5 = (16, compiler, 8 idle lanes).
- * The binary layout ignores the variant. */
-constexpr int kTraceVariants = 6;
+ * 4 = (16, compiler, 16 idle lanes), 5 = (16, compiler, 8 idle lanes),
+ * 6 = (8, 6 waves, 16 idle lanes). The binary layout ignores the variant. */
+constexpr int kTraceVariants = 7;
 /* Rays of several scenes in one traversal launch: ray i belongs to scene
  * ray_scene[i] of the device table scenes (all with the same layout). */
 struct TraceBatch {
     const DScene *scenes;
     const uint32_t *ray_scene;
 };
-constexpr int kDefaultTraceVariant = 4;
+constexpr int kDefaultTraceVariant = 6;
 /* Work counters of one traversal launch: kWorkCounters counters (one per
  * XCD of the MI355X), kWorkStride words apart so that each sits in a cache
  * line of its own; all zero at launch. */

@@ -591,7 +591,7 @@ int launch_b(hipStream_t st, const DScene &sc, const TraceBatch &tb, const Ray *
         int2 *sp2 = (int2 *)spill;
         const int v = variant;
 #define LT_T8(B, L, W, R, S) hipLaunchKernelGGL((k_trace8<ANY, B, L, W, R, BATCH, S>), dim3(grid), dim3(B), 0, st, sc, tb, rays, hits, occl, ign, count_ptr, count, work, sp2, overflow)
-        /* the slot-indexed triangle mask only for the default variant; the
+        /* the slot-indexed triangle mask only for variants 4 and 6; the
          * others read the general leaf ranges (which also hold for it) */
         const bool slot = sc.slot_tris != 0;
 #define LT_T8B(B)                                              \
@@ -601,6 +601,8 @@ int launch_b(hipStream_t st, const DScene &sc, const TraceBatch &tb, const Ray *
         else if (v == 3) LT_T8(B, 8, 8, 0, false);             \
         else if (v == 4 && slot) LT_T8(B, 16, 0, 16, true);    \
         else if (v == 4) LT_T8(B, 16, 0, 16, false);           \
+        else if (v == 6 && slot) LT_T8(B, 8, 6, 16, true);     \
+        else if (v == 6) LT_T8(B, 8, 6, 16, false);            \
         else LT_T8(B, 16, 0, 8, false);
         switch (block) {
         case 64: LT_T8B(64); break;
@@ -644,7 +646,7 @@ int trace_default_block() { return 128; }
  * (binary: LDS_STACK x 4 B, 8-wide: LDS8 x 8 B per lane, of 160 KB) and by
  * 32 waves per CU. */
 int trace_grid(int num_cus, int block, int variant) {
-    const int lds8 = (variant == 0 || variant >= 4) ? 16 : 8;
+    const int lds8 = (variant == 0 || variant == 4 || variant == 5) ? 16 : 8;
     const int lds_per_lane = std::max(LDS_STACK * 4, lds8 * 8);
     const int by_lds = (160 * 1024) / (lds_per_lane * block);
     const int by_waves = std::max(1, 32 / (block / 64));

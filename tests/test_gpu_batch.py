@@ -113,16 +113,16 @@ def test_autotune_picks_a_valid_config_with_identical_results(gpu_device):
     params = dict(descs[0].params)
     lantern_amd.render_batch(ss, cams, params, fa)
     tune = lantern_amd.autotune(ss, cams, params, fb)
-    assert tune["candidates"] == 18                          # 3 block sizes x 6 kernel variants
+    assert tune["candidates"] == 21                          # 3 block sizes x 7 kernel variants
     assert tune["wave_paths"] == params.get("wave_paths", 0)  # the wave size is passed through
-    assert tune["block_size"] in (64, 128, 256) and 1 <= tune["trace_variant"] <= 6
+    assert tune["block_size"] in (64, 128, 256) and 1 <= tune["trace_variant"] <= 7
     assert tune["best_ms"] <= tune["default_ms"]
     assert all(int(f.download()[2].max()) == 0 for f in fb)   # films reset after tuning
     lantern_amd.render_batch(ss, cams, lantern_amd.tuned_params(params, tune), fb)
     for i in range(len(descs)):
         assert np.array_equal(fa[i].download()[0], fb[i].download()[0])
     tune2 = lantern_amd.autotune(ss, cams, params, fb, try_layouts=True)
-    assert tune2["candidates"] == 36 and tune2["layout"] in (0, 1)
+    assert tune2["candidates"] == 42 and tune2["layout"] in (0, 1)
     assert all(s.info()["layout"] == tune2["layout"] for s in ss)
     for f in fa + fb:
         f.close()

@@ -172,7 +172,7 @@ def test_trace_variants_give_identical_images(gpu_device):
     film = lantern_amd.Film(gpu_device, 128, 72)
     lantern_amd.render(gs, d.camera, dict(d.params), film)
     ref = film.download()
-    for variant in range(1, 7):
+    for variant in range(1, 8):
         for block in (64, 128, 256):
             film.reset()
             lantern_amd.render(gs, d.camera, dict(d.params, trace_variant=variant, block_size=block), film)
