@@ -643,14 +643,18 @@ int launch(hipStream_t st, const DScene &sc, const TraceBatch *tb, const Ray *ra
 int trace_default_block() { return 128; }
 
 /* Resident blocks per CU: limited by the LDS stack of the launched variant
- * (binary: LDS_STACK x 4 B, 8-wide: LDS8 x 8 B per lane, of 160 KB) and by
- * 32 waves per CU. */
+ * (binary: LDS_STACK x 4 B, 8-wide: LDS8 x 8 B per lane, of 160 KB), by the
+ * variant's occupancy target and by 32 waves per CU. */
 int trace_grid(int num_cus, int block, int variant) {
-    const int lds8 = (variant == 0 || variant == 4 || variant == 5) ? 16 : 8;
+    static const int lds8_of[kTraceVariants] = {16, 8, 8, 8, 16, 16, 8};
+    static const int waves_of[kTraceVariants] = {0, 0, 6, 8, 0, 0, 6};
+    const int lds8 = lds8_of[variant];
     const int lds_per_lane = std::max(LDS_STACK * 4, lds8 * 8);
+    const int wpb = std::max(1, block / 64);
     const int by_lds = (160 * 1024) / (lds_per_lane * block);
-    const int by_waves = std::max(1, 32 / (block / 64));
-    return std::max(1, num_cus) * std::max(1, std::min(by_lds, by_waves));
+    const int by_waves = std::max(1, 32 / wpb);
+    const int by_target = waves_of[variant] ? std::max(1, 4 * waves_of[variant] / wpb) : by_waves;
+    return std::max(1, num_cus) * std::max(1, std::min(std::min(by_lds, by_waves), by_target));
 }
 
 size_t trace_spill_bytes(int grid, int block) {
