@@ -163,6 +163,22 @@ def test_streams_give_identical_images(gpu_device):
     gs.close()
 
 
+def test_stream_parameters_are_checked(gpu_device):
+    """Out-of-range stream counts, traversal grid shares and kernel variants
+    are refused with an error before any launch; the film stays untouched."""
+    d = scenes.quad_scene(16, 16, 2)
+    gs = lantern_amd.Scene(gpu_device, d)
+    film = lantern_amd.Film(gpu_device, 16, 16)
+    for bad in (dict(streams=4), dict(trace_share=101), dict(trace_variant=8)):
+        with pytest.raises(lantern_amd.LanternError):
+            lantern_amd.render(gs, d.camera, dict(d.params, **bad), film)
+    assert int(film.download()[2].max()) == 0
+    st = lantern_amd.render(gs, d.camera, dict(d.params, streams=3, trace_share=1), film)
+    assert st["streams"] == 2   # one stream per sample at most
+    film.close()
+    gs.close()
+
+
 def test_trace_variants_give_identical_images(gpu_device):
     """Every traversal kernel variant (LDS stack depth, occupancy target) and
     block size visits each ray's nodes in the same order, so the films are
