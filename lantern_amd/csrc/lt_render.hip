@@ -401,12 +401,17 @@ void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, Nee
              uint32_t *nee_ignore, uint32_t *nee_cidx, uint32_t *nee_dest, Ctl *ctl) {
     __shared__ uint32_t s_order_all[(kShadeBlock / 64) * kShadeChunk];
     __shared__ uint8_t s_bin_all[(kShadeBlock / 64) * kShadeChunk];
+    /* per lane of the round: the next-event rays' origin and directions (and
+     * light geometry), kept here from the light sampling to the compacted
+     * writes at the end of the round instead of in registers */
+    __shared__ float4 s_nee_all[(kShadeBlock / 64) * 64 * 3];
     const uint32_t count = ctl->count[b];
     const int lane = threadIdx.x & 63;
     const float inf = __builtin_huge_valf();
     const unsigned long long lanes_below = (1ull << lane) - 1ull;
     uint32_t *s_order = s_order_all + (threadIdx.x >> 6) * kShadeChunk;
     uint8_t *s_bin = s_bin_all + (threadIdx.x >> 6) * kShadeChunk;
+    float4 *s_nee = s_nee_all + (threadIdx.x >> 6) * 64 * 3 + 3 * lane;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     const uint32_t rounds = max(1u, min((uint32_t)kShadeRounds, count / (nwaves * 64u)));
@@ -465,11 +470,8 @@ void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, Nee
             const bool hit = valid && s_bin[pos] != 0u;
             bool alive = false;
             bool nee1 = false, nee2 = false;
-            float tfar1 = 0.0f, tfar2 = 0.0f;
             uint32_t p = 0;
             V3 new_o = v3s(0.0f), new_d = v3s(0.0f);
-            V3 P = v3s(0.0f), wi1 = v3s(0.0f), wi2 = v3s(0.0f);
-            uint32_t ignore_geom = 0xFFFFFFFFu;
             uint32_t scene = 0;
             V3 C = v3s(0.0f);                        /* radiance so far */
             float4 nst0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);   /* next state of a live path */
@@ -587,7 +589,6 @@ void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, Nee
                         else it.tu = it.tv = 0.0f;
                         it.wo = -dir;
                         it.ioro = 0.0f;
-                        P = it.P;
                         if (ADJ && m.texture >= 0 && m.bsdf != BSDF_GGX) {
                             /* Lambert, mirror and dielectric: the next-event terms
                              * here and every later term carry this albedo once,
@@ -609,10 +610,12 @@ void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, Nee
                             do {
                                 li = rng.next_discrete(nl);
                             } while ((int)li == light);
-                            const DLight L = sc.lights[li];
+                            const DLight &L = sc.lights[li];
                             Interaction ic = it;
                             V3 f1 = v3s(0.0f), f2 = v3s(0.0f);
                             float p1 = 0.0f, p2 = 0.0f;
+                            V3 wi1 = v3s(0.0f), wi2 = v3s(0.0f);
+                            float tfar1 = 0.0f, tfar2 = 0.0f;
                             if ((supported_lobes(m.bsdf) & ~LOBE_SPECULAR) != 0) {
                                 /* AreaLight::SampleLi up to its cast (area_light.cpp:18-56) */
                                 float dx, dy;
@@ -668,7 +671,9 @@ void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, Nee
                                 nee.rec[3 * (size_t)q + 1] = make_float4(f1.x, f1.y, f1.z, p1);
                                 nee.rec[3 * (size_t)q + 2] = make_float4(f2.x, f2.y, f2.z, p2);
                                 nee_info = li | (nee1 ? 0x40000000u : 0u) | (nee2 ? 0x80000000u : 0u);
-                                ignore_geom = L.geom;
+                                s_nee[0] = make_float4(it.P.x, it.P.y, it.P.z, __uint_as_float(L.geom));
+                                s_nee[1] = make_float4(wi1.x, wi1.y, wi1.z, tfar1);
+                                s_nee[2] = make_float4(wi2.x, wi2.y, wi2.z, tfar2);
                             }
                         }
                         /* continuation (integrator.cpp:199-231) */
@@ -731,15 +736,18 @@ void k_shade(WaveArgs a, uint32_t b, Queue cur, const Hit *hits, Queue next, Nee
             if (nee1 || nee2) nee_dest[q] = alive ? slot0 + (uint32_t)__popcll(mask & below) : (0x80000000u | p);
             if (nee1 || nee2) {
                 uint32_t c = nslot0 + (uint32_t)(__popcll(nmask1 & below) + __popcll(nmask2 & below));
+                const float4 e0 = s_nee[0], e1 = s_nee[1], e2 = s_nee[2];
+                const V3 P = v3(e0.x, e0.y, e0.z);
+                const uint32_t ignore_geom = __float_as_uint(e0.w);
                 if (nee1) {
-                    store_ray(nee_rays + c, P, wi1, 0.001f, tfar1);
+                    store_ray(nee_rays + c, P, v3(e1.x, e1.y, e1.z), 0.001f, e1.w);
                     nee_ignore[c] = ignore_geom;
                     if (BATCH) a.batch.nee_sid[c] = scene;
                     nee_cidx[2 * (size_t)q] = c;
                     ++c;
                 }
                 if (nee2) {
-                    store_ray(nee_rays + c, P, wi2, 0.001f, tfar2);
+                    store_ray(nee_rays + c, P, v3(e2.x, e2.y, e2.z), 0.001f, e2.w);
                     nee_ignore[c] = ignore_geom;
                     if (BATCH) a.batch.nee_sid[c] = scene;
                     nee_cidx[2 * (size_t)q + 1] = c;
