@@ -31,7 +31,7 @@ RAY_BYTES = 48
 # commands and raw figures are in profiles/k_trace8_triangle_heavy.md.
 # Keyed by (workload, layout, width, height, spp).
 TRAFFIC = {
-    ("triangle_heavy", 0, 1280, 720, 64): round((342870.484375 * 2.0 + 161649.9640625 * 1.0) * 1024),
+    ("triangle_heavy", 0, 1280, 720, 64): round((343675.659375 * 2.0 + 161607.8328125 * 1.0) * 1024),
 }
 
 WORKLOADS = {
