@@ -294,6 +294,7 @@ int lt_device_destroy(lt_device *d) {
         (void)hipStreamDestroy(d->lane[k].stream);
     }
     if (d->pinned) (void)hipHostFree(d->pinned);
+    for (hipEvent_t e : d->ev_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(d->stream);
     delete d;
     return LT_OK;

@@ -958,26 +958,40 @@ int carve(lt::DevBuf &ws, size_t &ws_paths, size_t n, Workspace &w) {
 
 struct EventTimer {
     bool on;
+    std::vector<hipEvent_t> *pool;   /* the device's spare events, reused across calls */
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[6];
-    explicit EventTimer(bool enable) : on(enable) {}
+    EventTimer(bool enable, std::vector<hipEvent_t> *spare) : on(enable), pool(spare) {}
     ~EventTimer() {
         for (auto &v : ev)
             for (auto &e : v) {
-                (void)hipEventDestroy(e.first);
-                (void)hipEventDestroy(e.second);
+                pool->push_back(e.first);
+                pool->push_back(e.second);
             }
+    }
+    int take(hipEvent_t &e) {
+        if (!pool->empty()) {
+            e = pool->back();
+            pool->pop_back();
+            return 0;
+        }
+        LT_HIP(hipEventCreate(&e));
+        return 0;
     }
     int begin(int k, hipStream_t st) {
         if (!on) return 0;
-        hipEvent_t a, b;
-        LT_HIP(hipEventCreate(&a));
-        LT_HIP(hipEventCreate(&b));
+        hipEvent_t a = nullptr, b = nullptr;
+        int rc;
+        if ((rc = take(a))) return rc;
+        if ((rc = take(b))) {
+            pool->push_back(a);
+            return rc;
+        }
         ev[k].push_back({a, b});
         LT_HIP(hipEventRecord(a, st));
         return 0;
     }
     int end(int k, hipStream_t st) {
-        if (!on) return 0;
+        if (!on || ev[k].empty()) return 0;
         LT_HIP(hipEventRecord(ev[k].back().second, st));
         return 0;
     }
@@ -1085,7 +1099,7 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
     const bool graph = prm.use_graph != 0;
     const int sgrid = std::max(1, dev->num_cus) * 8;
     /* per-kernel event timing only outside graphs */
-    EventTimer tm(stats != nullptr && !graph);
+    EventTimer tm(stats != nullptr && !graph, &dev->ev_pool);
     uint32_t ext_launches = 0, sh_launches = 0;
     uint64_t samples = 0;
     /* per lane: the end of its last film accumulation, and of its last
