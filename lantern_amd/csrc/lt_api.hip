@@ -295,6 +295,7 @@ int lt_device_destroy(lt_device *d) {
     }
     if (d->pinned) (void)hipHostFree(d->pinned);
     for (hipEvent_t e : d->ev_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : d->sync_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(d->stream);
     delete d;
     return LT_OK;

@@ -155,7 +155,8 @@ struct lt_device {
     std::string graph_key;
     uint32_t graph_captures = 0;
     uint32_t *pinned = nullptr;   /* 64 words of pinned host memory: live counts read back during a render */
-    std::vector<hipEvent_t> ev_pool;   /* spare timing events of the render statistics */
+    std::vector<hipEvent_t> ev_pool;     /* spare timing events (render statistics, render time) */
+    std::vector<hipEvent_t> sync_pool;   /* spare events without timing (ordering between streams) */
 };
 
 struct lt_scene {

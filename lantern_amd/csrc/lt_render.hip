@@ -1105,16 +1105,28 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
     /* per lane: the end of its last film accumulation, and of its last
      * live-count copy */
     hipEvent_t acc_ev[kMaxLanes] = {}, chk_ev[kMaxLanes] = {};
+    /* events come from the device's pool and go back to it */
     struct EvGuard {
         hipEvent_t *e;
+        std::vector<hipEvent_t> *pool;
         ~EvGuard() {
             for (int k = 0; k < kMaxLanes; ++k)
-                if (e[k]) (void)hipEventDestroy(e[k]);
+                if (e[k]) pool->push_back(e[k]);
         }
-    } acc_guard{acc_ev}, chk_guard{chk_ev};
+    } acc_guard{acc_ev, &dev->sync_pool}, chk_guard{chk_ev, &dev->sync_pool};
+    /* timed: from ev_pool; ordering only (no timing): from sync_pool */
+    auto take_event = [&](hipEvent_t &e, bool timed) -> int {
+        std::vector<hipEvent_t> &pool = timed ? dev->ev_pool : dev->sync_pool;
+        if (!pool.empty()) {
+            e = pool.back();
+            pool.pop_back();
+            return LT_OK;
+        }
+        LT_HIP(hipEventCreateWithFlags(&e, timed ? hipEventDefault : hipEventDisableTiming));
+        return LT_OK;
+    };
     for (int k = 0; k < nlanes; ++k) {
-        LT_HIP(hipEventCreateWithFlags(&acc_ev[k], hipEventDisableTiming));
-        LT_HIP(hipEventCreateWithFlags(&chk_ev[k], hipEventDisableTiming));
+        if ((rc = take_event(acc_ev[k], false)) || (rc = take_event(chk_ev[k], false))) return rc;
     }
 
     /* The launch sequence of all waves. With early_exit, a group of waves
@@ -1248,9 +1260,8 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
     hipStream_t st = lanes[0].st;
     /* start and end of the whole sequence (timed) */
     hipEvent_t ends[kMaxLanes] = {};
-    EvGuard ends_guard{ends};
-    LT_HIP(hipEventCreate(&ends[0]));
-    LT_HIP(hipEventCreate(&ends[1]));
+    EvGuard ends_guard{ends, &dev->ev_pool};
+    if ((rc = take_event(ends[0], true)) || (rc = take_event(ends[1], true))) return rc;
     const hipEvent_t e0 = ends[0], e1 = ends[1];
     if (graph) {
         /* replay the cached graph when every kernel argument is the same */
@@ -1274,8 +1285,9 @@ int run_waves(lt_device *dev, WaveArgs a, const std::vector<WavePlan> &plan, con
             /* the other lanes join the capture through an event of the
              * first (fork) and are joined back before the capture ends */
             hipEvent_t fj[kMaxLanes] = {};
-            EvGuard fj_guard{fj};
-            for (int k = 0; k < nlanes; ++k) LT_HIP(hipEventCreateWithFlags(&fj[k], hipEventDisableTiming));
+            EvGuard fj_guard{fj, &dev->sync_pool};
+            for (int k = 0; k < nlanes; ++k)
+                if ((rc = take_event(fj[k], false))) return rc;
             LT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             rc = LT_OK;
             int forked = 1;
